@@ -1,0 +1,173 @@
+/*
+ * sinfvm_hip.h — C ABI of the MI355X (gfx950) shallow-water backend.
+ *
+ * This is the boundary a host language binds to (Julia `ccall`, Python
+ * `ctypes`).  Every entry point takes plain pointers and sizes.  The library
+ * owns all device memory behind an opaque handle; host pointers are borrowed
+ * for the duration of a call only.
+ *
+ * Array layout (all host buffers): column-major, x fastest, one plane per
+ * conserved variable — byte-identical to the reference's `Volume._data`
+ * (`zeros(realtype, nxT, nyT, nvar)`, src/backends/buffer.jl:29-31), so a
+ * Julia caller passes `pointer(vol._data)` without a transpose.  Variable 1
+ * holds the SURFACE ELEVATION w = h + B (the reference names it `:h`,
+ * src/equations/swe_2D.jl:67), variables 2 and 3 hold hu and hv.
+ * Bottom topography is the corner array of shape (nxT+1, nyT+1)
+ * (src/bottom_topography.jl:78-82), also x fastest.
+ *
+ * Threading: calls on one handle must not overlap.  Independent handles may
+ * be driven from different host threads (each handle owns its own stream).
+ */
+#ifndef SINFVM_HIP_H
+#define SINFVM_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes (mirror the reference's exception types) ------------ */
+enum sfv_status {
+    SFV_OK = 0,
+    SFV_ERR_DOMAIN = 1,       /* DomainError: array sizes (bottom_topography.jl:79-81,
+                                 source_term_infiltration.jl:61-65, source_term_rain.jl:36-38) */
+    SFV_ERR_ARGUMENT = 2,     /* ArgumentError: non-zero B without SourceTermBottom (equation.jl:74-84),
+                                 or an invalid enum / parameter */
+    SFV_ERR_ASSERTION = 3,    /* AssertionError: gc < 2 with linear reconstruction (linearreconstruction.jl:59) */
+    SFV_ERR_OUT_OF_MEMORY = 4,
+    SFV_ERR_NONFINITE_DT = 5, /* dt came out NaN or +-Inf (all-dry domain, simulator.jl:190-192) */
+    SFV_ERR_HIP = 6,          /* a HIP runtime call failed */
+    SFV_ERR_STATE = 7,        /* call made in the wrong order (e.g. step before set_state) */
+    SFV_ERR_BUDGET = 8        /* serving: request does not fit the HBM byte budget */
+};
+
+/* ---- configuration enums (mirror the reference's Julia types) ---------- */
+enum sfv_bc { SFV_BC_WALL = 0, SFV_BC_PERIODIC = 1, SFV_BC_NEUMANN = 2 };          /* grid.jl:21-28 */
+enum sfv_stepper { SFV_FORWARD_EULER = 1, SFV_RUNGE_KUTTA2 = 2 };                   /* timestepper/ */
+enum sfv_reconstruction { SFV_NO_RECONSTRUCTION = 0, SFV_LINEAR_RECONSTRUCTION = 1 };
+enum sfv_friction {                                                                 /* friction.jl:30-43 */
+    SFV_FRICTION_NONE = 0, SFV_FRICTION_BH2021 = 1, SFV_FRICTION_BSA2012 = 2, SFV_FRICTION_FCG2016 = 3
+};
+enum sfv_rain { SFV_RAIN_NONE = 0, SFV_RAIN_CONSTANT = 1, SFV_RAIN_TIME_TABLE = 2 }; /* source_term_rain.jl */
+enum sfv_infiltration {                                                             /* source_term_infiltration.jl */
+    SFV_INFILTRATION_NONE = 0, SFV_INFILTRATION_CONSTANT = 1, SFV_INFILTRATION_HORTON = 2
+};
+enum sfv_precision { SFV_F64 = 0, SFV_F32_STORAGE = 1 };
+enum sfv_layout { SFV_LAYOUT_FULL = 0, SFV_LAYOUT_INTERIOR = 1 };
+
+/*
+ * Everything the reference spreads over constructor keyword arguments:
+ *   CartesianGrid(nx, ny; gc, boundary, extent)          grid.jl:82-90
+ *   ShallowWaterEquations(B; ρ, g, depth_cutoff, desingularizing_kappa)  swe_2D.jl:28
+ *   LinearReconstruction(theta) / NoReconstruction()     linearreconstruction.jl:22-25
+ *   RungeKutta2() / ForwardEulerStepper(), Simulator(...; cfl)  simulator.jl:72
+ *   sources = [SourceTermBottom(), rain, infiltration]   system.jl:37
+ *   ImplicitFriction(; Cz, friction_function)             friction.jl:23-27
+ * dx, dy are grid.Δ as the reference computes them: (x1 - x0) / nx.
+ */
+typedef struct sfv_config {
+    int32_t nx, ny, gc;
+    double dx, dy;
+    int32_t bc;                    /* enum sfv_bc */
+    double g, rho, depth_cutoff, desingularizing_kappa;
+    int32_t reconstruction;        /* enum sfv_reconstruction */
+    double theta;
+    int32_t stepper;               /* enum sfv_stepper */
+    double cfl;
+    int32_t bottom_source;         /* 1 if SourceTermBottom is in the source list */
+    int32_t friction;              /* enum sfv_friction (ImplicitFriction), 0 = no implicit source */
+    double Cz;
+    int32_t rain;                  /* enum sfv_rain */
+    double rain_rate;              /* ConstantRain rate, in the reference's unit (divided by 3600 on use) */
+    int32_t rain_table_len;        /* TimeDependentRain: rates (reference unit) and start times */
+    const double* rain_table_rates;
+    const double* rain_table_times;
+    int32_t infiltration;          /* enum sfv_infiltration */
+    double infiltration_rate;      /* ConstantInfiltration rate (used as is) */
+    double horton_f0, horton_fc, horton_k;
+    double infiltration_factor_scalar;   /* HortonInfiltration factor when no field is given */
+    const double* infiltration_factor;   /* (nxT, nyT) field or NULL */
+    int32_t rain_before_infiltration;    /* 1: rain precedes infiltration in the source list */
+    int32_t precision;             /* enum sfv_precision */
+} sfv_config;
+
+/* Fill *cfg with the reference's defaults (gc=2, WallBC, g=9.81, ρ=1,
+ * depth_cutoff = desingularizing_kappa = 1e-5, θ=1.2, RK2, cfl=0.25, no sources). */
+void sfv_config_defaults(sfv_config* cfg, int32_t nx, int32_t ny, double dx, double dy);
+
+typedef struct sfv_handle sfv_handle;
+
+/* Lifecycle.  Replaces ConservedSystem(...) + Simulator(...) allocation
+ * (system.jl:37-53, simulator.jl:72-91).  Validates the configuration the way
+ * the reference constructors do; the topography is checked in
+ * sfv_set_topography and when stepping starts. */
+int sfv_create(const sfv_config* cfg, sfv_handle** out);
+void sfv_destroy(sfv_handle* h);
+/* Text of the last error on this handle ("" if none).  Valid until the next call. */
+const char* sfv_last_error(const sfv_handle* h);
+/* Text of the last error of sfv_create (thread-local), for a NULL handle. */
+const char* sfv_create_error(void);
+
+/* BottomTopography2D(B, backend, grid) (bottom_topography.jl:51-57): corner
+ * array (nbx, nby) must equal (nxT+1, nyT+1) else SFV_ERR_DOMAIN.  A non-zero
+ * B without bottom_source gives SFV_ERR_ARGUMENT (equation.jl:74-84). */
+int sfv_set_topography(sfv_handle* h, const double* B, int64_t nbx, int64_t nby);
+
+/* set_current_state! (simulator.jl:119-135).  SFV_LAYOUT_INTERIOR: U is
+ * (nx, ny, 3); the interior is written and the boundary condition applied,
+ * other ghost cells keep their values.  SFV_LAYOUT_FULL: U is (nxT, nyT, 3)
+ * and is copied verbatim (a checkpoint restore).  Element type is double for
+ * SFV_F64 and float for SFV_F32_STORAGE. */
+int sfv_set_state(sfv_handle* h, const void* U, int32_t layout);
+/* collect(current_state(sim)) / current_interior_state (simulator.jl:98-113). */
+int sfv_get_state(sfv_handle* h, void* U, int32_t layout);
+
+/* perform_step!(sim, max_dt) (simulator.jl:145-177): one full time step with
+ * sources evaluated at time t.  Writes the step's dt (current_timestep) to
+ * *dt_out.  Does NOT advance t — the caller does, as simulate_to_time does
+ * (simulator.jl:205). */
+int sfv_step(sfv_handle* h, double max_dt, double t, double* dt_out);
+
+/* simulate_to_time(sim, t_end; match_endtime, maximum_timestep) without a
+ * callback (simulator.jl:179-212), run device-resident: time and dt live in
+ * device memory and the host does not synchronise per step.  *t_io is the
+ * start time on entry and the end time on exit; at most max_steps steps are
+ * taken (max_steps <= 0: no limit).  maximum_timestep <= 0 means none. */
+int sfv_run(sfv_handle* h, double t_end, int32_t match_endtime, double maximum_timestep,
+            int64_t max_steps, double* t_io, int64_t* steps_out);
+
+/* nsteps calls of perform_step!(sim, max_dt) with t += dt after each
+ * (the fixed-step-count workload), device-resident.  *t_io in/out. */
+int sfv_advance(sfv_handle* h, int64_t nsteps, double max_dt, double* t_io);
+
+/* dt of the most recent step (current_timestep(sim), simulator.jl:141). */
+int sfv_current_timestep(sfv_handle* h, double* dt_out);
+
+/* Block until all work queued on the handle's stream has finished. */
+int sfv_synchronize(sfv_handle* h);
+
+/* Bytes of device memory the handle holds (for budgeting). */
+int64_t sfv_device_bytes(const sfv_handle* h);
+/* Bytes of device memory a handle with this configuration would hold. */
+int64_t sfv_device_bytes_for(const sfv_config* cfg);
+
+/* Kernel-timing hooks for measurement: launch only the named kernel of one
+ * RK2 step on the current state ("stage1", "stage2") n times and report the
+ * mean device time in milliseconds from HIP events on the handle's stream. */
+int sfv_time_kernel(sfv_handle* h, const char* which, int32_t n, double* ms_out);
+
+/* Number of visible HIP devices (0 when none).  Needs no handle. */
+int sfv_device_count(void);
+
+/* Select the fused stage kernels (1, the default for RungeKutta2 with a wall
+ * or periodic boundary) or the per-pass kernels (0) for this handle.  Both
+ * give identical results; the per-pass path is kept as the device-side
+ * reference and is the one NeumannBC and ForwardEuler use. */
+int sfv_set_fused(sfv_handle* h, int32_t fused);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SINFVM_HIP_H */

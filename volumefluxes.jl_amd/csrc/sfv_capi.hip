@@ -1,0 +1,422 @@
+// sfv_capi.hip — implementation of the C ABI in include/sinfvm_hip.h:
+// handle lifecycle, validation (mirroring the reference's exceptions), state
+// transfer, and the orchestration of one time step's kernels on the handle's
+// HIP stream.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/sinfvm_hip.h"
+#include "sfv_handle.hpp"
+
+using namespace sfv;
+
+namespace {
+
+thread_local std::string g_create_error;
+
+int fail(sfv_handle* h, int code, const std::string& msg) {
+    if (h) h->err = msg;
+    else g_create_error = msg;
+    return code;
+}
+
+#define HIP_TRY(h, expr)                                                                             \
+    do {                                                                                             \
+        hipError_t e_ = (expr);                                                                      \
+        if (e_ != hipSuccess)                                                                        \
+            return fail((h), SFV_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));        \
+    } while (0)
+
+// Replay bc.jl's per-ghost assignments (for g = 1..gc, low side then high
+// side) on an index map, so each ghost ends up naming the cell whose pre-BC
+// value it receives and how many times its normal momentum was negated.
+void ghost_maps(int bc, int gc, int N, int* lo, int* hi, int* flo, int* fhi) {
+    std::vector<int> cur(N), neg(N, 0);
+    for (int x = 0; x < N; ++x) cur[x] = x;
+    const int f = bc == SFV_BC_WALL ? 1 : 0;
+    for (int g = 1; g <= gc; ++g) {
+        int dl, sl, dr, sr;
+        if (bc == SFV_BC_WALL) {            // bc.jl:92-109
+            dl = g - 1; sl = 2 * gc - g; dr = N - gc + g - 1; sr = N - gc - g;
+        } else if (bc == SFV_BC_PERIODIC) { // bc.jl:31-39
+            dl = g - 1; sl = N + g - 2 * gc - 1; dr = N - (gc - g) - 1; sr = gc + g - 1;
+        } else {                            // bc.jl:51-59
+            dl = g - 1; sl = 2 * gc - g; dr = N - (gc - g) - 1; sr = N - g;
+        }
+        cur[dl] = cur[sl]; neg[dl] = neg[sl] ^ f;
+        cur[dr] = cur[sr]; neg[dr] = neg[sr] ^ f;
+    }
+    for (int g = 0; g < gc; ++g) {
+        lo[g] = cur[g]; flo[g] = neg[g];
+        hi[g] = cur[N - gc + g]; fhi[g] = neg[N - gc + g];
+    }
+}
+
+int validate(const sfv_config* c, std::string& why) {
+    if (!c) { why = "config is NULL"; return SFV_ERR_ARGUMENT; }
+    if (c->nx < 1 || c->ny < 1) { why = "nx and ny must be >= 1"; return SFV_ERR_DOMAIN; }
+    if (c->gc < 1 || c->gc > MAX_GC) { why = "gc must be in [1, 8]"; return SFV_ERR_DOMAIN; }
+    if (c->reconstruction == SFV_LINEAR_RECONSTRUCTION && !(c->gc > 1)) {
+        why = "LinearReconstruction needs grid.ghostcells > 1 (linearreconstruction.jl:59)";
+        return SFV_ERR_ASSERTION;
+    }
+    if (c->reconstruction != SFV_LINEAR_RECONSTRUCTION && c->reconstruction != SFV_NO_RECONSTRUCTION) {
+        why = "unknown reconstruction"; return SFV_ERR_ARGUMENT;
+    }
+    if (c->bc < SFV_BC_WALL || c->bc > SFV_BC_NEUMANN) { why = "unknown boundary condition"; return SFV_ERR_ARGUMENT; }
+    if (c->stepper != SFV_FORWARD_EULER && c->stepper != SFV_RUNGE_KUTTA2) { why = "unknown stepper"; return SFV_ERR_ARGUMENT; }
+    if (c->friction < SFV_FRICTION_NONE || c->friction > SFV_FRICTION_FCG2016) { why = "unknown friction law"; return SFV_ERR_ARGUMENT; }
+    if (c->rain < SFV_RAIN_NONE || c->rain > SFV_RAIN_TIME_TABLE) { why = "unknown rain kind"; return SFV_ERR_ARGUMENT; }
+    if (c->rain == SFV_RAIN_TIME_TABLE && (c->rain_table_len < 1 || !c->rain_table_rates || !c->rain_table_times)) {
+        why = "TimeDependentRain needs rates and times of equal, non-zero length (source_term_rain.jl:36-38)";
+        return SFV_ERR_DOMAIN;
+    }
+    if (c->infiltration < SFV_INFILTRATION_NONE || c->infiltration > SFV_INFILTRATION_HORTON) {
+        why = "unknown infiltration kind"; return SFV_ERR_ARGUMENT;
+    }
+    if (c->precision != SFV_F64 && c->precision != SFV_F32_STORAGE) { why = "unknown precision"; return SFV_ERR_ARGUMENT; }
+    return SFV_OK;
+}
+
+template <class S>
+int step_kernels(sfv_handle* h, int slot, int cur) {
+    const KParams& P = h->P;
+    hipStream_t st = h->stream;
+    S* U0 = (S*)h->U[cur];
+    S* U2 = (S*)h->U[cur ^ 1];
+    S* RHS = (S*)h->RHS;
+    if (h->cfg.stepper == SFV_FORWARD_EULER) {
+        HIP_TRY(h, launch_flux<S>(st, P, h->ctl, slot, U0, RHS, h->B, 1));
+        HIP_TRY(h, launch_finish<S>(st, P, h->ctl, slot, 1, U0, U0, RHS, U2, h->B));
+        HIP_TRY(h, launch_bc<S>(st, P, h->ctl, slot, U2, BC_LAST_STAGE));
+        return SFV_OK;
+    }
+    if (h->fused) {
+        HIP_TRY(h, launch_stage1<S>(st, P, h->ctl, slot, U0, RHS, h->B));
+        HIP_TRY(h, launch_stage2<S>(st, P, h->ctl, slot, U0, RHS, U2, h->B));
+        HIP_TRY(h, launch_bc<S>(st, P, h->ctl, slot, U2, BC_LAST_STAGE));
+        return SFV_OK;
+    }
+    S* U1 = (S*)h->U[2];
+    HIP_TRY(h, launch_flux<S>(st, P, h->ctl, slot, U0, RHS, h->B, 1));
+    HIP_TRY(h, launch_finish<S>(st, P, h->ctl, slot, 1, U0, U0, RHS, U1, h->B));
+    HIP_TRY(h, launch_bc<S>(st, P, h->ctl, slot, U1, BC_IN_STEP));
+    HIP_TRY(h, launch_flux<S>(st, P, h->ctl, slot, U1, RHS, h->B, 0));
+    HIP_TRY(h, launch_finish<S>(st, P, h->ctl, slot, 2, U0, U1, RHS, U2, h->B));
+    HIP_TRY(h, launch_bc<S>(st, P, h->ctl, slot, U2, BC_LAST_STAGE));
+    return SFV_OK;
+}
+
+int launch_steps(sfv_handle* h, int64_t nsteps) {
+    for (int64_t s = 0; s < nsteps; ++s) {
+        const int slot = (int)((h->step_parity + s) & 1);
+        const int cur = (int)((h->cur + s) & 1);
+        const int rc = h->cfg.precision == SFV_F64 ? step_kernels<double>(h, slot, cur)
+                                                   : step_kernels<float>(h, slot, cur);
+        if (rc) return rc;
+    }
+    return SFV_OK;
+}
+
+// Read back the step-control block and account for the steps that ran.
+int collect_ctl(sfv_handle* h) {
+    HIP_TRY(h, hipMemcpyAsync(h->ctl_host, h->ctl, sizeof(StepCtl), hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(h, hipStreamSynchronize(h->stream));
+    return SFV_OK;
+}
+
+int ensure_ready(sfv_handle* h) {
+    if (!h) return SFV_ERR_ARGUMENT;
+    if (!h->has_state) return fail(h, SFV_ERR_STATE, "set the state (set_current_state!) before stepping");
+    return SFV_OK;
+}
+
+// Run a prepared step loop of at most `nsteps` launched steps, in batches, and
+// finish when the device says the loop ended.
+int run_loop(sfv_handle* h, double t0, double endtime, int match_endtime, int has_max, double max_ts,
+             int64_t max_steps, int until_endtime, int64_t* steps_out, double* t_out) {
+    const int slot0 = (int)(h->step_parity & 1);
+    HIP_TRY(h, launch_prepare(h->stream, h->P, h->ctl, slot0, t0, endtime, match_endtime, has_max, max_ts,
+                              (long long)max_steps, until_endtime, h->cfg.precision));
+    int64_t launched = 0;
+    int64_t batch = until_endtime ? 8 : std::max<int64_t>(max_steps, 1);
+    const int64_t cap = 256;
+    for (;;) {
+        int64_t nb = batch;
+        if (max_steps > 0) nb = std::min<int64_t>(nb, max_steps - launched);
+        if (nb <= 0) break;
+        // steps launched past the end of the loop are no-ops on the device
+        const int64_t save_cur = h->cur, save_par = h->step_parity;
+        h->cur = (save_cur + launched) & 1;
+        h->step_parity = (save_par + launched) & 1;
+        const int rc = launch_steps(h, nb);
+        h->cur = save_cur;
+        h->step_parity = save_par;
+        if (rc) return rc;
+        launched += nb;
+        if (!until_endtime) break;
+        const int rc2 = collect_ctl(h);
+        if (rc2) return rc2;
+        if (!h->ctl_host->slot[(slot0 + launched) & 1].active) break;
+        batch = std::min<int64_t>(batch * 2, cap);
+    }
+    const int rc = collect_ctl(h);
+    if (rc) return rc;
+    const int64_t done = h->ctl_host->steps;
+    h->cur = (h->cur + done) & 1;
+    h->step_parity = (h->step_parity + done) & 1;
+    if (done > 0) { h->last_dt = h->ctl_host->dt; h->have_dt = true; }
+    if (steps_out) *steps_out = done;
+    if (t_out) *t_out = h->ctl_host->t;
+    if (h->ctl_host->nonfinite) return fail(h, SFV_ERR_NONFINITE_DT, "a time step produced a non-finite dt");
+    return SFV_OK;
+}
+
+}  // namespace
+
+void sfv_handle::ghost_maps_into(int bc, int gc, int N, int* lo, int* hi, int* flo, int* fhi) {
+    ghost_maps(bc, gc, N, lo, hi, flo, fhi);
+}
+
+extern "C" {
+
+void sfv_config_defaults(sfv_config* c, int32_t nx, int32_t ny, double dx, double dy) {
+    std::memset(c, 0, sizeof(*c));
+    c->nx = nx; c->ny = ny; c->gc = 2;
+    c->dx = dx; c->dy = dy;
+    c->bc = SFV_BC_WALL;
+    c->g = 9.81; c->rho = 1.0; c->depth_cutoff = 1e-5; c->desingularizing_kappa = 1e-5;
+    c->reconstruction = SFV_LINEAR_RECONSTRUCTION; c->theta = 1.2;
+    c->stepper = SFV_RUNGE_KUTTA2; c->cfl = 0.25;
+    c->Cz = 0.03 * 0.03;
+    c->horton_f0 = 1.977e-4; c->horton_fc = 3.272e-5; c->horton_k = 2.43e-3;
+    c->infiltration_factor_scalar = 1.0;
+    c->rain_before_infiltration = 1;
+    c->precision = SFV_F64;
+}
+
+const char* sfv_create_error(void) { return g_create_error.c_str(); }
+
+const char* sfv_last_error(const sfv_handle* h) { return h ? h->err.c_str() : g_create_error.c_str(); }
+
+int64_t sfv_device_bytes_for(const sfv_config* c) {
+    if (!c) return -1;
+    const int64_t nxT = c->nx + 2 * (int64_t)c->gc, nyT = c->ny + 2 * (int64_t)c->gc;
+    const int64_t elem = c->precision == SFV_F32_STORAGE ? 4 : 8;
+    const int64_t nstate = (c->stepper == SFV_RUNGE_KUTTA2) ? 3 : 2;   // ping-pong + (unfused) stage buffer
+    int64_t b = nstate * 3 * nxT * nyT * elem + 3 * nxT * nyT * elem + (nxT + 1) * (nyT + 1) * 8 + 4096;
+    if (c->infiltration == SFV_INFILTRATION_HORTON) b += nxT * nyT * 8;
+    return b;
+}
+
+int sfv_create(const sfv_config* cfg, sfv_handle** out) {
+    if (!out) return fail(nullptr, SFV_ERR_ARGUMENT, "out is NULL");
+    *out = nullptr;
+    std::string why;
+    const int vr = validate(cfg, why);
+    if (vr) return fail(nullptr, vr, why);
+    sfv_handle* h = new sfv_handle();
+    const int rc = h->init(*cfg, why);
+    if (rc) {
+        g_create_error = why;
+        delete h;
+        return rc;
+    }
+    *out = h;
+    return SFV_OK;
+}
+
+void sfv_destroy(sfv_handle* h) { delete h; }
+
+int sfv_set_topography(sfv_handle* h, const double* B, int64_t nbx, int64_t nby) {
+    if (!h || !B) return fail(h, SFV_ERR_ARGUMENT, "NULL argument");
+    const KParams& P = h->P;
+    if (nbx != P.nxT + 1 || nby != P.nyT + 1) {
+        char buf[160];
+        std::snprintf(buf, sizeof buf, "Bottom topography should be of size (%d, %d) but got (%lld, %lld)", P.nxT + 1,
+                      P.nyT + 1, (long long)nbx, (long long)nby);
+        return fail(h, SFV_ERR_DOMAIN, buf);
+    }
+    const size_t nb = (size_t)nbx * nby;
+    bool nonzero = false;
+    for (size_t k = 0; k < nb && !nonzero; ++k) nonzero = !(B[k] == 0.0);
+    if (nonzero && !h->cfg.bottom_source)
+        return fail(h, SFV_ERR_ARGUMENT,
+                    "Found non-zero bottom topography in equation, but no corresponding source term. "
+                    "Did you forget to add a source term to your system?");
+    HIP_TRY(h, hipSetDevice(h->device));
+    HIP_TRY(h, hipMemcpyAsync(h->B, B, nb * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    HIP_TRY(h, hipStreamSynchronize(h->stream));
+    return SFV_OK;
+}
+
+int sfv_set_state(sfv_handle* h, const void* U, int32_t layout) {
+    if (!h || !U) return fail(h, SFV_ERR_ARGUMENT, "NULL argument");
+    HIP_TRY(h, hipSetDevice(h->device));
+    const KParams& P = h->P;
+    const size_t es = h->elem;
+    void* dst = h->U[h->cur];
+    if (layout == SFV_LAYOUT_FULL) {
+        HIP_TRY(h, hipMemcpyAsync(dst, U, 3 * h->ncell * es, hipMemcpyHostToDevice, h->stream));
+    } else if (layout == SFV_LAYOUT_INTERIOR) {
+        for (int v = 0; v < 3; ++v) {
+            char* d = (char*)dst + ((size_t)v * h->ncell + (size_t)P.gc * P.nxT + P.gc) * es;
+            const char* s = (const char*)U + (size_t)v * P.nx * P.ny * es;
+            HIP_TRY(h, hipMemcpy2DAsync(d, (size_t)P.nxT * es, s, (size_t)P.nx * es, (size_t)P.nx * es, P.ny,
+                                        hipMemcpyHostToDevice, h->stream));
+        }
+        const int rc = h->cfg.precision == SFV_F64
+                           ? (int)launch_bc<double>(h->stream, P, h->ctl, 0, (double*)dst, BC_STANDALONE)
+                           : (int)launch_bc<float>(h->stream, P, h->ctl, 0, (float*)dst, BC_STANDALONE);
+        if (rc) return fail(h, SFV_ERR_HIP, "boundary-condition launch failed");
+    } else {
+        return fail(h, SFV_ERR_ARGUMENT, "unknown layout");
+    }
+    HIP_TRY(h, hipStreamSynchronize(h->stream));
+    h->has_state = true;
+    return SFV_OK;
+}
+
+int sfv_get_state(sfv_handle* h, void* U, int32_t layout) {
+    if (!h || !U) return fail(h, SFV_ERR_ARGUMENT, "NULL argument");
+    HIP_TRY(h, hipSetDevice(h->device));
+    const KParams& P = h->P;
+    const size_t es = h->elem;
+    const void* src = h->U[h->cur];
+    if (layout == SFV_LAYOUT_FULL) {
+        HIP_TRY(h, hipMemcpyAsync(U, src, 3 * h->ncell * es, hipMemcpyDeviceToHost, h->stream));
+    } else if (layout == SFV_LAYOUT_INTERIOR) {
+        for (int v = 0; v < 3; ++v) {
+            const char* s = (const char*)src + ((size_t)v * h->ncell + (size_t)P.gc * P.nxT + P.gc) * es;
+            char* d = (char*)U + (size_t)v * P.nx * P.ny * es;
+            HIP_TRY(h, hipMemcpy2DAsync(d, (size_t)P.nx * es, s, (size_t)P.nxT * es, (size_t)P.nx * es, P.ny,
+                                        hipMemcpyDeviceToHost, h->stream));
+        }
+    } else {
+        return fail(h, SFV_ERR_ARGUMENT, "unknown layout");
+    }
+    HIP_TRY(h, hipStreamSynchronize(h->stream));
+    return SFV_OK;
+}
+
+int sfv_step(sfv_handle* h, double max_dt, double t, double* dt_out) {
+    int rc = ensure_ready(h);
+    if (rc) return rc;
+    HIP_TRY(h, hipSetDevice(h->device));
+    int64_t steps = 0;
+    double tend = 0;
+    rc = run_loop(h, t, NAN, 0, 1, max_dt, 1, 0, &steps, &tend);
+    if (dt_out) *dt_out = h->last_dt;
+    return rc;
+}
+
+int sfv_run(sfv_handle* h, double t_end, int32_t match_endtime, double maximum_timestep, int64_t max_steps,
+            double* t_io, int64_t* steps_out) {
+    int rc = ensure_ready(h);
+    if (rc) return rc;
+    if (!t_io) return fail(h, SFV_ERR_ARGUMENT, "t_io is NULL");
+    HIP_TRY(h, hipSetDevice(h->device));
+    const int has_max = maximum_timestep > 0 ? 1 : 0;
+    return run_loop(h, *t_io, t_end, match_endtime ? 1 : 0, has_max, has_max ? maximum_timestep : 0.0,
+                    max_steps, 1, steps_out, t_io);
+}
+
+int sfv_advance(sfv_handle* h, int64_t nsteps, double max_dt, double* t_io) {
+    int rc = ensure_ready(h);
+    if (rc) return rc;
+    if (!t_io) return fail(h, SFV_ERR_ARGUMENT, "t_io is NULL");
+    if (nsteps <= 0) return SFV_OK;
+    HIP_TRY(h, hipSetDevice(h->device));
+    int64_t steps = 0;
+    return run_loop(h, *t_io, NAN, 0, 1, max_dt, nsteps, 0, &steps, t_io);
+}
+
+int sfv_current_timestep(sfv_handle* h, double* dt_out) {
+    if (!h || !dt_out) return fail(h, SFV_ERR_ARGUMENT, "NULL argument");
+    *dt_out = h->have_dt ? h->last_dt : 0.0;
+    return SFV_OK;
+}
+
+int sfv_synchronize(sfv_handle* h) {
+    if (!h) return SFV_ERR_ARGUMENT;
+    HIP_TRY(h, hipStreamSynchronize(h->stream));
+    return SFV_OK;
+}
+
+int64_t sfv_device_bytes(const sfv_handle* h) { return h ? h->device_bytes : 0; }
+
+int sfv_time_kernel(sfv_handle* h, const char* which, int32_t n, double* ms_out) {
+    int rc = ensure_ready(h);
+    if (rc) return rc;
+    if (!which || !ms_out || n < 1) return fail(h, SFV_ERR_ARGUMENT, "bad arguments");
+    if (h->cfg.stepper != SFV_RUNGE_KUTTA2) return fail(h, SFV_ERR_ARGUMENT, "kernel timing needs RungeKutta2");
+    const std::string w(which);
+    if (w != "stage1" && w != "stage2" && w != "bc" && w != "step")
+        return fail(h, SFV_ERR_ARGUMENT, "which must be stage1, stage2, bc or step");
+    HIP_TRY(h, hipSetDevice(h->device));
+    const int slot = (int)(h->step_parity & 1), cur = (int)(h->cur & 1);
+    // Time the kernels against a scratch output so the state is left as is.
+    HIP_TRY(h, launch_prepare(h->stream, h->P, h->ctl, slot, 0.0, NAN, 0, 0, 0.0, -1, 0, h->cfg.precision));
+    hipEvent_t e0, e1;
+    HIP_TRY(h, hipEventCreate(&e0));
+    HIP_TRY(h, hipEventCreate(&e1));
+    auto one = [&]() -> hipError_t {
+        const KParams& P = h->P;
+        if (h->cfg.precision == SFV_F64) {
+            double* U0 = (double*)h->U[cur];
+            double* Us = (double*)h->U[2];
+            double* R = (double*)h->RHS;
+            if (w == "stage1") return launch_stage1<double>(h->stream, P, h->ctl, slot, U0, R, h->B);
+            if (w == "stage2") return launch_stage2<double>(h->stream, P, h->ctl, slot, U0, R, Us, h->B);
+            if (w == "bc") return launch_bc<double>(h->stream, P, h->ctl, slot, Us, BC_IN_STEP);
+            hipError_t e = launch_stage1<double>(h->stream, P, h->ctl, slot, U0, R, h->B);
+            if (e == hipSuccess) e = launch_stage2<double>(h->stream, P, h->ctl, slot, U0, R, Us, h->B);
+            if (e == hipSuccess) e = launch_bc<double>(h->stream, P, h->ctl, slot, Us, BC_IN_STEP);
+            return e;
+        }
+        float* U0 = (float*)h->U[cur];
+        float* Us = (float*)h->U[2];
+        float* R = (float*)h->RHS;
+        if (w == "stage1") return launch_stage1<float>(h->stream, P, h->ctl, slot, U0, R, h->B);
+        if (w == "stage2") return launch_stage2<float>(h->stream, P, h->ctl, slot, U0, R, Us, h->B);
+        if (w == "bc") return launch_bc<float>(h->stream, P, h->ctl, slot, Us, BC_IN_STEP);
+        hipError_t e = launch_stage1<float>(h->stream, P, h->ctl, slot, U0, R, h->B);
+        if (e == hipSuccess) e = launch_stage2<float>(h->stream, P, h->ctl, slot, U0, R, Us, h->B);
+        if (e == hipSuccess) e = launch_bc<float>(h->stream, P, h->ctl, slot, Us, BC_IN_STEP);
+        return e;
+    };
+    HIP_TRY(h, one());   // warm-up
+    HIP_TRY(h, hipEventRecord(e0, h->stream));
+    for (int r = 0; r < n; ++r) HIP_TRY(h, one());
+    HIP_TRY(h, hipEventRecord(e1, h->stream));
+    HIP_TRY(h, hipEventSynchronize(e1));
+    float ms = 0;
+    HIP_TRY(h, hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *ms_out = (double)ms / n;
+    return SFV_OK;
+}
+
+int sfv_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+int sfv_set_fused(sfv_handle* h, int32_t fused) {
+    if (!h) return SFV_ERR_ARGUMENT;
+    if (fused && !h->fusable())
+        return fail(h, SFV_ERR_ARGUMENT,
+                    "fused kernels need RungeKutta2 and boundary-condition sources that are interior cells");
+    h->fused = fused ? 1 : 0;
+    return SFV_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,155 @@
+// sfv_handle.hpp — the object behind the C ABI's opaque handle: one grid's
+// device buffers, its stream and the kernel parameter block.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/sinfvm_hip.h"
+#include "swe_kernels.hpp"
+
+struct sfv_handle {
+    sfv_config cfg{};
+    std::vector<double> rain_rates, rain_times, factor;
+    sfv::KParams P{};
+    int device = 0;
+    hipStream_t stream = nullptr;
+    size_t elem = 8, ncell = 0;
+    // U[0], U[1]: ping-pong states (current = U[cur]); U[2]: the RK2 stage-1
+    // state of the unfused path and a scratch output for kernel timing.
+    void* U[3] = {nullptr, nullptr, nullptr};
+    void* RHS = nullptr;               // stage right-hand side, 3 planes
+    double* B = nullptr;               // corners (nxT+1, nyT+1)
+    double* d_rain = nullptr;          // rates then times
+    double* d_factor = nullptr;        // Horton factor field
+    sfv::StepCtl* ctl = nullptr;       // device step-control block
+    sfv::StepCtl* ctl_host = nullptr;  // pinned mirror
+    int64_t cur = 0;                   // index of the current state buffer
+    int64_t step_parity = 0;           // parity of the next step's slot
+    int64_t device_bytes = 0;
+    bool has_state = true;             // the zero-initialised state is a valid state
+    bool have_dt = false;
+    double last_dt = 0.0;
+    int fused = 1;
+    std::string err;
+
+    int alloc(void** p, size_t bytes, std::string& why) {
+        if (hipMalloc(p, bytes) != hipSuccess) {
+            why = "hipMalloc of " + std::to_string(bytes) + " bytes failed";
+            return SFV_ERR_OUT_OF_MEMORY;
+        }
+        device_bytes += (int64_t)bytes;
+        return SFV_OK;
+    }
+
+    int init(const sfv_config& c, std::string& why) {
+        cfg = c;
+        if (c.rain == SFV_RAIN_TIME_TABLE) {
+            rain_rates.assign(c.rain_table_rates, c.rain_table_rates + c.rain_table_len);
+            rain_times.assign(c.rain_table_times, c.rain_table_times + c.rain_table_len);
+        }
+        const int nxT = c.nx + 2 * c.gc, nyT = c.ny + 2 * c.gc;
+        ncell = (size_t)nxT * nyT;
+        if (c.infiltration == SFV_INFILTRATION_HORTON) {
+            if (c.infiltration_factor) factor.assign(c.infiltration_factor, c.infiltration_factor + ncell);
+            else factor.assign(ncell, c.infiltration_factor_scalar);
+        }
+        cfg.rain_table_rates = cfg.rain_table_times = nullptr;
+        cfg.infiltration_factor = nullptr;
+        elem = c.precision == SFV_F32_STORAGE ? 4 : 8;
+
+        if (hipGetDevice(&device) != hipSuccess) { why = "no HIP device"; return SFV_ERR_HIP; }
+        if (hipStreamCreateWithFlags(&stream, hipStreamNonBlocking) != hipSuccess) {
+            why = "hipStreamCreate failed"; return SFV_ERR_HIP;
+        }
+        int rc;
+        const int nstate = c.stepper == SFV_RUNGE_KUTTA2 ? 3 : 2;
+        for (int s = 0; s < nstate; ++s)
+            if ((rc = alloc(&U[s], 3 * ncell * elem, why))) return rc;
+        if ((rc = alloc(&RHS, 3 * ncell * elem, why))) return rc;
+        if ((rc = alloc((void**)&B, (size_t)(nxT + 1) * (nyT + 1) * sizeof(double), why))) return rc;
+        if ((rc = alloc((void**)&ctl, sizeof(sfv::StepCtl), why))) return rc;
+        if (hipHostMalloc((void**)&ctl_host, sizeof(sfv::StepCtl), hipHostMallocDefault) != hipSuccess) {
+            why = "hipHostMalloc failed"; return SFV_ERR_OUT_OF_MEMORY;
+        }
+        std::memset(ctl_host, 0, sizeof(sfv::StepCtl));
+        for (int s = 0; s < nstate; ++s) (void)hipMemsetAsync(U[s], 0, 3 * ncell * elem, stream);
+        (void)hipMemsetAsync(RHS, 0, 3 * ncell * elem, stream);
+        (void)hipMemsetAsync(B, 0, (size_t)(nxT + 1) * (nyT + 1) * sizeof(double), stream);
+        (void)hipMemsetAsync(ctl, 0, sizeof(sfv::StepCtl), stream);
+        if (!rain_rates.empty()) {
+            if ((rc = alloc((void**)&d_rain, 2 * rain_rates.size() * sizeof(double), why))) return rc;
+            (void)hipMemcpyAsync(d_rain, rain_rates.data(), rain_rates.size() * sizeof(double), hipMemcpyHostToDevice, stream);
+            (void)hipMemcpyAsync(d_rain + rain_rates.size(), rain_times.data(), rain_times.size() * sizeof(double),
+                           hipMemcpyHostToDevice, stream);
+        }
+        if (!factor.empty()) {
+            if ((rc = alloc((void**)&d_factor, ncell * sizeof(double), why))) return rc;
+            (void)hipMemcpyAsync(d_factor, factor.data(), ncell * sizeof(double), hipMemcpyHostToDevice, stream);
+        }
+        if (hipStreamSynchronize(stream) != hipSuccess) { why = "initialisation failed"; return SFV_ERR_HIP; }
+        build_params();
+        fused = fusable() ? 1 : 0;
+        return SFV_OK;
+    }
+
+    void build_params() {
+        const sfv_config& c = cfg;
+        sfv::KParams& p = P;
+        std::memset(&p, 0, sizeof(p));
+        p.nx = c.nx; p.ny = c.ny; p.gc = c.gc;
+        p.nxT = c.nx + 2 * c.gc; p.nyT = c.ny + 2 * c.gc;
+        p.dx = c.dx; p.dy = c.dy;
+        p.inv_dx = 1 / c.dx; p.inv_dy = 1 / c.dy;
+        p.phys.g = c.g; p.phys.rho = c.rho; p.phys.cutoff = c.depth_cutoff; p.phys.kappa = c.desingularizing_kappa;
+        p.theta = c.theta; p.cfl = c.cfl; p.Cz = c.Cz;
+        p.rec = c.reconstruction == SFV_LINEAR_RECONSTRUCTION ? 1 : 0;
+        p.friction = c.friction;
+        p.bottom = c.bottom_source ? 1 : 0;
+        p.rain_kind = c.rain;
+        p.infil_kind = c.infiltration;
+        p.rain_first = c.rain_before_infiltration ? 1 : 0;
+        p.stepper = c.stepper;
+        p.rain_n = (int)rain_rates.size();
+        p.rain_rates = d_rain;
+        p.rain_times = d_rain ? d_rain + rain_rates.size() : nullptr;
+        p.rain_const = c.rain_rate / 3600;
+        p.infil_rate = c.infiltration_rate;
+        p.f0 = c.horton_f0; p.fc = c.horton_fc; p.hk = c.horton_k;
+        p.factor = d_factor;
+        p.bc = c.bc;
+        ghost_maps_into(c.bc, c.gc, p.nxT, p.src_lo_x, p.src_hi_x, p.flip_lo_x, p.flip_hi_x);
+        ghost_maps_into(c.bc, c.gc, p.nyT, p.src_lo_y, p.src_hi_y, p.flip_lo_y, p.flip_hi_y);
+    }
+
+    static void ghost_maps_into(int bc, int gc, int N, int* lo, int* hi, int* flo, int* fhi);
+
+    // The fused stage kernels rebuild stage-1 ghost cells from their
+    // boundary-condition sources, which must then be interior cells.
+    bool fusable() const {
+        if (cfg.stepper != SFV_RUNGE_KUTTA2 || P.gc < 2) return false;
+        for (int g = 0; g < P.gc; ++g) {
+            const int sx[2] = {P.src_lo_x[g], P.src_hi_x[g]}, sy[2] = {P.src_lo_y[g], P.src_hi_y[g]};
+            for (int k = 0; k < 2; ++k) {
+                if (sx[k] < P.gc || sx[k] >= P.nxT - P.gc) return false;
+                if (sy[k] < P.gc || sy[k] >= P.nyT - P.gc) return false;
+            }
+        }
+        return true;
+    }
+
+    ~sfv_handle() {
+        if (stream) (void)hipStreamSynchronize(stream);
+        for (void* p : U) if (p) (void)hipFree(p);
+        if (RHS) (void)hipFree(RHS);
+        if (B) (void)hipFree(B);
+        if (d_rain) (void)hipFree(d_rain);
+        if (d_factor) (void)hipFree(d_factor);
+        if (ctl) (void)hipFree(ctl);
+        if (ctl_host) (void)hipHostFree(ctl_host);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};

@@ -1,0 +1,126 @@
+// swe_common.hpp — kernel parameter block, device step-control block and
+// indexing shared by the gfx950 kernels and the host side of the library.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "swe_physics.hpp"
+
+namespace sfv {
+
+constexpr int MAX_GC = 8;
+
+// Everything a kernel needs that is fixed for the life of a handle.
+struct KParams {
+    int nx, ny, gc, nxT, nyT;
+    double dx, dy, inv_dx, inv_dy;  // inv_d is 1/Δ as numericalflux.jl:27 computes it
+    PhysParams phys;
+    double theta, cfl, Cz;
+    int rec;        // 0 none, 1 linear
+    int friction;   // 0 none, 1 bh2021, 2 bsa2012, 3 fcg2016
+    int bottom;     // SourceTermBottom present
+    int rain_kind;  // 0 none, 1 constant, 2 table
+    int infil_kind; // 0 none, 1 constant, 2 horton
+    int rain_first;
+    int stepper;    // 1 FE, 2 RK2
+    int rain_n;
+    const double* rain_rates;  // device (reference unit)
+    const double* rain_times;  // device
+    double rain_const;         // ConstantRain rate / 3600
+    double infil_rate, f0, fc, hk;
+    const double* factor;      // device (nxT*nyT) Horton factor field
+    // Ghost-cell source maps for the boundary condition, by side: low-side
+    // ghost g is cell g, high-side ghost g is cell N-gc+g (N = nxT or nyT);
+    // it takes the pre-BC value of cell src (same row/column), with the
+    // normal momentum negated if flip is set.  Built on the host by
+    // replaying bc.jl's per-ghost assignments in order, so order-dependent
+    // maps (NeumannBC's high side, grids narrower than gc) are exact.
+    int bc;
+    int src_lo_x[MAX_GC], src_hi_x[MAX_GC], src_lo_y[MAX_GC], src_hi_y[MAX_GC];
+    int flip_lo_x[MAX_GC], flip_hi_x[MAX_GC], flip_lo_y[MAX_GC], flip_hi_y[MAX_GC];
+};
+
+// Per-step flags live in two slots indexed by step parity, so the kernel that
+// finishes step n can arm step n+1 while no kernel of step n reads that slot.
+struct StepSlot {
+    int32_t active;
+    int32_t pad;
+    unsigned long long wsmax[2];   // bit patterns of the non-negative max speeds (x, y)
+};
+
+struct StepCtl {
+    StepSlot slot[2];
+    double t;             // current time (realtype value)
+    double dt;            // current_timestep (realtype value)
+    double dt_stage1;     // Float64 dt of stage 1 (== dt for Float64)
+    double endtime;
+    double max_timestep;
+    int32_t match_endtime;
+    int32_t has_max_timestep;
+    int64_t steps;        // steps completed in this call
+    int64_t max_steps;    // <= 0: unlimited
+    double rain_now;      // compute_rain(rain, t)
+    double infil_tfac;    // fc + (f0 - fc)*exp(-k t) for Horton, the rate for constant
+    int32_t nonfinite;    // a step produced a NaN/Inf dt
+    int32_t precision;
+};
+
+__device__ __forceinline__ size_t cid(const KParams& P, int i, int j) { return (size_t)i + (size_t)P.nxT * j; }
+
+__device__ __forceinline__ double Bcorner(const KParams& P, const double* B, int i, int j) {
+    return B[(size_t)i + (size_t)(P.nxT + 1) * j];
+}
+// bottom_topography.jl:97-101
+__device__ __forceinline__ double B_cell(const KParams& P, const double* B, int i, int j) {
+    return 0.25 * (Bcorner(P, B, i, j) + Bcorner(P, B, i + 1, j) + Bcorner(P, B, i, j + 1) + Bcorner(P, B, i + 1, j + 1));
+}
+__device__ __forceinline__ double B_face_left(const KParams& P, const double* B, int i, int j, int d) {
+    return d == 0 ? 0.5 * (Bcorner(P, B, i, j) + Bcorner(P, B, i, j + 1))
+                  : 0.5 * (Bcorner(P, B, i, j) + Bcorner(P, B, i + 1, j));
+}
+__device__ __forceinline__ double B_face_right(const KParams& P, const double* B, int i, int j, int d) {
+    return d == 0 ? 0.5 * (Bcorner(P, B, i + 1, j) + Bcorner(P, B, i + 1, j + 1))
+                  : 0.5 * (Bcorner(P, B, i, j + 1) + Bcorner(P, B, i + 1, j + 1));
+}
+
+// source_term_rain.jl:45-54 scan of a TimeDependentRain table.
+__device__ inline double compute_rain(const KParams& P, double t) {
+    if (P.rain_kind == 1) return P.rain_const;
+    if (P.rain_kind == 2) {
+        int index = 0;
+        while (index + 1 < P.rain_n) {
+            if (t < P.rain_times[index + 1]) break;
+            index += 1;
+        }
+        return P.rain_rates[index] / 3600;
+    }
+    return 0.0;
+}
+
+// Non-directional sources on the w component, in source-list order
+// (system.jl:72-74): rain adds rate(t), infiltration subtracts its rate.
+template <class S>
+__device__ __forceinline__ S apply_sources(const KParams& P, const StepCtl* ctl, S w, size_t k) {
+    auto infil = [&](S x) -> S {
+        if (P.infil_kind == 2) return (S)(x - P.factor[k] * ctl->infil_tfac);
+        if (P.infil_kind == 1) return (S)(x - ctl->infil_tfac);
+        return x;
+    };
+    auto rain = [&](S x) -> S { return P.rain_kind ? (S)(x + ctl->rain_now) : x; };
+    return P.rain_first ? infil(rain(w)) : rain(infil(w));
+}
+
+// timestep_computer (simulator.jl:147-150) with the max_dt rule of
+// simulate_to_time (simulator.jl:200-203).
+template <class S>
+__device__ __forceinline__ double stage1_dt(const KParams& P, const StepCtl* ctl, const StepSlot& sl) {
+    double max_dt = ctl->match_endtime ? ctl->endtime - (S)ctl->t : __builtin_inf();
+    if (ctl->has_max_timestep) max_dt = jl_min(max_dt, ctl->max_timestep);
+    const double ax = __longlong_as_double((long long)sl.wsmax[0]);
+    const double ay = __longlong_as_double((long long)sl.wsmax[1]);
+    const S cfl = (S)P.cfl;
+    return jl_min(cfl * jl_min(P.dx / (S)ax, P.dy / (S)ay), max_dt);
+}
+
+}  // namespace sfv

@@ -1,0 +1,54 @@
+// swe_kernels.hpp — declarations of the gfx950 kernels of one time step.
+#pragma once
+
+#include "swe_common.hpp"
+
+namespace sfv {
+
+// ---- unfused per-pass kernels (swe_kernels_unfused.hip)
+template <class S>
+__global__ void k_flux(KParams P, StepCtl* ctl, int slot, const S* X, S* RHS, const double* B, int record);
+template <class S>
+__global__ void k_finish(KParams P, StepCtl* ctl, int slot, int stage, const S* U0, const S* X, const S* RHS, S* Out,
+                         const double* B);
+
+// ---- boundary condition and step control (swe_kernels_ctl.hip)
+template <class S>
+__global__ void k_bc(KParams P, StepCtl* ctl, int slot, S* U, int mode);
+__global__ void k_prepare(KParams P, StepCtl* ctl, int slot, double t, double endtime, int match_endtime,
+                          int has_max_timestep, double max_timestep, long long max_steps, int until_endtime,
+                          int precision);
+
+// ---- fused stage kernels (swe_kernels_fused.hip)
+// Tile of interior cells per block and threads per block.
+constexpr int FT_X = 64;
+constexpr int FT_Y = 8;
+constexpr int F_THREADS = 256;
+template <class S>
+__global__ void k_stage1(KParams P, StepCtl* ctl, int slot, const S* U0, S* RHS, const double* B);
+template <class S>
+__global__ void k_stage2(KParams P, StepCtl* ctl, int slot, const S* U0, const S* RHS1, S* U2, const double* B);
+
+// ---- host launchers (one per kernel, defined beside it); each returns the
+// launch status.  BC modes: 0 inside a step, 1 last stage of a step (closes
+// the step), 2 standalone (set_current_state!, no step-control check).
+enum { BC_IN_STEP = 0, BC_LAST_STAGE = 1, BC_STANDALONE = 2 };
+template <class S>
+hipError_t launch_flux(hipStream_t st, const KParams& P, StepCtl* ctl, int slot, const S* X, S* RHS, const double* B,
+                       int record);
+template <class S>
+hipError_t launch_finish(hipStream_t st, const KParams& P, StepCtl* ctl, int slot, int stage, const S* U0, const S* X,
+                         const S* RHS, S* Out, const double* B);
+template <class S>
+hipError_t launch_bc(hipStream_t st, const KParams& P, StepCtl* ctl, int slot, S* U, int mode);
+hipError_t launch_prepare(hipStream_t st, const KParams& P, StepCtl* ctl, int slot, double t, double endtime,
+                          int match_endtime, int has_max_timestep, double max_timestep, long long max_steps,
+                          int until_endtime, int precision);
+template <class S>
+hipError_t launch_stage1(hipStream_t st, const KParams& P, StepCtl* ctl, int slot, const S* U0, S* RHS,
+                         const double* B);
+template <class S>
+hipError_t launch_stage2(hipStream_t st, const KParams& P, StepCtl* ctl, int slot, const S* U0, const S* RHS1, S* U2,
+                         const double* B);
+
+}  // namespace sfv

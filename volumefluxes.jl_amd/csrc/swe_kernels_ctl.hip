@@ -1,0 +1,122 @@
+// swe_kernels_ctl.hip — ghost-cell boundary condition and the device-side
+// step control (time, dt, per-step flags) that lets a run advance without a
+// host synchronisation per step.
+#include "swe_kernels.hpp"
+
+namespace sfv {
+
+__device__ void set_rates(const KParams& P, StepCtl* c, double t) {
+    c->rain_now = compute_rain(P, t);
+    if (P.infil_kind == 2) c->infil_tfac = P.fc + (P.f0 - P.fc) * exp(-P.hk * t);
+    else if (P.infil_kind == 1) c->infil_tfac = P.infil_rate;
+    else c->infil_tfac = 0.0;
+}
+
+// update_bc! (bc.jl:29-110): X ghosts of every interior row, then Y ghosts of
+// every interior column; corners are never written.  One thread per row or
+// column reads all source cells first, then writes the ghosts, which gives
+// the reference's sequential per-line semantics (KParams::src_* maps).
+// With last_stage set, thread 0 also closes the step: t += dt
+// (simulator.jl:205), and arms the next step's slot.
+template <class S>
+__global__ void __launch_bounds__(256) k_bc(KParams P, StepCtl* ctl, int slot, S* __restrict__ U, int mode) {
+    const int last_stage = mode == BC_LAST_STAGE;
+    if (mode != BC_STANDALONE && !ctl->slot[slot].active) {
+        // An inactive step passes inactivity on to the next slot, so steps
+        // launched past the end of a run stay no-ops.
+        if (last_stage && blockIdx.x == 0 && threadIdx.x == 0) ctl->slot[1 - slot].active = 0;
+        return;
+    }
+    const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t n = (size_t)P.nxT * P.nyT;
+    const int gc = P.gc;
+    if (tid < P.ny) {
+        const int j = gc + tid;
+        S lo[MAX_GC][3], hi[MAX_GC][3];
+        for (int g = 0; g < gc; ++g)
+            for (int v = 0; v < 3; ++v) {
+                lo[g][v] = U[v * n + cid(P, P.src_lo_x[g], j)];
+                hi[g][v] = U[v * n + cid(P, P.src_hi_x[g], j)];
+            }
+        for (int g = 0; g < gc; ++g)
+            for (int v = 0; v < 3; ++v) {
+                const bool nl = P.flip_lo_x[g] && v == 1, nh = P.flip_hi_x[g] && v == 1;
+                U[v * n + cid(P, g, j)] = nl ? (S)(-lo[g][v]) : lo[g][v];
+                U[v * n + cid(P, P.nxT - gc + g, j)] = nh ? (S)(-hi[g][v]) : hi[g][v];
+            }
+    } else if (tid < P.ny + P.nx) {
+        const int i = gc + (tid - P.ny);
+        S lo[MAX_GC][3], hi[MAX_GC][3];
+        for (int g = 0; g < gc; ++g)
+            for (int v = 0; v < 3; ++v) {
+                lo[g][v] = U[v * n + cid(P, i, P.src_lo_y[g])];
+                hi[g][v] = U[v * n + cid(P, i, P.src_hi_y[g])];
+            }
+        for (int g = 0; g < gc; ++g)
+            for (int v = 0; v < 3; ++v) {
+                const bool nl = P.flip_lo_y[g] && v == 2, nh = P.flip_hi_y[g] && v == 2;
+                U[v * n + cid(P, i, g)] = nl ? (S)(-lo[g][v]) : lo[g][v];
+                U[v * n + cid(P, i, P.nyT - gc + g)] = nh ? (S)(-hi[g][v]) : hi[g][v];
+            }
+    }
+    if (last_stage && tid == 0) {
+        StepCtl* c = ctl;
+        const S dt = (S)c->dt;
+        const S t = (S)((S)c->t + dt);
+        c->t = (double)t;
+        c->steps += 1;
+        if (!__builtin_isfinite((double)dt)) c->nonfinite = 1;
+        bool act = c->max_steps <= 0 || c->steps < c->max_steps;
+        if (c->endtime == c->endtime) act = act && (t < c->endtime);   // NaN endtime: fixed step count
+        StepSlot& nx = c->slot[1 - slot];
+        nx.wsmax[0] = 0ull;
+        nx.wsmax[1] = 0ull;
+        nx.active = act ? 1 : 0;
+        set_rates(P, c, (double)t);
+    }
+}
+
+// Initialise the step-control block for a host call.  until_endtime = 0 runs
+// a fixed number of steps (endtime is then NaN and never ends the loop).
+__global__ void k_prepare(KParams P, StepCtl* ctl, int slot, double t, double endtime, int match_endtime,
+                          int has_max_timestep, double max_timestep, long long max_steps, int until_endtime,
+                          int precision) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    StepCtl* c = ctl;
+    c->t = t;
+    c->endtime = until_endtime ? endtime : __builtin_nan("");
+    c->match_endtime = match_endtime;
+    c->has_max_timestep = has_max_timestep;
+    c->max_timestep = max_timestep;
+    c->steps = 0;
+    c->max_steps = max_steps;
+    c->nonfinite = 0;
+    c->precision = precision;
+    bool act = true;
+    if (until_endtime) act = t < endtime;
+    for (int s = 0; s < 2; ++s) {
+        c->slot[s].wsmax[0] = 0ull;
+        c->slot[s].wsmax[1] = 0ull;
+        c->slot[s].active = 0;
+    }
+    c->slot[slot].active = act ? 1 : 0;
+    set_rates(P, c, t);
+}
+
+template <class S>
+hipError_t launch_bc(hipStream_t st, const KParams& P, StepCtl* ctl, int slot, S* U, int mode) {
+    const int nthreads = P.nx + P.ny;
+    hipLaunchKernelGGL(k_bc<S>, dim3((nthreads + 255) / 256), dim3(256), 0, st, P, ctl, slot, U, mode);
+    return hipPeekAtLastError();
+}
+hipError_t launch_prepare(hipStream_t st, const KParams& P, StepCtl* ctl, int slot, double t, double endtime,
+                          int match_endtime, int has_max_timestep, double max_timestep, long long max_steps,
+                          int until_endtime, int precision) {
+    hipLaunchKernelGGL(k_prepare, dim3(1), dim3(64), 0, st, P, ctl, slot, t, endtime, match_endtime, has_max_timestep,
+                       max_timestep, max_steps, until_endtime, precision);
+    return hipPeekAtLastError();
+}
+template hipError_t launch_bc<double>(hipStream_t, const KParams&, StepCtl*, int, double*, int);
+template hipError_t launch_bc<float>(hipStream_t, const KParams&, StepCtl*, int, float*, int);
+
+}  // namespace sfv
