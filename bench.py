@@ -1,0 +1,215 @@
+#!/usr/bin/env python3
+"""Benchmark: 2D shallow-water RK2 step on one MI355X (gfx950).
+
+Workload (the single-GPU terrain run): 2048 x 2048 cells at dx = dy = 1 m,
+bottom = 1e-3 slope + 32 seeded Gaussian bumps (amplitude U[0.5, 5] m, width
+U[20, 200] m), initial surface w = max(B_cell, 2 m) so cells with B > 2 m
+start dry, Manning n = 0.03 (bh2021 law, Cz = n^2), uniform rain 1e-5 m/s
+(ConstantRain(0.036) m/h), wall boundaries, desingularisation kappa = 1e-4,
+depth cutoff 1e-5, RungeKutta2, CFL 0.25.  A "step" is one full RK2 step of
+the whole grid; the timed region runs --steps steps device-resident with the
+state already in HBM.
+
+Prints one JSON line: throughput (cell-updates/s), the roofline of the
+dominant kernel from HIP-event timings on the kernel's own stream, and the
+CPU oracle timed on this host's cores on a bounded sample of the same run.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import statistics
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "volumefluxes.jl_amd"))
+
+import ctypes  # noqa: E402
+
+import numpy as np  # noqa: E402
+
+import volumefluxes as VF  # noqa: E402
+from volumefluxes import _capi as C  # noqa: E402
+from volumefluxes.scenarios import terrain_initial_state, terrain_topography  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0                 # MI355X HBM3E, 8 TB/s
+BYTES_PER_CELL_STEP = {"f64": 136, "f32": 80}   # compulsory HBM bytes per RK2 cell-update (B stays f64)
+BYTES_PER_CELL_STAGE = {"f64": {"stage1": 56, "stage2": 80}, "f32": {"stage1": 36, "stage2": 44}}
+METRIC = "cell-updates/s + HBM GB/s % peak (2D SWE RK2 FP64); ensemble p50/p99 vs plain"
+
+
+def build_scenario(n: int, precision: str, seed: int):
+    grid = VF.CartesianGrid(n, n, gc=2, boundary=VF.WallBC(), extent=[[0.0, float(n)], [0.0, float(n)]])
+    B = terrain_topography(grid, seed=seed, n_bumps=32, slope=1e-3, amp=(0.5, 5.0), width=(20.0, 200.0))
+    eq = VF.ShallowWaterEquations(VF.BottomTopography2D(B, None, grid), desingularizing_kappa=1e-4)
+    manning_n = 0.03
+    system = VF.ConservedSystem(None, VF.LinearReconstruction(), VF.CentralUpwind(eq), eq, grid,
+                                [VF.SourceTermBottom(), VF.ConstantRain(1e-5 * 3600)],
+                                VF.ImplicitFriction(Cz=manning_n * manning_n))
+    realtype = np.float64 if precision == "f64" else np.float32
+    U0 = terrain_initial_state(grid, B, 2.0)
+    return grid, system, realtype, U0, B
+
+
+def read_traffic(kernel: str, workload: str):
+    """Measured HBM bytes per launch of `kernel` from the committed rocprofv3
+    PMC summary (FETCH_SIZE, WRITE_SIZE, calibrated against a copy kernel)."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{workload}.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        parts = line.split()
+        if parts and parts[0] == kernel:
+            kv = dict(p.split("=") for p in parts[1:] if "=" in p)
+            if "bytes_per_launch" in kv:
+                return float(kv["bytes_per_launch"])
+    return None
+
+
+def cpu_baseline(grid, system, realtype, U0, B, steps_per_run: int, runs: int, threads: int):
+    """The CPU oracle (a loop-faithful restatement of the reference CPU path)
+    on this host: 1 warm-up step, then the median of `runs` timed runs of
+    `steps_per_run` steps, OpenMP over grid rows on `threads` threads."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O  # the checker / baseline, never the measured product
+    O.set_threads(threads)
+    cfg = VF.build_config(system, VF.RungeKutta2(), grid, 0.25, C.SFV_F64 if realtype == np.float64 else 1)
+    o = O.Oracle2D(cfg, B)
+    o.set_interior(U0)
+    t = 0.0
+    t += o.step(float("inf"), t)
+    first = o.state()
+    times = []
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        for _ in range(steps_per_run):
+            t += o.step(float("inf"), t)
+        times.append(time.perf_counter() - t0)
+    med = statistics.median(times)
+    return first, grid.nx * grid.ny * steps_per_run / med, med
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--size", type=int, default=2048)
+    ap.add_argument("--precision", choices=["f64", "f32"], default="f64")
+    ap.add_argument("--seed", type=int, default=2048)
+    ap.add_argument("--kernels", choices=["march", "tiled", "per-pass"], default="march",
+                    help="kernel variant of the RK2 step (all bitwise identical)")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-steps", type=int, default=4, help="steps per timed CPU run")
+    ap.add_argument("--cpu-runs", type=int, default=3)
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(json.dumps({"error": "this backend runs on one GPU; --gpus must be 1 (multi-GPU sharding is a "
+                                   "design note, not built)", "n_gpus": args.gpus}))
+        return 2
+    rank = int(os.environ.get("RANK", "0"))
+
+    grid, system, realtype, U0, B = build_scenario(args.size, args.precision, args.seed)
+    backend = VF.make_hip_backend(realtype)
+    sim = VF.Simulator(backend, system, VF.RungeKutta2(), grid, cfl=0.25)
+    sim.set_kernels(args.kernels)
+    L = C.lib()
+
+    # parity probe: the first step, compared bitwise with the oracle's below
+    VF.set_current_state(sim, U0)
+    VF.perform_step(sim, float("inf"))
+    first_gpu = sim.get_full_state()
+
+    VF.set_current_state(sim, U0)
+    sim.t = 0.0
+    if args.warmup > 0:
+        VF.advance_steps(sim, args.warmup)
+    C.check(sim.handle, L.sfv_synchronize(sim.handle))
+    t0 = time.perf_counter()
+    VF.advance_steps(sim, args.steps)
+    C.check(sim.handle, L.sfv_synchronize(sim.handle))
+    elapsed = time.perf_counter() - t0
+    cells = grid.nx * grid.ny
+    value = cells * args.steps / elapsed
+    U = sim.get_full_state()
+    finite = bool(np.isfinite(U).all())
+
+    # per-kernel device time from HIP events on the handle's stream
+    ms = {}
+    timing_sim = sim
+    if args.kernels == "per-pass":       # the stage kernels timed are the fused march ones
+        timing_sim = VF.Simulator(backend, system, VF.RungeKutta2(), grid, cfl=0.25)
+        VF.set_current_state(timing_sim, U0)
+    for k in ("stage1", "stage2", "bc", "step"):
+        out = ctypes.c_double()
+        C.check(timing_sim.handle, L.sfv_time_kernel(timing_sim.handle, k.encode(), 20, ctypes.byref(out)))
+        ms[k] = out.value
+    dominant = max(("stage1", "stage2"), key=lambda k: ms[k])
+    alg_bytes = BYTES_PER_CELL_STAGE[args.precision][dominant] * cells
+    achieved = alg_bytes / (ms[dominant] * 1e-3) / 1e9
+    workload = f"terrain{args.size}_{args.precision}"
+    traffic = read_traffic(dominant, workload)
+    step_gbs = value * BYTES_PER_CELL_STEP[args.precision] / 1e9
+
+    result = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": args.precision,
+        "data": "synthetic",
+        "config": {
+            "workload": (f"terrain {args.size}x{args.size}, dx=1 m, 32 seeded Gaussian bumps + 1e-3 slope, "
+                         f"w=max(B,2 m), Manning n=0.03, rain 1e-5 m/s, wall BCs, kappa=1e-4, RK2, CFL 0.25, "
+                         f"{args.steps} steps"),
+            "nx": args.size, "ny": args.size, "precision": args.precision,
+            "kernels": args.kernels,
+            "parallelism": "single-gpu",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": dominant,
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": traffic,
+            "algorithmic_bytes_per_launch": alg_bytes,
+            "kernel_ms": ms,
+        },
+        "step_roofline": {"bytes_per_cell_update": BYTES_PER_CELL_STEP[args.precision],
+                          "achieved_gbs": step_gbs, "frac": step_gbs / HBM_PEAK_GBS},
+        "state_finite": finite,
+    }
+
+    if rank == 0 and not args.no_cpu:
+        threads = os.cpu_count() or 1
+        try:
+            threads = min(threads, int(os.environ.get("OMP_NUM_THREADS", threads)))
+        except ValueError:
+            pass
+        first_cpu, cpu_value, cpu_med = cpu_baseline(grid, system, realtype, U0, B, args.cpu_steps, args.cpu_runs,
+                                                     threads)
+        result["cpu_baseline"] = {
+            "value": cpu_value, "unit": "cell-updates/s", "cores": threads, "kind": "port",
+            "sample": (f"C++ oracle (loop-faithful restatement of the reference CPU path), same workload: "
+                       f"median of {args.cpu_runs} runs of {args.cpu_steps} RK2 steps after 1 warm-up step, "
+                       f"OpenMP over rows"),
+        }
+        result["parity_first_step"] = "bitwise" if np.array_equal(first_gpu, first_cpu) else "MISMATCH"
+        result["gpu_over_cpu"] = value / cpu_value
+    print(json.dumps(result))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

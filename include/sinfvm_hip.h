@@ -160,10 +160,12 @@ int sfv_time_kernel(sfv_handle* h, const char* which, int32_t n, double* ms_out)
 /* Number of visible HIP devices (0 when none).  Needs no handle. */
 int sfv_device_count(void);
 
-/* Select the fused stage kernels (1, the default for RungeKutta2 with a wall
- * or periodic boundary) or the per-pass kernels (0) for this handle.  Both
- * give identical results; the per-pass path is kept as the device-side
- * reference and is the one NeumannBC and ForwardEuler use. */
+/* Select the kernels of an RK2 step for this handle: 2 = fused stage kernels,
+ * one wavefront per 60-column strip marching in y (the default whenever the
+ * boundary-condition sources of the ghost cells are interior cells); 1 =
+ * fused stage kernels tiled through LDS; 0 = one kernel per reference pass
+ * (the device-side reference, and the path for NeumannBC and ForwardEuler).
+ * All give bitwise identical results.  No counterpart in the reference. */
 int sfv_set_fused(sfv_handle* h, int32_t fused);
 
 #ifdef __cplusplus

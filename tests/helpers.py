@@ -44,11 +44,11 @@ def make_oracle(sc: Scenario):
     return o
 
 
-def make_hip(sc: Scenario, fused: bool | None = None):
+def make_hip(sc: Scenario, kernels: str | None = None):
     backend = VF.HIPBackend(sc.realtype)
     sim = VF.Simulator(backend, sc.system, sc.stepper, sc.grid, cfl=sc.cfl)
-    if fused is not None:
-        sim.set_fused(fused)
+    if kernels is not None:
+        sim.set_kernels(kernels)
     VF.set_current_state(sim, sc.U0)
     return sim
 
@@ -86,11 +86,11 @@ def dam_break(nx, ny, *, hl=0.005, hr=0.001, L=10.0, stepper=None, bc=None, dire
     return Scenario(name or f"dambreak{nx}x{ny}", grid, system, stepper or VF.RungeKutta2(), cfl, U0)
 
 
-def run_both_compare(sc: Scenario, nsteps: int, *, fused=None, max_dt=float("inf")):
+def run_both_compare(sc: Scenario, nsteps: int, *, kernels=None, max_dt=float("inf")):
     """Step HIP and oracle nsteps each; return the list of (dt_hip, dt_ref)
     and the final full states."""
     o = make_oracle(sc)
-    sim = make_hip(sc, fused)
+    sim = make_hip(sc, kernels)
     dts = []
     t = 0.0
     for _ in range(nsteps):

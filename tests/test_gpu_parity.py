@@ -12,24 +12,27 @@ from helpers import dam_break, max_rel_diff, run_both_compare, terrain
 pytestmark = pytest.mark.gpu
 
 
-def _assert_bitwise(sc, nsteps, fused=None, **kw):
-    dts, U_hip, U_ref = run_both_compare(sc, nsteps, fused=fused, **kw)
-    print(f"[parity] {sc.name} fused={fused} steps={nsteps} dt0={dts[0]} rel={max_rel_diff(U_hip, U_ref):.3e}",
-          flush=True)
+KERNELS = ["march", "tiled", "per-pass"]
+
+
+def _assert_bitwise(sc, nsteps, kernels=None, **kw):
+    print(f"[parity] {sc.name} kernels={kernels} steps={nsteps}", flush=True)
+    dts, U_hip, U_ref = run_both_compare(sc, nsteps, kernels=kernels, **kw)
+    print(f"[parity]   dt0={dts[0]} rel={max_rel_diff(U_hip, U_ref):.3e}", flush=True)
     for k, (dh, dr) in enumerate(dts):
         assert dh == dr or (np.isnan(dh) and np.isnan(dr)), f"dt differs at step {k}: {dh!r} vs {dr!r}"
     diff = np.argwhere(U_hip != U_ref)
     assert diff.size == 0, f"{len(diff)} cells differ, first {diff[:5].tolist()}, rel={max_rel_diff(U_hip, U_ref)}"
 
 
-@pytest.mark.parametrize("fused", [True, False])
-def test_terrain_rk2_wall(fused):
-    _assert_bitwise(terrain(70, 45, seed=3), 30, fused=fused)
+@pytest.mark.parametrize("kernels", KERNELS)
+def test_terrain_rk2_wall(kernels):
+    _assert_bitwise(terrain(70, 45, seed=3), 30, kernels=kernels)
 
 
-@pytest.mark.parametrize("fused", [True, False])
-def test_terrain_rk2_periodic(fused):
-    _assert_bitwise(terrain(64, 16, seed=5, bc=VF.PeriodicBC()), 25, fused=fused)
+@pytest.mark.parametrize("kernels", KERNELS)
+def test_terrain_rk2_periodic(kernels):
+    _assert_bitwise(terrain(64, 16, seed=5, bc=VF.PeriodicBC()), 25, kernels=kernels)
 
 
 def test_terrain_neumann_rk2():
@@ -44,13 +47,24 @@ def test_no_reconstruction():
     _assert_bitwise(terrain(50, 20, seed=11, rec=VF.NoReconstruction()), 20)
 
 
-@pytest.mark.parametrize("shape", [(1, 1), (3, 2), (5, 3), (3, 70), (130, 3), (65, 9)])
+@pytest.mark.parametrize("shape", [(1, 1), (3, 2), (5, 3), (3, 70), (130, 3), (65, 9), (61, 17), (121, 33)])
 def test_small_and_ragged(shape):
     _assert_bitwise(terrain(*shape, seed=13, dx=1.0, n_bumps=3), 12)
 
 
+@pytest.mark.parametrize("kernels", KERNELS)
+def test_ragged_tiles(kernels):
+    # widths and heights that leave partial tiles and partial wave strips
+    _assert_bitwise(terrain(125, 37, seed=19), 10, kernels=kernels)
+
+
 def test_gc3():
     _assert_bitwise(terrain(40, 24, seed=15, gc=3), 15)
+
+
+@pytest.mark.parametrize("kernels", KERNELS)
+def test_no_reconstruction_fused(kernels):
+    _assert_bitwise(terrain(70, 30, seed=23, rec=VF.NoReconstruction()), 10, kernels=kernels)
 
 
 def test_dam_break_dry_front():
@@ -61,9 +75,11 @@ def test_dam_break_y_direction():
     _assert_bitwise(dam_break(6, 150, hl=0.005, hr=0.001, direction=1), 40)
 
 
-def test_terrain_f32_storage():
-    _assert_bitwise(terrain(48, 40, seed=17, realtype=np.float32), 20)
+@pytest.mark.parametrize("kernels", KERNELS)
+def test_terrain_f32_storage(kernels):
+    _assert_bitwise(terrain(48, 40, seed=17, realtype=np.float32), 20, kernels=kernels)
 
 
-def test_terrain_256_20_steps():
-    _assert_bitwise(terrain(256, 256, seed=21), 20)
+@pytest.mark.parametrize("kernels", KERNELS)
+def test_terrain_256_20_steps(kernels):
+    _assert_bitwise(terrain(256, 256, seed=21), 20, kernels=kernels)

@@ -9,6 +9,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/sinfvm_hip.h"
@@ -97,7 +98,13 @@ int step_kernels(sfv_handle* h, int slot, int cur) {
         HIP_TRY(h, launch_bc<S>(st, P, h->ctl, slot, U2, BC_LAST_STAGE));
         return SFV_OK;
     }
-    if (h->fused) {
+    if (h->fused == KV_MARCH) {
+        HIP_TRY(h, launch_march<S>(st, P, h->ctl, slot, 1, U0, nullptr, RHS, h->B));
+        HIP_TRY(h, launch_march<S>(st, P, h->ctl, slot, 2, U0, RHS, U2, h->B));
+        HIP_TRY(h, launch_bc<S>(st, P, h->ctl, slot, U2, BC_LAST_STAGE));
+        return SFV_OK;
+    }
+    if (h->fused == KV_TILED) {
         HIP_TRY(h, launch_stage1<S>(st, P, h->ctl, slot, U0, RHS, h->B));
         HIP_TRY(h, launch_stage2<S>(st, P, h->ctl, slot, U0, RHS, U2, h->B));
         HIP_TRY(h, launch_bc<S>(st, P, h->ctl, slot, U2, BC_LAST_STAGE));
@@ -355,7 +362,8 @@ int sfv_time_kernel(sfv_handle* h, const char* which, int32_t n, double* ms_out)
     int rc = ensure_ready(h);
     if (rc) return rc;
     if (!which || !ms_out || n < 1) return fail(h, SFV_ERR_ARGUMENT, "bad arguments");
-    if (h->cfg.stepper != SFV_RUNGE_KUTTA2) return fail(h, SFV_ERR_ARGUMENT, "kernel timing needs RungeKutta2");
+    if (h->cfg.stepper != SFV_RUNGE_KUTTA2 || !h->fusable())
+        return fail(h, SFV_ERR_ARGUMENT, "kernel timing needs RungeKutta2 with fused stage kernels");
     const std::string w(which);
     if (w != "stage1" && w != "stage2" && w != "bc" && w != "step")
         return fail(h, SFV_ERR_ARGUMENT, "which must be stage1, stage2, bc or step");
@@ -366,30 +374,29 @@ int sfv_time_kernel(sfv_handle* h, const char* which, int32_t n, double* ms_out)
     hipEvent_t e0, e1;
     HIP_TRY(h, hipEventCreate(&e0));
     HIP_TRY(h, hipEventCreate(&e1));
-    auto one = [&]() -> hipError_t {
+    auto stage = [&](int which, auto* U0, auto* R, auto* Us) -> hipError_t {
+        using S = std::remove_pointer_t<decltype(U0)>;
         const KParams& P = h->P;
-        if (h->cfg.precision == SFV_F64) {
-            double* U0 = (double*)h->U[cur];
-            double* Us = (double*)h->U[2];
-            double* R = (double*)h->RHS;
-            if (w == "stage1") return launch_stage1<double>(h->stream, P, h->ctl, slot, U0, R, h->B);
-            if (w == "stage2") return launch_stage2<double>(h->stream, P, h->ctl, slot, U0, R, Us, h->B);
-            if (w == "bc") return launch_bc<double>(h->stream, P, h->ctl, slot, Us, BC_IN_STEP);
-            hipError_t e = launch_stage1<double>(h->stream, P, h->ctl, slot, U0, R, h->B);
-            if (e == hipSuccess) e = launch_stage2<double>(h->stream, P, h->ctl, slot, U0, R, Us, h->B);
-            if (e == hipSuccess) e = launch_bc<double>(h->stream, P, h->ctl, slot, Us, BC_IN_STEP);
-            return e;
-        }
-        float* U0 = (float*)h->U[cur];
-        float* Us = (float*)h->U[2];
-        float* R = (float*)h->RHS;
-        if (w == "stage1") return launch_stage1<float>(h->stream, P, h->ctl, slot, U0, R, h->B);
-        if (w == "stage2") return launch_stage2<float>(h->stream, P, h->ctl, slot, U0, R, Us, h->B);
-        if (w == "bc") return launch_bc<float>(h->stream, P, h->ctl, slot, Us, BC_IN_STEP);
-        hipError_t e = launch_stage1<float>(h->stream, P, h->ctl, slot, U0, R, h->B);
-        if (e == hipSuccess) e = launch_stage2<float>(h->stream, P, h->ctl, slot, U0, R, Us, h->B);
-        if (e == hipSuccess) e = launch_bc<float>(h->stream, P, h->ctl, slot, Us, BC_IN_STEP);
+        if (h->fused == KV_TILED)
+            return which == 1 ? launch_stage1<S>(h->stream, P, h->ctl, slot, U0, R, h->B)
+                              : launch_stage2<S>(h->stream, P, h->ctl, slot, U0, R, Us, h->B);
+        return which == 1 ? launch_march<S>(h->stream, P, h->ctl, slot, 1, U0, nullptr, R, h->B)
+                          : launch_march<S>(h->stream, P, h->ctl, slot, 2, U0, R, Us, h->B);
+    };
+    auto run = [&](auto* U0, auto* R, auto* Us) -> hipError_t {
+        using S = std::remove_pointer_t<decltype(U0)>;
+        if (w == "stage1") return stage(1, U0, R, Us);
+        if (w == "stage2") return stage(2, U0, R, Us);
+        if (w == "bc") return launch_bc<S>(h->stream, h->P, h->ctl, slot, Us, BC_IN_STEP);
+        hipError_t e = stage(1, U0, R, Us);
+        if (e == hipSuccess) e = stage(2, U0, R, Us);
+        if (e == hipSuccess) e = launch_bc<S>(h->stream, h->P, h->ctl, slot, Us, BC_IN_STEP);
         return e;
+    };
+    auto one = [&]() -> hipError_t {
+        if (h->cfg.precision == SFV_F64)
+            return run((double*)h->U[cur], (double*)h->RHS, (double*)h->U[2]);
+        return run((float*)h->U[cur], (float*)h->RHS, (float*)h->U[2]);
     };
     HIP_TRY(h, one());   // warm-up
     HIP_TRY(h, hipEventRecord(e0, h->stream));
@@ -412,10 +419,11 @@ int sfv_device_count(void) {
 
 int sfv_set_fused(sfv_handle* h, int32_t fused) {
     if (!h) return SFV_ERR_ARGUMENT;
-    if (fused && !h->fusable())
+    if (fused < KV_PER_PASS || fused > KV_MARCH) return fail(h, SFV_ERR_ARGUMENT, "unknown kernel variant");
+    if (fused != KV_PER_PASS && !h->fusable())
         return fail(h, SFV_ERR_ARGUMENT,
-                    "fused kernels need RungeKutta2 and boundary-condition sources that are interior cells");
-    h->fused = fused ? 1 : 0;
+                    "fused kernels need RungeKutta2, gc >= 2 and boundary-condition sources that are interior cells");
+    h->fused = fused;
     return SFV_OK;
 }
 

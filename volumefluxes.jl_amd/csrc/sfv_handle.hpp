@@ -4,12 +4,16 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <cstring>
 #include <string>
 #include <vector>
 
 #include "../../include/sinfvm_hip.h"
 #include "swe_kernels.hpp"
+
+// Kernel variants of an RK2 step (all give identical results).
+enum { KV_PER_PASS = 0, KV_TILED = 1, KV_MARCH = 2 };
 
 struct sfv_handle {
     sfv_config cfg{};
@@ -33,7 +37,7 @@ struct sfv_handle {
     bool has_state = true;             // the zero-initialised state is a valid state
     bool have_dt = false;
     double last_dt = 0.0;
-    int fused = 1;
+    int fused = KV_MARCH;               // kernel variant, KV_*
     std::string err;
 
     int alloc(void** p, size_t bytes, std::string& why) {
@@ -92,7 +96,7 @@ struct sfv_handle {
         }
         if (hipStreamSynchronize(stream) != hipSuccess) { why = "initialisation failed"; return SFV_ERR_HIP; }
         build_params();
-        fused = fusable() ? 1 : 0;
+        fused = fusable() ? KV_MARCH : KV_PER_PASS;
         return SFV_OK;
     }
 
@@ -104,6 +108,9 @@ struct sfv_handle {
         p.nxT = c.nx + 2 * c.gc; p.nyT = c.ny + 2 * c.gc;
         p.dx = c.dx; p.dy = c.dy;
         p.inv_dx = 1 / c.dx; p.inv_dy = 1 / c.dy;
+        auto pow2 = [](double d) { int e; return std::isfinite(d) && d > 0 && std::frexp(d, &e) == 0.5; };
+        p.dx_pow2 = pow2(c.dx) ? 1 : 0;
+        p.dy_pow2 = pow2(c.dy) ? 1 : 0;
         p.phys.g = c.g; p.phys.rho = c.rho; p.phys.cutoff = c.depth_cutoff; p.phys.kappa = c.desingularizing_kappa;
         p.theta = c.theta; p.cfl = c.cfl; p.Cz = c.Cz;
         p.rec = c.reconstruction == SFV_LINEAR_RECONSTRUCTION ? 1 : 0;

@@ -15,6 +15,7 @@ constexpr int MAX_GC = 8;
 struct KParams {
     int nx, ny, gc, nxT, nyT;
     double dx, dy, inv_dx, inv_dy;  // inv_d is 1/Δ as numericalflux.jl:27 computes it
+    int dx_pow2, dy_pow2;           // Δ is a power of two (x/Δ == x*(1/Δ) exactly)
     PhysParams phys;
     double theta, cfl, Cz;
     int rec;        // 0 none, 1 linear

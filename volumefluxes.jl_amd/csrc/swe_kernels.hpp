@@ -47,6 +47,10 @@ hipError_t launch_prepare(hipStream_t st, const KParams& P, StepCtl* ctl, int sl
 template <class S>
 hipError_t launch_stage1(hipStream_t st, const KParams& P, StepCtl* ctl, int slot, const S* U0, S* RHS,
                          const double* B);
+// stage = 1: U0 -> OUT = RHS1 (+ speeds); stage = 2: U0, RHS1 -> OUT = U2.
+template <class S>
+hipError_t launch_march(hipStream_t st, const KParams& P, StepCtl* ctl, int slot, int stage, const S* U0,
+                        const S* RHS1, S* OUT, const double* B);
 template <class S>
 hipError_t launch_stage2(hipStream_t st, const KParams& P, StepCtl* ctl, int slot, const S* U0, const S* RHS1, S* U2,
                          const double* B);

@@ -175,7 +175,7 @@ __device__ __forceinline__ void apply_faces(const KParams& P, const TileGeom& T,
             const S hR = (S)s.f[3][fp];   // right face depth of this cell
             const S hL = (S)s.f[4][fm];   // left face depth of this cell
             const S hsum = hR + hL;
-            acc[q][1 + d] = (S)(acc[q][1 + d] + -P.phys.g * ((Br - Bl) / dd) * (hsum / 2.0));
+            acc[q][1 + d] = (S)(acc[q][1 + d] + -P.phys.g * slope_div(Br - Bl, dd, d == 0 ? P.inv_dx : P.inv_dy, d == 0 ? P.dx_pow2 : P.dy_pow2) * (hsum / 2.0));
         }
     }
 }
@@ -250,9 +250,7 @@ __device__ __forceinline__ void finish1(const KParams& P, S x0, S x1, S x2, S r0
     o[2] = (S)(x2 + dt_d * r2);
     if (P.friction) {
         const double fs = friction_scalar(P.friction, P.Cz, P.phys.kappa, x0, x1, x2, Bm);
-        o[0] = (S)(o[0] / (1 - dt_s * 0.0));
-        o[1] = (S)(o[1] / (1 - dt_s * fs));
-        o[2] = (S)(o[2] / (1 - dt_s * fs));
+        apply_friction(o, dt_s, fs);
     }
     if (o[0] - Bm < P.phys.cutoff) {
         o[0] = (S)jl_max((double)o[0], Bm);
@@ -333,9 +331,7 @@ __global__ void __launch_bounds__(F_THREADS) k_stage2(KParams P, StepCtl* ctl, i
         const double Bm = 0.25 * (s.b[by][bx] + s.b[by][bx + 1] + s.b[by + 1][bx] + s.b[by + 1][bx + 1]);
         if (P.friction) {
             const double fs = friction_scalar(P.friction, P.Cz, P.phys.kappa, x0, x1, x2, Bm);
-            o[0] = (S)(o[0] / (1 - dt_s * 0.0));
-            o[1] = (S)(o[1] / (1 - dt_s * fs));
-            o[2] = (S)(o[2] / (1 - dt_s * fs));
+            apply_friction(o, dt_s, fs);
         }
         if (o[0] - Bm < P.phys.cutoff) {
             o[0] = (S)jl_max((double)o[0], Bm);
@@ -377,9 +373,7 @@ __global__ void __launch_bounds__(F_THREADS) k_stage2(KParams P, StepCtl* ctl, i
             o[2] = (S)(0.5 * (x2 + o[2]));
             if (P.friction) {
                 const double fs = friction_scalar(P.friction, P.Cz, P.phys.kappa, u1[0], u1[1], u1[2], Bm);
-                o[0] = (S)(o[0] / (1 - dt_s * 0.0));
-                o[1] = (S)(o[1] / (1 - dt_s * fs));
-                o[2] = (S)(o[2] / (1 - dt_s * fs));
+                apply_friction(o, dt_s, fs);
             }
             if (o[0] - Bm < P.phys.cutoff) {
                 o[0] = (S)jl_max((double)o[0], Bm);

@@ -1,0 +1,443 @@
+// swe_kernels_march.hip — RK2 stage kernels, one wavefront per strip.
+//
+// A wave owns MX = 60 output columns (lanes 2..61 of its 64; lanes 0, 1,
+// 62, 63 are the x halo) and a segment of MY rows, and marches up the
+// segment one row per iteration.  x neighbours come from the adjacent lanes
+// (cross-lane shuffles), y neighbours from a sliding window of rows held in
+// registers, so every cell is loaded once per wave, every reconstruction
+// (linearreconstruction.jl:58-102) is evaluated once per cell and direction,
+// and every face flux (centralupwind.jl:48-77) once per face — the reference
+// evaluates each face from both sides with identical arguments
+// (numericalflux.jl:25-26), so sharing it is bit-exact.  No LDS and no
+// block-level barriers: waves are independent.
+//
+// The segment height is chosen per grid by the host so that the waves fill
+// the GPU's resident wave slots in whole rounds.  Per iteration r the wave
+// takes row jr = j0-2+r (its loads were issued one iteration earlier) and
+//   1. y-reconstructs row jr-1 from rows jr-2..jr,
+//   2. evaluates the y face between rows jr-2 and jr-1,
+//   3. finishes row jr-2 (its x part was done one iteration earlier, its
+//      y faces are now both known): flux difference in the reference's
+//      X-then-Y order, bottom source, rain/infiltration, then either stores
+//      the stage right-hand side (stage 1) or the RK2 result (stage 2),
+//   4. x-reconstructs row jr-1 and evaluates its x faces (the x part).
+// Stage 2 rebuilds U1 = clamp(friction(U0 + dt*RHS1)) for every loaded cell
+// (ghost cells through the boundary-condition source maps), so the stage-1
+// state never touches HBM.
+#include <type_traits>
+
+#include "swe_kernels.hpp"
+
+namespace sfv {
+
+// Build-time tuning knobs (defaults are the measured best on MI355X):
+//   SFV_MARCH_PREFETCH    issue the next row's loads one iteration early
+//   SFV_MARCH_BRANCHFREE  evaluate the x and y faces with the branch-free forms
+//   SFV_MARCH_ROWS        rows per wave segment (0: chosen per grid)
+//   SFV_MARCH_MIN_WAVES   minimum waves per SIMD the register allocation must allow
+#ifndef SFV_MARCH_PREFETCH
+#define SFV_MARCH_PREFETCH 0
+#endif
+#ifndef SFV_MARCH_BRANCHFREE
+#define SFV_MARCH_BRANCHFREE 0
+#endif
+#ifndef SFV_MARCH_ROWS
+#define SFV_MARCH_ROWS 16
+#endif
+#ifndef SFV_MARCH_MIN_WAVES
+#define SFV_MARCH_MIN_WAVES 1
+#endif
+
+constexpr int MW = 64;          // lanes
+constexpr int MX = 60;          // output columns per wave
+constexpr int MWAVES = 4;       // waves per block
+
+template <class T>
+__device__ __forceinline__ T shfl_up1(T v) { return __shfl_up(v, 1, MW); }
+template <class T>
+__device__ __forceinline__ T shfl_dn1(T v) { return __shfl_down(v, 1, MW); }
+
+__device__ __forceinline__ unsigned long long wave_max_u64m(unsigned long long v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const unsigned long long o = __shfl_xor(v, off, MW);
+        v = o > v ? o : v;
+    }
+    return v;
+}
+
+__device__ __forceinline__ double Bload(const KParams& P, const double* __restrict__ B, int i, int j) {
+    return (i <= P.nxT && j <= P.nyT) ? B[(size_t)i + (size_t)(P.nxT + 1) * j] : 0.0;
+}
+
+// Stage-1 result of one cell (rungekutta.jl:38-41 with the Float64 dt,
+// friction.jl:80-84 from U0 with the stored dt, timestepper.jl:53-63).
+template <class S>
+__device__ __forceinline__ void stage1_cell(const KParams& P, const S x[3], const S r[3], double Bm, double dt_d,
+                                            S dt_s, S o[3]) {
+    o[0] = (S)(x[0] + dt_d * r[0]);
+    o[1] = (S)(x[1] + dt_d * r[1]);
+    o[2] = (S)(x[2] + dt_d * r[2]);
+    if (P.friction) {
+        const double fs = friction_scalar(P.friction, P.Cz, P.phys.kappa, x[0], x[1], x[2], Bm);
+        apply_friction(o, dt_s, fs);
+    }
+    if (o[0] - Bm < P.phys.cutoff) {
+        o[0] = (S)jl_max((double)o[0], Bm);
+        o[1] = (S)0.0;
+        o[2] = (S)0.0;
+    }
+}
+
+template <class S, int STAGE>
+__global__ void __launch_bounds__(MW* MWAVES) __attribute__((amdgpu_waves_per_eu(SFV_MARCH_MIN_WAVES)))
+k_march(KParams P, StepCtl* ctl, int slot, int my,
+                                                      const S* __restrict__ U0, const S* __restrict__ RHS1,
+                                                      S* __restrict__ OUT, const double* __restrict__ B) {
+    const StepSlot& sl = ctl->slot[slot];
+    if (!sl.active) return;
+    const int lane = threadIdx.x & (MW - 1), wv = threadIdx.x / MW;
+    const int seg = blockIdx.y * MWAVES + wv;
+    const int gc = P.gc, nxT = P.nxT, nyT = P.nyT;
+    const int j0 = gc + seg * my;
+    if (j0 >= nyT - gc) return;                    // whole wave; the kernel has no barriers
+    const int th = min(my, nyT - gc - j0);
+    const int i = gc + blockIdx.x * MX + lane - 2; // this lane's column (>= 0 since gc >= 2)
+    const bool in_dom = i < nxT;
+    const bool out_col = lane >= 2 && lane < MX + 2 && i < nxT - gc;
+    const size_t n = (size_t)nxT * nyT;
+
+    double dt_d = 0.0;
+    S dt_s = S(0);
+    if (STAGE == 2) {
+        dt_d = stage1_dt<S>(P, ctl, sl);
+        dt_s = (S)dt_d;
+        if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) {
+            ctl->dt = (double)dt_s;
+            ctl->dt_stage1 = dt_d;
+        }
+    }
+
+    // Column remap of a ghost column to its boundary-condition source (stage 2).
+    int sci = i, fxc = 0;
+    if (STAGE == 2 && in_dom) {
+        if (i < gc) { sci = P.src_lo_x[i]; fxc = P.flip_lo_x[i]; }
+        else if (i >= nxT - gc) { sci = P.src_hi_x[i - (nxT - gc)]; fxc = P.flip_hi_x[i - (nxT - gc)]; }
+    }
+
+    // ---- raw inputs, fetched one row ahead so the loads overlap a row of work
+    struct Raw {
+        S x[3];          // stage input (stage 1) or U0 (stage 2) at the source cell
+        S r[3];          // RHS1 at the source cell (stage 2)
+        double bsrc;     // B_cell of a remapped source cell (stage 2)
+        int fy;          // hv negated (stage 2, ghost row of a wall)
+        bool remap;
+    };
+    auto fetch = [&](int jr, Raw& o) {
+        o.remap = false;
+        o.fy = 0;
+        o.bsrc = 0.0;
+        if (!in_dom) {
+            o.x[0] = o.x[1] = o.x[2] = S(0);
+            o.r[0] = o.r[1] = o.r[2] = S(0);
+            return;
+        }
+        int sj = jr;
+        if (STAGE == 2) {
+            if (jr < gc) { sj = P.src_lo_y[jr]; o.fy = P.flip_lo_y[jr]; }
+            else if (jr >= nyT - gc) { sj = P.src_hi_y[jr - (nyT - gc)]; o.fy = P.flip_hi_y[jr - (nyT - gc)]; }
+        }
+        const size_t k = (size_t)(STAGE == 2 ? sci : i) + (size_t)nxT * sj;
+        o.x[0] = U0[k]; o.x[1] = U0[n + k]; o.x[2] = U0[2 * n + k];
+        if (STAGE == 2) {
+            o.r[0] = RHS1[k]; o.r[1] = RHS1[n + k]; o.r[2] = RHS1[2 * n + k];
+            o.remap = sci != i || sj != jr;
+            if (o.remap)   // the source cell's bottom
+                o.bsrc = 0.25 * (Bload(P, B, sci, sj) + Bload(P, B, sci + 1, sj) + Bload(P, B, sci, sj + 1) +
+                                 Bload(P, B, sci + 1, sj + 1));
+        }
+    };
+    // bottom corner row: own column, and (lane 63 only) the next column
+    auto fetch_b = [&](int row, double& own, double& extra) {
+        own = in_dom ? Bload(P, B, i, row) : 0.0;
+        extra = (lane == MW - 1 && in_dom) ? Bload(P, B, i + 1, row) : 0.0;
+    };
+    auto next_col = [&](double own, double extra) {
+        const double v = shfl_dn1(own);
+        return lane == MW - 1 ? extra : v;
+    };
+
+    // corner window: rows jr-1 (bm1), jr (b0), jr+1 (bp1); n* = column i+1
+    double bm1 = 0.0, nm1 = 0.0, b0, n0, bp1, np1, e_;
+    fetch_b(j0 - 2, b0, e_);
+    n0 = next_col(b0, e_);
+    fetch_b(j0 - 1, bp1, e_);
+    np1 = next_col(bp1, e_);
+    Raw cur, nxt;
+    if (SFV_MARCH_PREFETCH) fetch(j0 - 2, cur);
+
+    S cA[3] = {S(0), S(0), S(0)}, cB[3] = {S(0), S(0), S(0)};
+    S Lyp0 = S(0), Ryp[3] = {S(0), S(0), S(0)};   // previous row's y faces (left: depth only)
+    double Blyp = 0.0, Bryp = 0.0;                 // its y face bottoms
+    double Fyp[3] = {0.0, 0.0, 0.0};               // y flux below that row
+    S accx[3] = {S(0), S(0), S(0)};                // x part of the pending row
+    double bcp = 0.0;                              // B_cell of the pending row
+    unsigned long long smx = 0, smy = 0;
+    const double g = P.phys.g;
+    // x faces this lane evaluates (its left face) that border an output cell
+    const bool face_x_ok = in_dom && lane >= 2 && lane <= MX + 2 && (lane == 2 ? i < nxT - gc : i - 1 < nxT - gc);
+
+    auto body = [&](int r, auto full_tag) {
+        constexpr bool FULL = decltype(full_tag)::value;   // r >= 4: every step applies
+        const int jr = j0 - 2 + r;
+        double nb, ne;
+        fetch_b(jr + 2, nb, ne);                   // next corner row (Bload bounds-checks row nyT+1)
+        if (SFV_MARCH_PREFETCH) {
+            if (r + 1 < th + 4) fetch(jr + 1, nxt);    // rows up to j0+th+1 <= nyT-1 (gc >= 2)
+        } else {
+            fetch(jr, cur);
+        }
+
+        // stage input of row jr
+        S cC[3];
+        if (STAGE == 1) {
+            cC[0] = cur.x[0]; cC[1] = cur.x[1]; cC[2] = cur.x[2];
+        } else if (in_dom) {
+            const double Bm = cur.remap ? cur.bsrc : 0.25 * (((b0 + n0) + bp1) + np1);
+            stage1_cell<S>(P, cur.x, cur.r, Bm, dt_d, dt_s, cC);
+            if (fxc) cC[1] = (S)(-cC[1]);
+            if (cur.fy) cC[2] = (S)(-cC[2]);
+        } else {
+            cC[0] = cC[1] = cC[2] = S(0);
+        }
+
+        // 1. y reconstruction of row m = jr-1 (rows jr-2, jr-1, jr) and x
+        //    reconstruction of row q = jr-1 (this lane's cell cB and its x
+        //    neighbours; bottom corner rows jr-1, jr)
+        S Ly[3] = {S(0), S(0), S(0)}, Ry[3] = {S(0), S(0), S(0)};
+        const double Bly = 0.5 * (bm1 + nm1), Bry = 0.5 * (b0 + n0);
+        const double bcq = 0.25 * (((bm1 + nm1) + b0) + n0);    // B_cell of row jr-1
+        S Lx[3] = {S(0), S(0), S(0)}, Rx[3] = {S(0), S(0), S(0)}, Rxl[3] = {S(0), S(0), S(0)};
+        const double Blx = 0.5 * (bm1 + b0), Brx = 0.5 * (nm1 + n0);
+        if (FULL || r >= 1) {
+            S cl[3], cr[3];
+#pragma unroll
+            for (int v = 0; v < 3; ++v) { cl[v] = shfl_up1(cB[v]); cr[v] = shfl_dn1(cB[v]); }
+            if (P.rec == 1) {
+#if SFV_MARCH_BRANCHFREE
+                if (FULL || r >= 2) reconstruct_cell_nb(cA, cB, cC, P.theta, Bly, Bry, Ly, Ry);
+                reconstruct_cell_nb(cl, cB, cr, P.theta, Blx, Brx, Lx, Rx);
+#else
+                if (FULL || r >= 2) reconstruct_cell(cA, cB, cC, P.theta, Bly, Bry, Ly, Ry);
+                reconstruct_cell(cl, cB, cr, P.theta, Blx, Brx, Lx, Rx);
+#endif
+            } else {
+#pragma unroll
+                for (int v = 0; v < 3; ++v) { Ly[v] = Ry[v] = Lx[v] = Rx[v] = cB[v]; }
+                Ly[0] = Ry[0] = Lx[0] = Rx[0] = (S)(cB[0] - bcq);
+            }
+#pragma unroll
+            for (int v = 0; v < 3; ++v) Rxl[v] = shfl_up1(Rx[v]);
+        }
+        // 2. both faces: y face between rows jr-2 and jr-1, and this lane's
+        //    left x face in row jr-1 (independent chains, one basic block)
+        double Fy[3] = {0.0, 0.0, 0.0}, Fxm[3] = {0.0, 0.0, 0.0};
+        if (FULL || r >= 1) {
+#if SFV_MARCH_BRANCHFREE
+            const S* hp[4] = {&Ryp[0], &Ly[0], &Rxl[0], &Lx[0]};
+            double hs[4];
+            desingularize_n<S, 4>(hp, P.phys.kappa, hs);
+            const double spy = central_upwind_nb<1>(Ryp, Ly, hs[0], hs[1], P.phys, Fy);
+            const double spx = central_upwind_nb<0>(Rxl, Lx, hs[2], hs[3], P.phys, Fxm);
+#else
+            const double spy = central_upwind(1, Ryp, Ly, P.phys, Fy);
+            const double spx = central_upwind(0, Rxl, Lx, P.phys, Fxm);
+#endif
+            if (STAGE == 1) {
+                const int m = jr - 1;
+                if ((FULL || r >= 3) && out_col && m >= j0 && m <= j0 + th) {
+                    const unsigned long long bits = (unsigned long long)__double_as_longlong((double)(S)spy);
+                    smy = bits > smy ? bits : smy;
+                }
+                if (face_x_ok && m >= j0 && m < j0 + th) {
+                    const unsigned long long bits = (unsigned long long)__double_as_longlong((double)(S)spx);
+                    smx = bits > smx ? bits : smx;
+                }
+            }
+        }
+        // 3. x part of row jr-1 (finished next iteration)
+        S accx_n[3] = {S(0), S(0), S(0)};
+        if (FULL || r >= 1) {
+            double Fxp[3];
+#pragma unroll
+            for (int v = 0; v < 3; ++v) Fxp[v] = shfl_dn1(Fxm[v]);
+#pragma unroll
+            for (int v = 0; v < 3; ++v) accx_n[v] = (S)(S(0) - P.inv_dx * (Fxp[v] - Fxm[v]));
+            if (P.bottom) {
+                const S hsum = Rx[0] + Lx[0];
+                accx_n[1] = (S)(accx_n[1] + -g * slope_div(Brx - Blx, P.dx, P.inv_dx, P.dx_pow2) * (hsum / 2.0));
+            }
+        }
+        // 4. finish row p = jr-2: y part on top of its x part, then sources
+        if ((FULL || r >= 4) && out_col) {
+            const int p = jr - 2;
+            const size_t k = (size_t)i + (size_t)nxT * p;
+            S acc[3] = {accx[0], accx[1], accx[2]};
+#pragma unroll
+            for (int v = 0; v < 3; ++v) acc[v] = (S)(acc[v] - P.inv_dy * (Fy[v] - Fyp[v]));
+            if (P.bottom) {
+                const S hsum = Ryp[0] + Lyp0;
+                acc[2] = (S)(acc[2] + -g * slope_div(Bryp - Blyp, P.dy, P.inv_dy, P.dy_pow2) * (hsum / 2.0));
+            }
+            acc[0] = apply_sources(P, ctl, acc[0], k);
+            if (STAGE == 1) {
+                OUT[k] = acc[0];
+                OUT[n + k] = acc[1];
+                OUT[2 * n + k] = acc[2];
+            } else {
+                // cA is U1 of row p
+                S o[3];
+#pragma unroll
+                for (int v = 0; v < 3; ++v) o[v] = (S)(cA[v] + dt_s * acc[v]);
+                o[0] = (S)(0.5 * (U0[k] + o[0]));
+                o[1] = (S)(0.5 * (U0[n + k] + o[1]));
+                o[2] = (S)(0.5 * (U0[2 * n + k] + o[2]));
+                if (P.friction) {
+                    const double fs = friction_scalar(P.friction, P.Cz, P.phys.kappa, cA[0], cA[1], cA[2], bcp);
+                    apply_friction(o, dt_s, fs);
+                }
+                const bool dry = o[0] - bcp < P.phys.cutoff;
+                const S wd = (S)jl_max((double)o[0], bcp);
+                o[0] = dry ? wd : o[0];
+                o[1] = dry ? (S)0.0 : o[1];
+                o[2] = dry ? (S)0.0 : o[2];
+                OUT[k] = o[0];
+                OUT[n + k] = o[1];
+                OUT[2 * n + k] = o[2];
+            }
+        }
+        // shift the windows
+#pragma unroll
+        for (int v = 0; v < 3; ++v) {
+            cA[v] = cB[v]; cB[v] = cC[v];
+            Ryp[v] = Ry[v];
+            Fyp[v] = Fy[v];
+            accx[v] = accx_n[v];
+        }
+        Lyp0 = Ly[0];
+        Blyp = Bly; Bryp = Bry;
+        bcp = bcq;
+        bm1 = b0; nm1 = n0;
+        b0 = bp1; n0 = np1;
+        bp1 = nb; np1 = next_col(nb, ne);
+        if (SFV_MARCH_PREFETCH) cur = nxt;
+    };
+
+    const int rmain = min(4, th + 4);
+    for (int r = 0; r < rmain; ++r) body(r, std::false_type{});
+    for (int r = rmain; r < th + 4; ++r) body(r, std::true_type{});
+
+    if (STAGE == 1) {
+        smx = wave_max_u64m(smx);
+        smy = wave_max_u64m(smy);
+        if (lane == 0) {
+            // The maxima are reached by a few waves early on: skip the atomic
+            // (all waves hit the same two addresses) when the slot already
+            // holds at least this wave's value.
+            StepSlot* s = &ctl->slot[slot];
+            const unsigned long long cx = __hip_atomic_load(&s->wsmax[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            const unsigned long long cy = __hip_atomic_load(&s->wsmax[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (smx > cx) atomicMax(&s->wsmax[0], smx);
+            if (smy > cy) atomicMax(&s->wsmax[1], smy);
+        }
+    }
+
+    // The gc x gc corner cells of the domain: never written by the boundary
+    // condition (bc.jl:92-109) nor read by a stencil, but part of the state;
+    // both stages run sources, RK update, friction and clamp on them.
+    if (STAGE == 2 && blockIdx.x == 0 && blockIdx.y == 0 && wv == 0) {
+        for (int e = lane; e < 4 * gc * gc; e += MW) {
+            const int corner = e / (gc * gc), rr = e % (gc * gc);
+            const int ci = (corner & 1) ? nxT - gc + rr % gc : rr % gc;
+            const int cj = (corner >> 1) ? nyT - gc + rr / gc : rr / gc;
+            const size_t k = (size_t)ci + (size_t)nxT * cj;
+            const double Bm = 0.25 * (Bload(P, B, ci, cj) + Bload(P, B, ci + 1, cj) + Bload(P, B, ci, cj + 1) +
+                                      Bload(P, B, ci + 1, cj + 1));
+            const S src = apply_sources(P, ctl, S(0), k);
+            const S x[3] = {U0[k], U0[n + k], U0[2 * n + k]};
+            const S rs[3] = {src, S(0), S(0)};
+            S u1[3];
+            stage1_cell<S>(P, x, rs, Bm, dt_d, dt_s, u1);
+            S o[3];
+#pragma unroll
+            for (int v = 0; v < 3; ++v) o[v] = (S)(u1[v] + dt_s * rs[v]);
+#pragma unroll
+            for (int v = 0; v < 3; ++v) o[v] = (S)(0.5 * (x[v] + o[v]));
+            if (P.friction) {
+                const double fs = friction_scalar(P.friction, P.Cz, P.phys.kappa, u1[0], u1[1], u1[2], Bm);
+                apply_friction(o, dt_s, fs);
+            }
+            if (o[0] - Bm < P.phys.cutoff) {
+                o[0] = (S)jl_max((double)o[0], Bm);
+                o[1] = (S)0.0;
+                o[2] = (S)0.0;
+            }
+            OUT[k] = o[0];
+            OUT[n + k] = o[1];
+            OUT[2 * n + k] = o[2];
+        }
+    }
+}
+
+// Rows per wave segment for this grid: the makespan is about the number of
+// rounds of resident waves times the rows each wave marches (segment + 4
+// halo rows), so pick the height in [8, 64] that minimises it.
+int march_rows_per_wave(const KParams& P, int stage) {
+    if (SFV_MARCH_ROWS > 0) return SFV_MARCH_ROWS;
+    static int cus = 0, wpc[2] = {0, 0};
+    if (!cus) {
+        int dev = 0;
+        hipDeviceProp_t prop;
+        if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess)
+            cus = prop.multiProcessorCount;
+        if (cus <= 0) cus = 256;
+        for (int s = 0; s < 2; ++s) {
+            int blocks = 0;
+            const hipError_t e = s == 0 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                                              &blocks, k_march<double, 1>, MW * MWAVES, 0)
+                                        : hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                                              &blocks, k_march<double, 2>, MW * MWAVES, 0);
+            wpc[s] = (e == hipSuccess && blocks > 0) ? blocks * MWAVES : 2 * MWAVES;
+        }
+    }
+    const long slots = (long)cus * wpc[stage == 1 ? 0 : 1];
+    const long strips = (P.nx + MX - 1) / MX;
+    int best = 16;
+    long best_cost = -1;
+    for (int my = 8; my <= 64; ++my) {
+        const long waves = strips * ((P.ny + my - 1) / my);
+        const long rounds = (waves + slots - 1) / slots;
+        const long cost = rounds * (my + 4);
+        if (best_cost < 0 || cost < best_cost) { best_cost = cost; best = my; }
+    }
+    return best;
+}
+
+template <class S>
+hipError_t launch_march(hipStream_t st, const KParams& P, StepCtl* ctl, int slot, int stage, const S* U0,
+                        const S* RHS1, S* OUT, const double* B) {
+    const int my = march_rows_per_wave(P, stage);
+    const int segs = (P.ny + my - 1) / my;
+    const dim3 grid((P.nx + MX - 1) / MX, (segs + MWAVES - 1) / MWAVES);
+    if (stage == 1)
+        hipLaunchKernelGGL((k_march<S, 1>), grid, dim3(MW * MWAVES), 0, st, P, ctl, slot, my, U0, RHS1, OUT, B);
+    else
+        hipLaunchKernelGGL((k_march<S, 2>), grid, dim3(MW * MWAVES), 0, st, P, ctl, slot, my, U0, RHS1, OUT, B);
+    return hipPeekAtLastError();
+}
+template hipError_t launch_march<double>(hipStream_t, const KParams&, StepCtl*, int, int, const double*,
+                                         const double*, double*, const double*);
+template hipError_t launch_march<float>(hipStream_t, const KParams&, StepCtl*, int, int, const float*, const float*,
+                                        float*, const double*);
+
+}  // namespace sfv

@@ -100,7 +100,7 @@ __global__ void __launch_bounds__(256) k_flux(KParams P, StepCtl* ctl, int slot,
                     const double Br = B_face_right(P, B, i, j, d), Bl = B_face_left(P, B, i, j, d);
                     const double dd = d == 0 ? P.dx : P.dy;
                     const S hsum = Rf[1][0] + Lf[1][0];
-                    acc[1 + d] = (S)(acc[1 + d] + -P.phys.g * ((Br - Bl) / dd) * (hsum / 2.0));
+                    acc[1 + d] = (S)(acc[1 + d] + -P.phys.g * slope_div(Br - Bl, dd, inv, d == 0 ? P.dx_pow2 : P.dy_pow2) * (hsum / 2.0));
                 }
             }
         }
@@ -156,9 +156,7 @@ __global__ void __launch_bounds__(256) k_finish(KParams P, StepCtl* ctl, int slo
     const double Bm = B_cell(P, B, i, j);
     if (P.friction) {
         const double fs = friction_scalar(P.friction, P.Cz, P.phys.kappa, x[0], x[1], x[2], Bm);
-        o[0] = (S)(o[0] / (1 - dt_s * 0.0));
-        o[1] = (S)(o[1] / (1 - dt_s * fs));
-        o[2] = (S)(o[2] / (1 - dt_s * fs));
+        apply_friction(o, dt_s, fs);
     }
     if (o[0] - Bm < P.phys.cutoff) {
         o[0] = (S)jl_max((double)o[0], Bm);

@@ -32,11 +32,29 @@ __device__ __forceinline__ T jl_min(T x, T y) {
     return __builtin_signbit(x) ? x : y;
 }
 
-// linearreconstruction.jl:28-35
+// max / min of two values neither of which is -0.0 (wave speeds and
+// eigenvalues: u +- c with c > 0 rounds to +0.0 at worst, dry sides give
+// +0.0): one hardware max/min, and Julia's NaN propagation kept by an
+// unordered check.  For such inputs this equals jl_max / jl_min.
+__device__ __forceinline__ double nz_max(double a, double b) {
+    const double m = __builtin_fmax(a, b);
+    return __builtin_isunordered(a, b) ? a + b : m;
+}
+__device__ __forceinline__ double nz_min(double a, double b) {
+    const double m = __builtin_fmin(a, b);
+    return __builtin_isunordered(a, b) ? a + b : m;
+}
+
+// linearreconstruction.jl:28-35.  The min/max branch is only taken when all
+// three are > 0 (resp. < 0), so no NaN and no signed zero reaches it and the
+// hardware min/max give Julia's min/max; a NaN makes every comparison false
+// and returns +0.0, as the reference does.
 __device__ __forceinline__ double minmod(double a, double b, double c) {
-    if ((a > 0) && (b > 0) && (c > 0)) return jl_min(jl_min(a, b), c);
-    if (a < 0 && b < 0 && c < 0) return jl_max(jl_max(a, b), c);
-    return 0.0;
+    const bool pos = (a > 0) && (b > 0) && (c > 0);
+    const bool neg = (a < 0) && (b < 0) && (c < 0);
+    const double lo = __builtin_fmin(__builtin_fmin(a, b), c);
+    const double hi = __builtin_fmax(__builtin_fmax(a, b), c);
+    return pos ? lo : (neg ? hi : 0.0);
 }
 
 // equation.jl:56 (Brodtkorb & Holm 2021, eq. 26).  When |h| >= κ the inner
@@ -45,10 +63,11 @@ __device__ __forceinline__ double minmod(double a, double b, double c) {
 template <class H>
 __device__ __forceinline__ double desingularize(H h, double kappa) {
     const double ah = (double)__builtin_fabs(h);
-    if (ah >= kappa) return (double)h;
+    if (!(ah < kappa)) return (double)h;          // |h| >= κ, or NaN (which propagates)
     const H h2 = h * h;
     const double sgn = __builtin_signbit(h) ? -1.0 : 1.0;
-    return sgn * jl_max(ah, jl_min(h2 / (2 * kappa) + kappa / 2.0, kappa));
+    // |h| < κ here, and h^2/(2κ) + κ/2 >= κ/2 > 0: no NaN or -0.0 reaches max/min
+    return sgn * __builtin_fmax(ah, __builtin_fmin(h2 / (2 * kappa) + kappa / 2.0, kappa));
 }
 
 struct PhysParams {
@@ -90,8 +109,8 @@ __device__ __forceinline__ double central_upwind(int d, const S m[3], const S q[
     double fp[3] = {0.0, 0.0, 0.0}, ep[2] = {0.0, 0.0};
     if (m[0] > p.cutoff) phys_flux(d, m[0], m[1], m[2], p, fm, em);
     if (q[0] > p.cutoff) phys_flux(d, q[0], q[1], q[2], p, fp, ep);
-    const double aplus = jl_max(jl_max(ep[0], em[0]), 0.0);
-    const double aminus = jl_min(jl_min(ep[1], em[1]), 0.0);
+    const double aplus = nz_max(nz_max(ep[0], em[0]), 0.0);
+    const double aminus = nz_min(nz_min(ep[1], em[1]), 0.0);
     const double da = aplus - aminus;
     if (__builtin_fabs(da) < p.kappa) {
         F[0] = F[1] = F[2] = 0.0;
@@ -104,7 +123,7 @@ __device__ __forceinline__ double central_upwind(int d, const S m[3], const S q[
         F[v] = (aplus * fm[v] - aminus * fp[v]) / da + pm * diff;
     }
     if (m[0] < p.cutoff && q[0] < p.cutoff) return 0.0;
-    return jl_max(__builtin_fabs(aplus), __builtin_fabs(aminus));
+    return nz_max(__builtin_fabs(aplus), __builtin_fabs(aminus));
 }
 
 // minmod_slope + the SWE dry-face slope fix + face values
@@ -142,6 +161,22 @@ __device__ __forceinline__ double friction_law(int law, double c, double h, doub
     return -c * speed / denom;
 }
 
+// implicit_friction's `out ./ (1 .- dt .* [0, f, f])` (friction.jl:77).  For
+// the w component the divisor 1 - dt*0.0 is exactly 1.0 for finite dt and NaN
+// otherwise, so multiplying by it is the same as dividing.
+template <class S>
+__device__ __forceinline__ void apply_friction(S o[3], S dt_s, double fs) {
+    o[0] = (S)(o[0] * (1 - dt_s * 0.0));
+    o[1] = (S)(o[1] / (1 - dt_s * fs));
+    o[2] = (S)(o[2] / (1 - dt_s * fs));
+}
+
+// (B_R - B_L)/Δ of the bottom source (source_term_bottom.jl:38).  When Δ is a
+// power of two, x/Δ and x*(1/Δ) are the same correctly rounded value.
+__device__ __forceinline__ double slope_div(double x, double d, double inv_d, int pow2) {
+    return pow2 ? x * inv_d : x / d;
+}
+
 // friction.jl:62-78: the implicit friction scalar from the substep input
 // state (w, hu, hv) and B_cell.
 template <class S>
@@ -151,6 +186,109 @@ __device__ __forceinline__ double friction_scalar(int law, double Cz, double kap
     double speed = sqrt(u * u + v * v);
     if (speed == 0.0) speed = 0.0;
     return friction_law(law, Cz, hs, speed);
+}
+
+
+// ---------------------------------------------------------------------------
+// Branch-free forms for the fused march kernels.  Each equals the function
+// above for every input; the reference's branches become selects so that a
+// wave's face work is straight-line code the scheduler can interleave.
+
+// minmod slopes + SWE slope fix + face values, with the fix as a select.
+template <class S>
+__device__ __forceinline__ void reconstruct_cell_nb(const S l[3], const S c[3], const S r[3], double theta,
+                                                    double Bl, double Br, S L[3], S R[3]) {
+    double slope[3];
+#pragma unroll
+    for (int v = 0; v < 3; ++v) {
+        const S fwd = r[v] - c[v];
+        const S bwd = c[v] - l[v];
+        const double central = (fwd + bwd) / 2.0;
+        slope[v] = minmod(theta * fwd, central, theta * bwd);
+    }
+    const S w = c[0];
+    const bool fix_l = w - 0.5 * slope[0] < Bl;
+    const bool fix_r = w + 0.5 * slope[0] < Br;
+    const double sl = 2.0 * (w - Bl), sr = 2.0 * (Br - w);
+    slope[0] = fix_l ? sl : (fix_r ? sr : slope[0]);
+#pragma unroll
+    for (int v = 0; v < 3; ++v) {
+        L[v] = (S)(c[v] - 0.5 * slope[v]);
+        R[v] = (S)(c[v] + 0.5 * slope[v]);
+    }
+    L[0] = (S)(L[0] - Bl);
+    R[0] = (S)(R[0] - Br);
+}
+
+// Desingularised depths of N face sides: the |h| >= κ case (h* = h) inline,
+// the rare |h| < κ case for all of them in one branch.
+template <class S, int N>
+__device__ __forceinline__ void desingularize_n(const S* const h[N], double kappa, double hs[N]) {
+    bool slow = false;
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+        hs[k] = (double)*h[k];
+        slow |= (double)__builtin_fabs(*h[k]) < kappa;
+    }
+    if (slow) {
+#pragma unroll
+        for (int k = 0; k < N; ++k)
+            if ((double)__builtin_fabs(*h[k]) < kappa) hs[k] = desingularize(*h[k], kappa);
+    }
+}
+
+// centralupwind.jl:48-77 for direction D with the desingularised depths of
+// both sides given: both sides are evaluated and dry sides (h <= cutoff)
+// select zero flux and eigenvalues, exactly as the reference's zero(...)
+// initialisers; the degenerate |a+ - a-| < κ and both-dry cases select the
+// reference's early returns.
+template <int D, class S>
+__device__ __forceinline__ double central_upwind_nb(const S m[3], const S q[3], double hsm, double hsq,
+                                                    const PhysParams& p, double F[3]) {
+    const bool wm = m[0] > p.cutoff, wq = q[0] > p.cutoff;
+    double fm[3], fp[3], em[2], ep[2];
+    {
+        const double u = m[1] / hsm, v = m[2] / hsm;
+        const S h2 = m[0] * m[0];
+        const double c = sqrt(p.g * m[0]);
+        const double a = D == 0 ? u : v;
+        const double f0 = p.rho * m[0] * a;
+        const double f1 = D == 0 ? p.rho * m[0] * u * u + 0.5 * p.rho * p.g * h2 : p.rho * m[0] * u * v;
+        const double f2 = D == 0 ? p.rho * m[0] * u * v : p.rho * m[0] * v * v + 0.5 * p.rho * p.g * h2;
+        fm[0] = wm ? f0 : 0.0;
+        fm[1] = wm ? f1 : 0.0;
+        fm[2] = wm ? f2 : 0.0;
+        em[0] = wm ? a + c : 0.0;
+        em[1] = wm ? a - c : 0.0;
+    }
+    {
+        const double u = q[1] / hsq, v = q[2] / hsq;
+        const S h2 = q[0] * q[0];
+        const double c = sqrt(p.g * q[0]);
+        const double a = D == 0 ? u : v;
+        const double f0 = p.rho * q[0] * a;
+        const double f1 = D == 0 ? p.rho * q[0] * u * u + 0.5 * p.rho * p.g * h2 : p.rho * q[0] * u * v;
+        const double f2 = D == 0 ? p.rho * q[0] * u * v : p.rho * q[0] * v * v + 0.5 * p.rho * p.g * h2;
+        fp[0] = wq ? f0 : 0.0;
+        fp[1] = wq ? f1 : 0.0;
+        fp[2] = wq ? f2 : 0.0;
+        ep[0] = wq ? a + c : 0.0;
+        ep[1] = wq ? a - c : 0.0;
+    }
+    const double aplus = nz_max(nz_max(ep[0], em[0]), 0.0);
+    const double aminus = nz_min(nz_min(ep[1], em[1]), 0.0);
+    const double da = aplus - aminus;
+    const bool degenerate = __builtin_fabs(da) < p.kappa;
+    const double pm = (aplus * aminus) / da;
+#pragma unroll
+    for (int v = 0; v < 3; ++v) {
+        const S diff = q[v] - m[v];
+        const double f = (aplus * fm[v] - aminus * fp[v]) / da + pm * diff;
+        F[v] = degenerate ? 0.0 : f;
+    }
+    const bool both_dry = m[0] < p.cutoff && q[0] < p.cutoff;
+    const double sp = nz_max(__builtin_fabs(aplus), __builtin_fabs(aminus));
+    return (degenerate || both_dry) ? 0.0 : sp;
 }
 
 }  // namespace sfv

@@ -110,9 +110,15 @@ class Simulator:
             raise C.DomainError(C.SFV_ERR_DOMAIN, f"state should be of size {self._state_shape()} but got {U.shape}")
         C.check(self._h, C.lib().sfv_set_state(self._h, U.ctypes.data_as(ctypes.c_void_p), C.SFV_LAYOUT_FULL))
 
+    KERNELS = {"per-pass": 0, "tiled": 1, "march": 2}
+
+    def set_kernels(self, variant: str) -> None:
+        """Choose the kernels of an RK2 step: "march" (default), "tiled" or
+        "per-pass".  All give bitwise identical results."""
+        C.check(self._h, C.lib().sfv_set_fused(self._h, self.KERNELS[variant]))
+
     def set_fused(self, fused: bool) -> None:
-        """Choose the fused stage kernels or the per-pass kernels (same results)."""
-        C.check(self._h, C.lib().sfv_set_fused(self._h, 1 if fused else 0))
+        self.set_kernels("march" if fused else "per-pass")
 
     def _step_status(self, rc: int) -> None:
         if rc == C.SFV_ERR_NONFINITE_DT:
