@@ -62,6 +62,7 @@ def lib():
         L.oracle2d_simulate_to_time.argtypes = [c_void_p, d, c_int, d, c_int64, POINTER(d)]
         L.oracle2d_simulate_to_time.restype = c_int64
         L.oracle2d_advance.argtypes = [c_void_p, c_int64, d, POINTER(d)]
+        L.oracle2d_apply_bc.argtypes = [c_void_p]
         L.oracle2d_B_lookup.argtypes = [c_void_p, c_int, c_int, c_int, c_int]
         L.oracle2d_B_lookup.restype = d
         L.oracle1d_create.argtypes = [cfgp]
@@ -114,6 +115,9 @@ class Oracle2D:
 
     def B_lookup(self, which: int, i: int, j: int, d: int = 0) -> float:
         return lib().oracle2d_B_lookup(self._h, which, i, j, d)
+
+    def apply_bc(self) -> None:
+        lib().oracle2d_apply_bc(self._h)
 
     def set_interior(self, U):
         U = np.asfortranarray(U, dtype=np.float64)

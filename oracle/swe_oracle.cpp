@@ -162,6 +162,13 @@ double oracle2d_B_lookup(void* hp, int which, int i, int j, int d) {
     return which == 0 ? h->f->B_cell(i, j) : which == 1 ? h->f->B_face_left(i, j, d) : h->f->B_face_right(i, j, d);
 }
 
+// update_bc! on the current state (bc.jl:29-110).
+void oracle2d_apply_bc(void* hp) {
+    Handle2D* h = (Handle2D*)hp;
+    if (h->d) h->d->update_bc(h->d->out[0]);
+    else h->f->update_bc(h->f->out[0]);
+}
+
 // ---- 1D simulator (Float64)
 void* oracle1d_create(const sfv_config* cfg) { return new Sim1D(*cfg); }
 void oracle1d_destroy(void* hp) { delete (Sim1D*)hp; }
