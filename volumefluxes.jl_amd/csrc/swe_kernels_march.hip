@@ -33,6 +33,7 @@ namespace sfv {
 // Build-time tuning knobs (defaults are the measured best on MI355X):
 //   SFV_MARCH_PREFETCH    issue the next row's loads one iteration early
 //   SFV_MARCH_BRANCHFREE  evaluate the x and y faces with the branch-free forms
+//   SFV_MARCH_WETFAST     straight-line face evaluation for all-wet waves
 //   SFV_MARCH_ROWS        rows per wave segment (0: chosen per grid)
 //   SFV_MARCH_MIN_WAVES   minimum waves per SIMD the register allocation must allow
 #ifndef SFV_MARCH_PREFETCH
@@ -41,8 +42,11 @@ namespace sfv {
 #ifndef SFV_MARCH_BRANCHFREE
 #define SFV_MARCH_BRANCHFREE 0
 #endif
+#ifndef SFV_MARCH_WETFAST
+#define SFV_MARCH_WETFAST 0
+#endif
 #ifndef SFV_MARCH_ROWS
-#define SFV_MARCH_ROWS 16
+#define SFV_MARCH_ROWS 12
 #endif
 #ifndef SFV_MARCH_MIN_WAVES
 #define SFV_MARCH_MIN_WAVES 1
@@ -52,10 +56,25 @@ constexpr int MW = 64;          // lanes
 constexpr int MX = 60;          // output columns per wave
 constexpr int MWAVES = 4;       // waves per block
 
+// Lane shifts by one across the whole wave with DPP (wave_shr:1 / wave_shl:1,
+// VALU latency instead of an LDS round trip): lane i receives lane i-1
+// (shfl_up1) or lane i+1 (shfl_dn1); lane 0 / lane 63 keep their own value.
+template <int CTRL>
+__device__ __forceinline__ int dpp_shift(int v) { return __builtin_amdgcn_update_dpp(v, v, CTRL, 0xf, 0xf, false); }
+template <int CTRL>
+__device__ __forceinline__ double dpp_shift(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = dpp_shift<CTRL>((int)(unsigned long long)b);
+    const int hi = dpp_shift<CTRL>((int)((unsigned long long)b >> 32));
+    return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+template <int CTRL>
+__device__ __forceinline__ float dpp_shift(float v) { return __int_as_float(dpp_shift<CTRL>(__float_as_int(v))); }
+constexpr int DPP_WAVE_SHR1 = 0x138, DPP_WAVE_SHL1 = 0x130;
 template <class T>
-__device__ __forceinline__ T shfl_up1(T v) { return __shfl_up(v, 1, MW); }
+__device__ __forceinline__ T shfl_up1(T v) { return dpp_shift<DPP_WAVE_SHR1>(v); }
 template <class T>
-__device__ __forceinline__ T shfl_dn1(T v) { return __shfl_down(v, 1, MW); }
+__device__ __forceinline__ T shfl_dn1(T v) { return dpp_shift<DPP_WAVE_SHL1>(v); }
 
 __device__ __forceinline__ unsigned long long wave_max_u64m(unsigned long long v) {
 #pragma unroll
@@ -249,6 +268,9 @@ k_march(KParams P, StepCtl* ctl, int slot, int my,
             desingularize_n<S, 4>(hp, P.phys.kappa, hs);
             const double spy = central_upwind_nb<1>(Ryp, Ly, hs[0], hs[1], P.phys, Fy);
             const double spx = central_upwind_nb<0>(Rxl, Lx, hs[2], hs[3], P.phys, Fxm);
+#elif SFV_MARCH_WETFAST
+            const double spy = central_upwind_auto<1>(Ryp, Ly, P.phys, Fy);
+            const double spx = central_upwind_auto<0>(Rxl, Lx, P.phys, Fxm);
 #else
             const double spy = central_upwind(1, Ryp, Ly, P.phys, Fy);
             const double spx = central_upwind(0, Rxl, Lx, P.phys, Fxm);
@@ -335,6 +357,7 @@ k_march(KParams P, StepCtl* ctl, int slot, int my,
 
     const int rmain = min(4, th + 4);
     for (int r = 0; r < rmain; ++r) body(r, std::false_type{});
+#pragma unroll 2
     for (int r = rmain; r < th + 4; ++r) body(r, std::true_type{});
 
     if (STAGE == 1) {

@@ -36,25 +36,22 @@ __device__ __forceinline__ T jl_min(T x, T y) {
 // eigenvalues: u +- c with c > 0 rounds to +0.0 at worst, dry sides give
 // +0.0): one hardware max/min, and Julia's NaN propagation kept by an
 // unordered check.  For such inputs this equals jl_max / jl_min.
-__device__ __forceinline__ double nz_max(double a, double b) {
-    const double m = __builtin_fmax(a, b);
-    return __builtin_isunordered(a, b) ? a + b : m;
-}
-__device__ __forceinline__ double nz_min(double a, double b) {
-    const double m = __builtin_fmin(a, b);
-    return __builtin_isunordered(a, b) ? a + b : m;
-}
+__device__ __forceinline__ double nz_max(double a, double b) { return (a > b || a != a) ? a : b; }
+__device__ __forceinline__ double nz_min(double a, double b) { return (a < b || a != a) ? a : b; }
 
-// linearreconstruction.jl:28-35.  The min/max branch is only taken when all
-// three are > 0 (resp. < 0), so no NaN and no signed zero reaches it and the
-// hardware min/max give Julia's min/max; a NaN makes every comparison false
-// and returns +0.0, as the reference does.
-__device__ __forceinline__ double minmod(double a, double b, double c) {
-    const bool pos = (a > 0) && (b > 0) && (c > 0);
-    const bool neg = (a < 0) && (b < 0) && (c < 0);
+// linearreconstruction.jl:28-35: min(a,b,c) if all > 0, max(a,b,c) if all
+// < 0, else +0.0 (also when any is NaN, as every comparison is then false).
+// Evaluated as max(lo, 0) + min(hi, 0) (+0.0) with lo/hi the hardware
+// min/max: all > 0 gives lo, all < 0 gives hi, mixed signs or zeros give a
+// sum of two zeros, and the trailing +0.0 turns -0.0 + -0.0 into +0.0.
+// NaN: with `regular` (θ finite and nonzero) a NaN in a or c implies one in b
+// (a = θ·fwd, c = θ·bwd, b = (fwd + bwd)/2), so checking b suffices.
+__device__ __forceinline__ double minmod(double a, double b, double c, bool regular = false) {
     const double lo = __builtin_fmin(__builtin_fmin(a, b), c);
     const double hi = __builtin_fmax(__builtin_fmax(a, b), c);
-    return pos ? lo : (neg ? hi : 0.0);
+    const double r = (__builtin_fmax(lo, 0.0) + __builtin_fmin(hi, 0.0)) + 0.0;
+    const bool nan = regular ? (b != b) : (__builtin_isunordered(a, b) || c != c);
+    return nan ? 0.0 : r;
 }
 
 // equation.jl:56 (Brodtkorb & Holm 2021, eq. 26).  When |h| >= κ the inner
@@ -133,13 +130,14 @@ __device__ __forceinline__ double central_upwind(int d, const S m[3], const S q[
 template <class S>
 __device__ __forceinline__ void reconstruct_cell(const S l[3], const S c[3], const S r[3], double theta,
                                                  double Bl, double Br, S L[3], S R[3]) {
+    const bool regular = __builtin_isfinite(theta) && theta != 0.0;
     double slope[3];
 #pragma unroll
     for (int v = 0; v < 3; ++v) {
         const S fwd = r[v] - c[v];
         const S bwd = c[v] - l[v];
         const double central = (fwd + bwd) / 2.0;
-        slope[v] = minmod(theta * fwd, central, theta * bwd);
+        slope[v] = minmod(theta * fwd, central, theta * bwd, regular);
     }
     const S w = c[0];
     if (w - 0.5 * slope[0] < Bl) slope[0] = 2.0 * (w - Bl);
@@ -198,13 +196,14 @@ __device__ __forceinline__ double friction_scalar(int law, double Cz, double kap
 template <class S>
 __device__ __forceinline__ void reconstruct_cell_nb(const S l[3], const S c[3], const S r[3], double theta,
                                                     double Bl, double Br, S L[3], S R[3]) {
+    const bool regular = __builtin_isfinite(theta) && theta != 0.0;
     double slope[3];
 #pragma unroll
     for (int v = 0; v < 3; ++v) {
         const S fwd = r[v] - c[v];
         const S bwd = c[v] - l[v];
         const double central = (fwd + bwd) / 2.0;
-        slope[v] = minmod(theta * fwd, central, theta * bwd);
+        slope[v] = minmod(theta * fwd, central, theta * bwd, regular);
     }
     const S w = c[0];
     const bool fix_l = w - 0.5 * slope[0] < Bl;
@@ -289,6 +288,63 @@ __device__ __forceinline__ double central_upwind_nb(const S m[3], const S q[3], 
     const bool both_dry = m[0] < p.cutoff && q[0] < p.cutoff;
     const double sp = nz_max(__builtin_fabs(aplus), __builtin_fabs(aminus));
     return (degenerate || both_dry) ? 0.0 : sp;
+}
+
+// centralupwind.jl:48-77 for direction D when both sides are wet and
+// |h| >= κ on both (so h* = h, equation.jl:56): the reference's branches are
+// all decided, and what is left is straight-line code — the two sides and the
+// four quotients by a+ - a- are independent chains.  Only the degenerate
+// |a+ - a-| < κ return remains, as a select.
+template <int D, class S>
+__device__ __forceinline__ double central_upwind_wet(const S m[3], const S q[3], const PhysParams& p, double F[3]) {
+    const double um = m[1] / (double)m[0], vm = m[2] / (double)m[0];
+    const double uq = q[1] / (double)q[0], vq = q[2] / (double)q[0];
+    const S hm2 = m[0] * m[0], hq2 = q[0] * q[0];
+    const double cm = sqrt(p.g * m[0]), cq = sqrt(p.g * q[0]);
+    const double am = D == 0 ? um : vm, aq = D == 0 ? uq : vq;
+    const double rgh = 0.5 * p.rho * p.g;
+    double fm[3], fp[3];
+    fm[0] = p.rho * m[0] * am;
+    fp[0] = p.rho * q[0] * aq;
+    if (D == 0) {
+        fm[1] = p.rho * m[0] * um * um + rgh * hm2;
+        fm[2] = p.rho * m[0] * um * vm;
+        fp[1] = p.rho * q[0] * uq * uq + rgh * hq2;
+        fp[2] = p.rho * q[0] * uq * vq;
+    } else {
+        fm[1] = p.rho * m[0] * um * vm;
+        fm[2] = p.rho * m[0] * vm * vm + rgh * hm2;
+        fp[1] = p.rho * q[0] * uq * vq;
+        fp[2] = p.rho * q[0] * vq * vq + rgh * hq2;
+    }
+    const double aplus = nz_max(nz_max(aq + cq, am + cm), 0.0);
+    const double aminus = nz_min(nz_min(aq - cq, am - cm), 0.0);
+    const double da = aplus - aminus;
+    const bool degenerate = __builtin_fabs(da) < p.kappa;
+    const double pm = (aplus * aminus) / da;
+#pragma unroll
+    for (int v = 0; v < 3; ++v) {
+        const S diff = q[v] - m[v];
+        const double f = (aplus * fm[v] - aminus * fp[v]) / da + pm * diff;
+        F[v] = degenerate ? 0.0 : f;
+    }
+    const double sp = nz_max(__builtin_fabs(aplus), __builtin_fabs(aminus));
+    return degenerate ? 0.0 : sp;
+}
+
+// Both sides of a face wet with |h| >= κ: central_upwind_wet applies.
+template <class S>
+__device__ __forceinline__ bool face_is_wet(const S m[3], const S q[3], const PhysParams& p) {
+    return m[0] > p.cutoff && q[0] > p.cutoff && (double)__builtin_fabs(m[0]) >= p.kappa &&
+           (double)__builtin_fabs(q[0]) >= p.kappa;
+}
+
+// One face, with the straight-line evaluation when the whole wave's faces are
+// wet (a wave-uniform branch) and the general evaluation otherwise.
+template <int D, class S>
+__device__ __forceinline__ double central_upwind_auto(const S m[3], const S q[3], const PhysParams& p, double F[3]) {
+    if (__all(face_is_wet(m, q, p))) return central_upwind_wet<D>(m, q, p, F);
+    return central_upwind(D, m, q, p, F);
 }
 
 }  // namespace sfv
