@@ -102,7 +102,7 @@ def main():
     ap.add_argument("--kernels", choices=["march", "tiled", "per-pass"], default="march",
                     help="kernel variant of the RK2 step (all bitwise identical)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--cpu-steps", type=int, default=4, help="steps per timed CPU run")
+    ap.add_argument("--cpu-steps", type=int, default=48, help="steps per timed CPU run")
     ap.add_argument("--cpu-runs", type=int, default=3)
     args = ap.parse_args()
 
@@ -143,10 +143,13 @@ def main():
     if args.kernels == "per-pass":       # the stage kernels timed are the fused march ones
         timing_sim = VF.Simulator(backend, system, VF.RungeKutta2(), grid, cfl=0.25)
         VF.set_current_state(timing_sim, U0)
-    for k in ("stage1", "stage2", "bc", "step"):
+    for k in ("stage1", "stage2", "bc", "step", "copy"):
         out = ctypes.c_double()
         C.check(timing_sim.handle, L.sfv_time_kernel(timing_sim.handle, k.encode(), 20, ctypes.byref(out)))
         ms[k] = out.value
+    elem = 8 if args.precision == "f64" else 4
+    copy_bytes = 2 * 3 * (grid.nx + 2 * grid.gc) * (grid.ny + 2 * grid.gc) * elem   # read + write of one state
+    copy_gbs = copy_bytes / (ms["copy"] * 1e-3) / 1e9
     dominant = max(("stage1", "stage2"), key=lambda k: ms[k])
     alg_bytes = BYTES_PER_CELL_STAGE[args.precision][dominant] * cells
     achieved = alg_bytes / (ms[dominant] * 1e-3) / 1e9
@@ -185,6 +188,7 @@ def main():
             "traffic": traffic,
             "algorithmic_bytes_per_launch": alg_bytes,
             "kernel_ms": ms,
+            "copy_kernel_gbs": copy_gbs,
         },
         "step_roofline": {"bytes_per_cell_update": BYTES_PER_CELL_STEP[args.precision],
                           "achieved_gbs": step_gbs, "frac": step_gbs / HBM_PEAK_GBS},
@@ -199,11 +203,13 @@ def main():
             pass
         first_cpu, cpu_value, cpu_med = cpu_baseline(grid, system, realtype, U0, B, args.cpu_steps, args.cpu_runs,
                                                      threads)
+        _, cpu1_value, _ = cpu_baseline(grid, system, realtype, U0, B, 2, 1, 1)
         result["cpu_baseline"] = {
             "value": cpu_value, "unit": "cell-updates/s", "cores": threads, "kind": "port",
             "sample": (f"C++ oracle (loop-faithful restatement of the reference CPU path), same workload: "
                        f"median of {args.cpu_runs} runs of {args.cpu_steps} RK2 steps after 1 warm-up step, "
                        f"OpenMP over rows"),
+            "single_thread_value": cpu1_value,
         }
         result["parity_first_step"] = "bitwise" if np.array_equal(first_gpu, first_cpu) else "MISMATCH"
         result["gpu_over_cpu"] = value / cpu_value

@@ -365,8 +365,8 @@ int sfv_time_kernel(sfv_handle* h, const char* which, int32_t n, double* ms_out)
     if (h->cfg.stepper != SFV_RUNGE_KUTTA2 || !h->fusable())
         return fail(h, SFV_ERR_ARGUMENT, "kernel timing needs RungeKutta2 with fused stage kernels");
     const std::string w(which);
-    if (w != "stage1" && w != "stage2" && w != "bc" && w != "step")
-        return fail(h, SFV_ERR_ARGUMENT, "which must be stage1, stage2, bc or step");
+    if (w != "stage1" && w != "stage2" && w != "bc" && w != "step" && w != "copy")
+        return fail(h, SFV_ERR_ARGUMENT, "which must be stage1, stage2, bc, step or copy");
     HIP_TRY(h, hipSetDevice(h->device));
     const int slot = (int)(h->step_parity & 1), cur = (int)(h->cur & 1);
     // Time the kernels against a scratch output so the state is left as is.
@@ -394,6 +394,7 @@ int sfv_time_kernel(sfv_handle* h, const char* which, int32_t n, double* ms_out)
         return e;
     };
     auto one = [&]() -> hipError_t {
+        if (w == "copy") return launch_copy(h->stream, h->U[cur], h->U[2], 3 * h->ncell * h->elem);
         if (h->cfg.precision == SFV_F64)
             return run((double*)h->U[cur], (double*)h->RHS, (double*)h->U[2]);
         return run((float*)h->U[cur], (float*)h->RHS, (float*)h->U[2]);

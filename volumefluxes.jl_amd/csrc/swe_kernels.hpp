@@ -44,6 +44,8 @@ hipError_t launch_bc(hipStream_t st, const KParams& P, StepCtl* ctl, int slot, S
 hipError_t launch_prepare(hipStream_t st, const KParams& P, StepCtl* ctl, int slot, double t, double endtime,
                           int match_endtime, int has_max_timestep, double max_timestep, long long max_steps,
                           int until_endtime, int precision);
+// Plain copy of `bytes` (a multiple of 4) from src to dst.
+hipError_t launch_copy(hipStream_t st, const void* src, void* dst, size_t bytes);
 template <class S>
 hipError_t launch_stage1(hipStream_t st, const KParams& P, StepCtl* ctl, int slot, const S* U0, S* RHS,
                          const double* B);

@@ -116,6 +116,32 @@ hipError_t launch_prepare(hipStream_t st, const KParams& P, StepCtl* ctl, int sl
                        max_timestep, max_steps, until_endtime, precision);
     return hipPeekAtLastError();
 }
+// A plain device copy of `words` 4-byte words (16 bytes per thread and
+// iteration, grid-stride), used as the known-traffic reference kernel when
+// HBM byte counters are calibrated.  Both pointers are hipMalloc results
+// (256-byte aligned).
+__global__ void __launch_bounds__(256) k_copy_words(const unsigned* __restrict__ src, unsigned* __restrict__ dst,
+                                                    size_t words) {
+    const size_t n16 = words / 4;
+    const size_t tid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    const uint4* s4 = reinterpret_cast<const uint4*>(src);
+    uint4* d4 = reinterpret_cast<uint4*>(dst);
+    for (size_t i = tid; i < n16; i += stride) d4[i] = s4[i];
+    if (tid < words - 4 * n16) dst[4 * n16 + tid] = src[4 * n16 + tid];
+}
+
+hipError_t launch_copy(hipStream_t st, const void* src, void* dst, size_t bytes) {
+    const size_t words = bytes / 4;
+    const size_t n16 = words / 4;
+    size_t blocks = (n16 + 255) / 256;
+    if (blocks < 1) blocks = 1;
+    if (blocks > 16384) blocks = 16384;
+    hipLaunchKernelGGL(k_copy_words, dim3((unsigned)blocks), dim3(256), 0, st, (const unsigned*)src, (unsigned*)dst,
+                       words);
+    return hipPeekAtLastError();
+}
+
 template hipError_t launch_bc<double>(hipStream_t, const KParams&, StepCtl*, int, double*, int);
 template hipError_t launch_bc<float>(hipStream_t, const KParams&, StepCtl*, int, float*, int);
 

@@ -36,6 +36,9 @@ namespace sfv {
 //   SFV_MARCH_WETFAST     straight-line face evaluation for all-wet waves
 //   SFV_MARCH_ROWS        rows per wave segment (0: chosen per grid)
 //   SFV_MARCH_MIN_WAVES   minimum waves per SIMD the register allocation must allow
+//   SFV_MARCH_XCD         block order: 1 = each XCD gets a contiguous run of tiles,
+//                         segments fastest (vertical halos stay in one L2);
+//                         2 = same, strips fastest; 0 = hardware order
 #ifndef SFV_MARCH_PREFETCH
 #define SFV_MARCH_PREFETCH 0
 #endif
@@ -50,6 +53,9 @@ namespace sfv {
 #endif
 #ifndef SFV_MARCH_MIN_WAVES
 #define SFV_MARCH_MIN_WAVES 1
+#endif
+#ifndef SFV_MARCH_XCD
+#define SFV_MARCH_XCD 2
 #endif
 
 constexpr int MW = 64;          // lanes
@@ -108,20 +114,22 @@ __device__ __forceinline__ void stage1_cell(const KParams& P, const S x[3], cons
     }
 }
 
+// One wave's work: output columns gc + 60*strip .. +59 (lanes 2..61) of rows
+// gc + seg*my .. +my-1.  Shared by the single-grid kernel (strip, seg from
+// the block index) and the batched kernel (from the batch's tile table).
+// Has no barriers: every wave is independent.
 template <class S, int STAGE>
-__global__ void __launch_bounds__(MW* MWAVES) __attribute__((amdgpu_waves_per_eu(SFV_MARCH_MIN_WAVES)))
-k_march(KParams P, StepCtl* ctl, int slot, int my,
-                                                      const S* __restrict__ U0, const S* __restrict__ RHS1,
-                                                      S* __restrict__ OUT, const double* __restrict__ B) {
+__device__ __forceinline__ void march_tile(const KParams& P, StepCtl* ctl, int slot, int my, int strip, int seg,
+                                           const S* __restrict__ U0, const S* __restrict__ RHS1,
+                                           S* __restrict__ OUT, const double* __restrict__ B) {
     const StepSlot& sl = ctl->slot[slot];
     if (!sl.active) return;
-    const int lane = threadIdx.x & (MW - 1), wv = threadIdx.x / MW;
-    const int seg = blockIdx.y * MWAVES + wv;
+    const int lane = threadIdx.x & (MW - 1);
     const int gc = P.gc, nxT = P.nxT, nyT = P.nyT;
     const int j0 = gc + seg * my;
-    if (j0 >= nyT - gc) return;                    // whole wave; the kernel has no barriers
+    if (j0 >= nyT - gc) return;                    // whole wave
     const int th = min(my, nyT - gc - j0);
-    const int i = gc + blockIdx.x * MX + lane - 2; // this lane's column (>= 0 since gc >= 2)
+    const int i = gc + strip * MX + lane - 2;      // this lane's column (>= 0 since gc >= 2)
     const bool in_dom = i < nxT;
     const bool out_col = lane >= 2 && lane < MX + 2 && i < nxT - gc;
     const size_t n = (size_t)nxT * nyT;
@@ -131,7 +139,7 @@ k_march(KParams P, StepCtl* ctl, int slot, int my,
     if (STAGE == 2) {
         dt_d = stage1_dt<S>(P, ctl, sl);
         dt_s = (S)dt_d;
-        if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) {
+        if (strip == 0 && seg == 0 && lane == 0) {
             ctl->dt = (double)dt_s;
             ctl->dt_stage1 = dt_d;
         }
@@ -378,7 +386,7 @@ k_march(KParams P, StepCtl* ctl, int slot, int my,
     // The gc x gc corner cells of the domain: never written by the boundary
     // condition (bc.jl:92-109) nor read by a stencil, but part of the state;
     // both stages run sources, RK update, friction and clamp on them.
-    if (STAGE == 2 && blockIdx.x == 0 && blockIdx.y == 0 && wv == 0) {
+    if (STAGE == 2 && strip == 0 && seg == 0) {
         for (int e = lane; e < 4 * gc * gc; e += MW) {
             const int corner = e / (gc * gc), rr = e % (gc * gc);
             const int ci = (corner & 1) ? nxT - gc + rr % gc : rr % gc;
@@ -410,6 +418,37 @@ k_march(KParams P, StepCtl* ctl, int slot, int my,
             OUT[2 * n + k] = o[2];
         }
     }
+}
+
+// The hardware hands workgroup L of a launch to XCD L mod 8, and each XCD
+// has its own L2.  Block L therefore takes logical tile
+//   prefix(L mod 8) + L div 8,
+// so the blocks of one XCD work on one contiguous run of tiles, where
+// neighbouring segments (which share 4 halo rows) are resident together.
+constexpr int N_XCD = 8;
+__device__ __forceinline__ int xcd_tile(int L, int N) {
+#if SFV_MARCH_XCD
+    const int x = L % N_XCD, k = L / N_XCD, q = N / N_XCD, r = N % N_XCD;
+    return x * q + min(x, r) + k;
+#else
+    (void)N;
+    return L;
+#endif
+}
+
+// 1D grid of strips x seggroups blocks; a block is 4 waves on 4 consecutive
+// segments of one strip.
+template <class S, int STAGE>
+__global__ void __launch_bounds__(MW* MWAVES) __attribute__((amdgpu_waves_per_eu(SFV_MARCH_MIN_WAVES)))
+k_march(KParams P, StepCtl* ctl, int slot, int my, int strips, int seggroups, const S* __restrict__ U0,
+        const S* __restrict__ RHS1, S* __restrict__ OUT, const double* __restrict__ B) {
+    const int t = xcd_tile(blockIdx.x, strips * seggroups);
+#if SFV_MARCH_XCD == 2
+    const int strip = t % strips, sg = t / strips;
+#else
+    const int strip = t / seggroups, sg = t % seggroups;
+#endif
+    march_tile<S, STAGE>(P, ctl, slot, my, strip, sg * MWAVES + threadIdx.x / MW, U0, RHS1, OUT, B);
 }
 
 // Rows per wave segment for this grid: the makespan is about the number of
@@ -451,11 +490,14 @@ hipError_t launch_march(hipStream_t st, const KParams& P, StepCtl* ctl, int slot
                         const S* RHS1, S* OUT, const double* B) {
     const int my = march_rows_per_wave(P, stage);
     const int segs = (P.ny + my - 1) / my;
-    const dim3 grid((P.nx + MX - 1) / MX, (segs + MWAVES - 1) / MWAVES);
+    const int strips = (P.nx + MX - 1) / MX, seggroups = (segs + MWAVES - 1) / MWAVES;
+    const dim3 grid(strips * seggroups);
     if (stage == 1)
-        hipLaunchKernelGGL((k_march<S, 1>), grid, dim3(MW * MWAVES), 0, st, P, ctl, slot, my, U0, RHS1, OUT, B);
+        hipLaunchKernelGGL((k_march<S, 1>), grid, dim3(MW * MWAVES), 0, st, P, ctl, slot, my, strips, seggroups, U0,
+                           RHS1, OUT, B);
     else
-        hipLaunchKernelGGL((k_march<S, 2>), grid, dim3(MW * MWAVES), 0, st, P, ctl, slot, my, U0, RHS1, OUT, B);
+        hipLaunchKernelGGL((k_march<S, 2>), grid, dim3(MW * MWAVES), 0, st, P, ctl, slot, my, strips, seggroups, U0,
+                           RHS1, OUT, B);
     return hipPeekAtLastError();
 }
 template hipError_t launch_march<double>(hipStream_t, const KParams&, StepCtl*, int, int, const double*,
