@@ -91,6 +91,97 @@ def cpu_baseline(grid, system, realtype, U0, B, steps_per_run: int, runs: int, t
     return first, grid.nx * grid.ny * steps_per_run / med, med
 
 
+ENSEMBLE_SIDES = (128, 256, 512, 768, 1024)
+
+
+def ensemble_scenarios(members: int, seed: int):
+    """The ensemble workload: grid sides drawn per instance (and per axis)
+    from ENSEMBLE_SIDES, each instance with its own seeded bumps, rain rate
+    U[0, 5e-5] m/s and Manning n U[0.01, 0.05] (Cz = n^2); f64, RK2, walls."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(members):
+        nx, ny = (int(v) for v in rng.choice(ENSEMBLE_SIDES, 2))
+        rain = float(rng.uniform(0.0, 5e-5))
+        n_manning = float(rng.uniform(0.01, 0.05))
+        grid = VF.CartesianGrid(nx, ny, gc=2, boundary=VF.WallBC(), extent=[[0.0, float(nx)], [0.0, float(ny)]])
+        B = terrain_topography(grid, seed=seed * 1000 + k, n_bumps=32, slope=1e-3, amp=(0.5, 5.0),
+                               width=(20.0, 200.0))
+        eq = VF.ShallowWaterEquations(VF.BottomTopography2D(B, None, grid), desingularizing_kappa=1e-4)
+        system = VF.ConservedSystem(None, VF.LinearReconstruction(), VF.CentralUpwind(eq), eq, grid,
+                                    [VF.SourceTermBottom(), VF.ConstantRain(rain * 3600)],
+                                    VF.ImplicitFriction(Cz=n_manning * n_manning))
+        out.append((grid, system, terrain_initial_state(grid, B, 2.0)))
+    return out
+
+
+def ensemble_bench(members: int, steps: int, seed: int):
+    """Batched (one persistent launch per RK stage for all members, graph
+    replays) against plain (the same instances one at a time, each a
+    device-resident advance_steps), all submitted at t = 0.  Time-to-result
+    of an instance is when its final state is ready; p50/p99 over instances."""
+    scen = ensemble_scenarios(members, seed)
+    backend = VF.make_hip_backend(np.float64)
+
+    def sims():
+        out = []
+        for grid, system, U0 in scen:
+            sim = VF.Simulator(backend, system, VF.RungeKutta2(), grid, cfl=0.25)
+            VF.set_current_state(sim, U0)
+            out.append(sim)
+        return out
+
+    L = C.lib()
+    cells = sum(g.nx * g.ny for g, _, _ in scen)
+    # warm-up of both paths on throw-away copies of the first members
+    warm = sims()[:8]
+    for sim in warm:
+        VF.advance_steps(sim, 2)
+    ew = VF.Ensemble(warm)
+    ew.advance_steps(2)
+    ew.close()
+    del warm
+
+    plain = sims()
+    for sim in plain:
+        C.check(sim.handle, L.sfv_synchronize(sim.handle))
+    done_plain = []
+    t0 = time.perf_counter()
+    for sim in plain:
+        VF.advance_steps(sim, steps)
+        done_plain.append(time.perf_counter() - t0)
+    plain_s = done_plain[-1]
+
+    batched = sims()
+    for sim in batched:
+        C.check(sim.handle, L.sfv_synchronize(sim.handle))
+    ens = VF.Ensemble(batched)
+    t0 = time.perf_counter()
+    ens.advance_steps(steps)
+    batched_s = time.perf_counter() - t0
+    done_batched = [batched_s] * members
+
+    picks = sorted(set([0, members // 3, members // 2, members - 1] + list(range(0, members, max(1, members // 8)))))
+    same = all(np.array_equal(plain[k].get_full_state(), batched[k].get_full_state()) and plain[k].t == batched[k].t
+               for k in picks)
+    ens.close()
+
+    def pct(v, q):
+        return float(np.percentile(np.asarray(v), q))
+    return {
+        "workload": (f"{members} instances, nx and ny each drawn from {list(ENSEMBLE_SIDES)}, own seeded bumps, "
+                     f"rain U[0,5e-5] m/s, Manning n U[0.01,0.05], f64 RK2, {steps} steps each, own dt"),
+        "cells_total": cells,
+        "batched": {"cell_updates_per_s": cells * steps / batched_s, "seconds": batched_s,
+                    "p50_s": pct(done_batched, 50), "p99_s": pct(done_batched, 99)},
+        "plain": {"cell_updates_per_s": cells * steps / plain_s, "seconds": plain_s,
+                  "p50_s": pct(done_plain, 50), "p99_s": pct(done_plain, 99)},
+        "throughput_ratio": plain_s / batched_s,
+        "p99_ratio": pct(done_plain, 99) / pct(done_batched, 99),
+        "parity_batched_vs_plain": ("bitwise" if same else "MISMATCH") + f" ({len(picks)} members compared)",
+    }
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -104,6 +195,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-steps", type=int, default=48, help="steps per timed CPU run")
     ap.add_argument("--cpu-runs", type=int, default=3)
+    ap.add_argument("--ensemble", type=int, default=256, help="ensemble members (0: skip the ensemble leg)")
+    ap.add_argument("--ensemble-steps", type=int, default=200)
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -213,6 +306,8 @@ def main():
         }
         result["parity_first_step"] = "bitwise" if np.array_equal(first_gpu, first_cpu) else "MISMATCH"
         result["gpu_over_cpu"] = value / cpu_value
+    if rank == 0 and args.ensemble > 0:
+        result["ensemble"] = ensemble_bench(args.ensemble, args.ensemble_steps, args.seed)
     print(json.dumps(result))
     return 0
 

@@ -171,6 +171,36 @@ int sfv_device_count(void);
  * All give bitwise identical results.  No counterpart in the reference. */
 int sfv_set_fused(sfv_handle* h, int32_t fused);
 
+/* ---------------------------------------------------------------- ensembles
+ * No counterpart in the reference (a reference user runs one Simulator at a
+ * time).  A batch groups existing handles — grids of any size, each with its
+ * own configuration, topography, state, time and time step — and advances
+ * them together: one persistent launch per RK stage covers every member and
+ * the step sequence is a captured HIP graph.  Each member's results are
+ * bitwise those of stepping it alone.  Members must use RungeKutta2 with the
+ * fused kernels (ghost-cell sources inside the grid: WallBC or PeriodicBC
+ * with nx, ny >= gc), one device and one precision.  The batch borrows the
+ * handles: destroy it before them, and do not step a member through its own
+ * handle while a batch run is in progress.  Read results with
+ * sfv_get_state on each member. */
+typedef struct sfv_batch sfv_batch;
+
+int sfv_batch_create(sfv_handle* const* handles, int32_t n, sfv_batch** out);
+void sfv_batch_destroy(sfv_batch* b);
+/* Message of the last failed call on b (b = NULL: of sfv_batch_create). */
+const char* sfv_batch_last_error(const sfv_batch* b);
+int32_t sfv_batch_size(const sfv_batch* b);
+
+/* simulate_to_time (simulator.jl:179-212) for every member at once, each
+ * from its own t_io[i]: with t_end (n values) member i runs until t_end[i]
+ * (match_endtime and maximum_timestep as in sfv_run), otherwise exactly
+ * max_steps steps each (max_dt = maximum_timestep if > 0).  max_steps > 0
+ * also caps the steps of a t_end run.  On return t_io[i] is member i's time
+ * and steps_out[i] (may be NULL) its step count.  Status SFV_ERR_NONFINITE_DT
+ * names the first member whose dt was NaN or Inf. */
+int sfv_batch_run(sfv_batch* b, const double* t_end, int32_t match_endtime, double maximum_timestep,
+                  int64_t max_steps, double* t_io, int64_t* steps_out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -82,8 +82,22 @@ def test_device_bytes_for_2048_f64():
     cfg.nx = cfg.ny = 2048
     b = C.lib().sfv_device_bytes_for(ctypes.byref(cfg))
     nxT = 2052
-    assert b >= 4 * 3 * nxT * nxT * 8 + 2053 * 2053 * 8
-    assert b < 600e6
+    # ping-pong pair + right-hand side (the fused kernels need no stage-1 state)
+    assert b >= 3 * 3 * nxT * nxT * 8 + 2053 * 2053 * 8
+    assert b < 3 * 3 * nxT * nxT * 8 + 2053 * 2053 * 8 + 1e6
+    cfg.bc = C.SFV_BC_NEUMANN            # per-pass path: + the stage-1 state
+    assert C.lib().sfv_device_bytes_for(ctypes.byref(cfg)) >= 4 * 3 * nxT * nxT * 8
+
+
+def test_batch_create_argument_errors():
+    L = C.lib()
+    b = ctypes.c_void_p()
+    assert L.sfv_batch_create(None, 0, ctypes.byref(b)) == C.SFV_ERR_ARGUMENT
+    assert L.sfv_batch_last_error(None)
+    arr = (ctypes.c_void_p * 2)(None, None)
+    assert L.sfv_batch_create(arr, 2, ctypes.byref(b)) == C.SFV_ERR_ARGUMENT
+    assert not b.value
+    assert L.sfv_batch_size(None) == 0
 
 
 # ---- the Python mirror raises the reference's exception types

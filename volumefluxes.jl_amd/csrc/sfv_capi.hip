@@ -121,6 +121,11 @@ int step_kernels(sfv_handle* h, int slot, int cur) {
 }
 
 int launch_steps(sfv_handle* h, int64_t nsteps) {
+    if (h->cfg.stepper == SFV_RUNGE_KUTTA2 && h->fused == KV_PER_PASS) {
+        std::string why;
+        const int rc = h->ensure_scratch(why);
+        if (rc) return fail(h, rc, why);
+    }
     for (int64_t s = 0; s < nsteps; ++s) {
         const int slot = (int)((h->step_parity + s) & 1);
         const int cur = (int)((h->cur + s) & 1);
@@ -216,8 +221,16 @@ int64_t sfv_device_bytes_for(const sfv_config* c) {
     if (!c) return -1;
     const int64_t nxT = c->nx + 2 * (int64_t)c->gc, nyT = c->ny + 2 * (int64_t)c->gc;
     const int64_t elem = c->precision == SFV_F32_STORAGE ? 4 : 8;
-    const int64_t nstate = (c->stepper == SFV_RUNGE_KUTTA2) ? 3 : 2;   // ping-pong + (unfused) stage buffer
-    int64_t b = nstate * 3 * nxT * nyT * elem + 3 * nxT * nyT * elem + (nxT + 1) * (nyT + 1) * 8 + 4096;
+    // ping-pong pair + right-hand side (+ the stage-1 state of the per-pass
+    // path, which RK2 grids the fused kernels cannot take use)
+    int64_t nstate = 3;
+    if (c->stepper == SFV_RUNGE_KUTTA2) {
+        sfv_handle probe;
+        probe.cfg = *c;
+        probe.build_params();
+        if (!probe.fusable()) nstate = 4;
+    }
+    int64_t b = nstate * 3 * nxT * nyT * elem + (nxT + 1) * (nyT + 1) * 8 + 4096;
     if (c->infiltration == SFV_INFILTRATION_HORTON) b += nxT * nyT * 8;
     return b;
 }
@@ -368,6 +381,11 @@ int sfv_time_kernel(sfv_handle* h, const char* which, int32_t n, double* ms_out)
     if (w != "stage1" && w != "stage2" && w != "bc" && w != "step" && w != "copy")
         return fail(h, SFV_ERR_ARGUMENT, "which must be stage1, stage2, bc, step or copy");
     HIP_TRY(h, hipSetDevice(h->device));
+    {
+        std::string why;
+        const int rc2 = h->ensure_scratch(why);
+        if (rc2) return fail(h, rc2, why);
+    }
     const int slot = (int)(h->step_parity & 1), cur = (int)(h->cur & 1);
     // Time the kernels against a scratch output so the state is left as is.
     HIP_TRY(h, launch_prepare(h->stream, h->P, h->ctl, slot, 0.0, NAN, 0, 0, 0.0, -1, 0, h->cfg.precision));

@@ -23,7 +23,8 @@ struct sfv_handle {
     hipStream_t stream = nullptr;
     size_t elem = 8, ncell = 0;
     // U[0], U[1]: ping-pong states (current = U[cur]); U[2]: the RK2 stage-1
-    // state of the unfused path and a scratch output for kernel timing.
+    // state of the per-pass path and a scratch output for kernel timing,
+    // allocated on first use.
     void* U[3] = {nullptr, nullptr, nullptr};
     void* RHS = nullptr;               // stage right-hand side, 3 planes
     double* B = nullptr;               // corners (nxT+1, nyT+1)
@@ -70,8 +71,7 @@ struct sfv_handle {
             why = "hipStreamCreate failed"; return SFV_ERR_HIP;
         }
         int rc;
-        const int nstate = c.stepper == SFV_RUNGE_KUTTA2 ? 3 : 2;
-        for (int s = 0; s < nstate; ++s)
+        for (int s = 0; s < 2; ++s)
             if ((rc = alloc(&U[s], 3 * ncell * elem, why))) return rc;
         if ((rc = alloc(&RHS, 3 * ncell * elem, why))) return rc;
         if ((rc = alloc((void**)&B, (size_t)(nxT + 1) * (nyT + 1) * sizeof(double), why))) return rc;
@@ -80,7 +80,7 @@ struct sfv_handle {
             why = "hipHostMalloc failed"; return SFV_ERR_OUT_OF_MEMORY;
         }
         std::memset(ctl_host, 0, sizeof(sfv::StepCtl));
-        for (int s = 0; s < nstate; ++s) (void)hipMemsetAsync(U[s], 0, 3 * ncell * elem, stream);
+        for (int s = 0; s < 2; ++s) (void)hipMemsetAsync(U[s], 0, 3 * ncell * elem, stream);
         (void)hipMemsetAsync(RHS, 0, 3 * ncell * elem, stream);
         (void)hipMemsetAsync(B, 0, (size_t)(nxT + 1) * (nyT + 1) * sizeof(double), stream);
         (void)hipMemsetAsync(ctl, 0, sizeof(sfv::StepCtl), stream);
@@ -98,6 +98,13 @@ struct sfv_handle {
         build_params();
         fused = fusable() ? KV_MARCH : KV_PER_PASS;
         return SFV_OK;
+    }
+
+    // U[2] is allocated on first use: only the per-pass RK2 path (its stage-1
+    // state) and kernel timing (a scratch output) need it.
+    int ensure_scratch(std::string& why) {
+        if (U[2]) return SFV_OK;
+        return alloc(&U[2], 3 * ncell * elem, why);
     }
 
     void build_params() {

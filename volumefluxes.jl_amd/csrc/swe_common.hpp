@@ -67,6 +67,51 @@ struct StepCtl {
     int32_t precision;
 };
 
+// One member of a batch as the batched kernels see it (a device array of
+// these, one per member, in the batch's tile order).  A launch with parity lp
+// works on state buffer U[cur0 ^ lp] and step slot slot0 ^ lp of the member,
+// so a fixed sequence of launches (a captured graph with an even number of
+// steps) serves any membership: the host rewrites the table between replays.
+struct BatchMember {
+    KParams P;
+    StepCtl* ctl;
+    void* U[2];           // the state ping-pong pair
+    void* RHS;            // stage-1 right-hand side
+    const double* B;
+    int cur0, slot0;
+    int my;               // rows per wave segment
+    int strips, seggroups;
+    int tile0;            // first block tile of this member in the batch order
+    int bc0;              // first boundary line of this member
+    int pad;
+    // run parameters, read by the batched prepare
+    double t0, endtime, max_timestep;
+    long long max_steps;
+    int match_endtime, has_max_timestep, until_endtime, precision;
+};
+// Batch-wide counts, in device memory beside the member table.
+struct BatchHeader {
+    int members;
+    int tiles;            // sum of strips * seggroups
+    int bc_lines;         // sum of nx + ny
+    int pad;
+};
+
+// The member owning flattened index t of the batch order: the last member
+// whose first index (tile0, or bc0 for boundary lines) is <= t.  Every member
+// has at least one tile and one line, so the first indices increase strictly.
+template <int LINES>
+__device__ __forceinline__ int batch_find(const BatchMember* __restrict__ mem, int members, int t) {
+    int lo = 0, hi = members - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        const int first = LINES ? mem[mid].bc0 : mem[mid].tile0;
+        if (first <= t) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
 __device__ __forceinline__ size_t cid(const KParams& P, int i, int j) { return (size_t)i + (size_t)P.nxT * j; }
 
 __device__ __forceinline__ double Bcorner(const KParams& P, const double* B, int i, int j) {

@@ -44,6 +44,20 @@ hipError_t launch_bc(hipStream_t st, const KParams& P, StepCtl* ctl, int slot, S
 hipError_t launch_prepare(hipStream_t st, const KParams& P, StepCtl* ctl, int slot, double t, double endtime,
                           int match_endtime, int has_max_timestep, double max_timestep, long long max_steps,
                           int until_endtime, int precision);
+// Batched kernels over a device member table (BatchMember[hdr->members]).
+// Stage kernels are persistent: `blocks` workgroups walk the batch's tiles.
+template <class S>
+hipError_t launch_march_batch(hipStream_t st, int stage, const BatchMember* mem, const BatchHeader* hdr, int lp,
+                              int blocks);
+template <class S>
+hipError_t launch_bc_batch(hipStream_t st, const BatchMember* mem, const BatchHeader* hdr, int lp, int blocks);
+hipError_t launch_prepare_batch(hipStream_t st, const BatchMember* mem, const BatchHeader* hdr, int members);
+// Resident workgroups per CU of the batched stage kernel (occupancy).
+int march_batch_blocks_per_cu(int precision, int stage);
+int march_rows_per_wave(const KParams& P, int stage);
+constexpr int MARCH_COLS = 60;   // output columns per wave strip
+constexpr int MARCH_WAVES = 4;   // waves (segments) per workgroup
+
 // Plain copy of `bytes` (a multiple of 4) from src to dst.
 hipError_t launch_copy(hipStream_t st, const void* src, void* dst, size_t bytes);
 template <class S>

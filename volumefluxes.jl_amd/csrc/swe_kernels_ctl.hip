@@ -16,18 +16,19 @@ __device__ void set_rates(const KParams& P, StepCtl* c, double t) {
 // every interior column; corners are never written.  One thread per row or
 // column reads all source cells first, then writes the ghosts, which gives
 // the reference's sequential per-line semantics (KParams::src_* maps).
-// With last_stage set, thread 0 also closes the step: t += dt
+// Line tid < ny is interior row gc+tid, ny <= tid < ny+nx interior column
+// gc+tid-ny.  With last_stage set, line 0 also closes the step: t += dt
 // (simulator.jl:205), and arms the next step's slot.
 template <class S>
-__global__ void __launch_bounds__(256) k_bc(KParams P, StepCtl* ctl, int slot, S* __restrict__ U, int mode) {
+__device__ __forceinline__ void bc_line(const KParams& P, StepCtl* ctl, int slot, S* __restrict__ U, int mode,
+                                        int tid) {
     const int last_stage = mode == BC_LAST_STAGE;
     if (mode != BC_STANDALONE && !ctl->slot[slot].active) {
         // An inactive step passes inactivity on to the next slot, so steps
         // launched past the end of a run stay no-ops.
-        if (last_stage && blockIdx.x == 0 && threadIdx.x == 0) ctl->slot[1 - slot].active = 0;
+        if (last_stage && tid == 0) ctl->slot[1 - slot].active = 0;
         return;
     }
-    const int tid = blockIdx.x * blockDim.x + threadIdx.x;
     const size_t n = (size_t)P.nxT * P.nyT;
     const int gc = P.gc;
     if (tid < P.ny) {
@@ -76,13 +77,16 @@ __global__ void __launch_bounds__(256) k_bc(KParams P, StepCtl* ctl, int slot, S
     }
 }
 
+template <class S>
+__global__ void __launch_bounds__(256) k_bc(KParams P, StepCtl* ctl, int slot, S* __restrict__ U, int mode) {
+    bc_line<S>(P, ctl, slot, U, mode, blockIdx.x * blockDim.x + threadIdx.x);
+}
+
 // Initialise the step-control block for a host call.  until_endtime = 0 runs
 // a fixed number of steps (endtime is then NaN and never ends the loop).
-__global__ void k_prepare(KParams P, StepCtl* ctl, int slot, double t, double endtime, int match_endtime,
-                          int has_max_timestep, double max_timestep, long long max_steps, int until_endtime,
-                          int precision) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    StepCtl* c = ctl;
+__device__ void prepare_ctl(const KParams& P, StepCtl* c, int slot, double t, double endtime, int match_endtime,
+                            int has_max_timestep, double max_timestep, long long max_steps, int until_endtime,
+                            int precision) {
     c->t = t;
     c->endtime = until_endtime ? endtime : __builtin_nan("");
     c->match_endtime = match_endtime;
@@ -102,6 +106,49 @@ __global__ void k_prepare(KParams P, StepCtl* ctl, int slot, double t, double en
     c->slot[slot].active = act ? 1 : 0;
     set_rates(P, c, t);
 }
+
+__global__ void k_prepare(KParams P, StepCtl* ctl, int slot, double t, double endtime, int match_endtime,
+                          int has_max_timestep, double max_timestep, long long max_steps, int until_endtime,
+                          int precision) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    prepare_ctl(P, ctl, slot, t, endtime, match_endtime, has_max_timestep, max_timestep, max_steps, until_endtime,
+                precision);
+}
+
+// Batched boundary condition (last stage of a step): a grid-stride walk over
+// the boundary lines of every member.
+template <class S>
+__global__ void __launch_bounds__(256) k_bc_batch(const BatchMember* __restrict__ mem,
+                                                  const BatchHeader* __restrict__ hdr, int lp) {
+    const int members = hdr->members, lines = hdr->bc_lines;
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < lines; t += gridDim.x * blockDim.x) {
+        const int m = batch_find<1>(mem, members, t);
+        const BatchMember& M = mem[m];
+        const int cur = (M.cur0 ^ lp) & 1, slot = (M.slot0 ^ lp) & 1;
+        bc_line<S>(M.P, M.ctl, slot, (S*)M.U[cur ^ 1], BC_LAST_STAGE, t - M.bc0);
+    }
+}
+
+// Batched prepare: one thread per member, with the member's run parameters.
+__global__ void k_prepare_batch(const BatchMember* __restrict__ mem, const BatchHeader* __restrict__ hdr) {
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= hdr->members) return;
+    const BatchMember& M = mem[m];
+    prepare_ctl(M.P, M.ctl, M.slot0 & 1, M.t0, M.endtime, M.match_endtime, M.has_max_timestep, M.max_timestep,
+                M.max_steps, M.until_endtime, M.precision);
+}
+
+template <class S>
+hipError_t launch_bc_batch(hipStream_t st, const BatchMember* mem, const BatchHeader* hdr, int lp, int blocks) {
+    hipLaunchKernelGGL(k_bc_batch<S>, dim3(blocks), dim3(256), 0, st, mem, hdr, lp);
+    return hipPeekAtLastError();
+}
+hipError_t launch_prepare_batch(hipStream_t st, const BatchMember* mem, const BatchHeader* hdr, int members) {
+    hipLaunchKernelGGL(k_prepare_batch, dim3((members + 63) / 64), dim3(64), 0, st, mem, hdr);
+    return hipPeekAtLastError();
+}
+template hipError_t launch_bc_batch<double>(hipStream_t, const BatchMember*, const BatchHeader*, int, int);
+template hipError_t launch_bc_batch<float>(hipStream_t, const BatchMember*, const BatchHeader*, int, int);
 
 template <class S>
 hipError_t launch_bc(hipStream_t st, const KParams& P, StepCtl* ctl, int slot, S* U, int mode) {

@@ -58,9 +58,9 @@ namespace sfv {
 #define SFV_MARCH_XCD 2
 #endif
 
-constexpr int MW = 64;          // lanes
-constexpr int MX = 60;          // output columns per wave
-constexpr int MWAVES = 4;       // waves per block
+constexpr int MW = 64;                // lanes
+constexpr int MX = MARCH_COLS;        // output columns per wave
+constexpr int MWAVES = MARCH_WAVES;   // waves per block
 
 // Lane shifts by one across the whole wave with DPP (wave_shr:1 / wave_shl:1,
 // VALU latency instead of an LDS round trip): lane i receives lane i-1
@@ -451,6 +451,32 @@ k_march(KParams P, StepCtl* ctl, int slot, int my, int strips, int seggroups, co
     march_tile<S, STAGE>(P, ctl, slot, my, strip, sg * MWAVES + threadIdx.x / MW, U0, RHS1, OUT, B);
 }
 
+// Batched stage kernel: a persistent grid (a multiple of 8 workgroups) walks
+// the tiles of every member.  The tile order is split into 8 contiguous
+// chunks, one per XCD, and the workgroups of XCD x (blockIdx mod 8 == x) walk
+// chunk x, so a member's neighbouring tiles share that XCD's L2.
+template <class S, int STAGE>
+__global__ void __launch_bounds__(MW* MWAVES) __attribute__((amdgpu_waves_per_eu(SFV_MARCH_MIN_WAVES)))
+k_march_batch(const BatchMember* __restrict__ mem, const BatchHeader* __restrict__ hdr, int lp) {
+    const int members = hdr->members, tiles = hdr->tiles;
+    const int x = blockIdx.x % N_XCD, k = blockIdx.x / N_XCD, per = gridDim.x / N_XCD;
+    const int q = tiles / N_XCD, r = tiles % N_XCD;
+    const int c0 = x * q + min(x, r), c1 = c0 + q + (x < r ? 1 : 0);
+    for (int t = c0 + k; t < c1; t += per) {
+        const int m = batch_find<0>(mem, members, t);
+        const BatchMember& M = mem[m];
+        const int local = t - M.tile0;
+        const int strip = local % M.strips, sg = local / M.strips;
+        const int cur = (M.cur0 ^ lp) & 1, slot = (M.slot0 ^ lp) & 1;
+        const int seg = sg * MWAVES + threadIdx.x / MW;
+        const S* U0 = (const S*)M.U[cur];
+        if (STAGE == 1)
+            march_tile<S, 1>(M.P, M.ctl, slot, M.my, strip, seg, U0, nullptr, (S*)M.RHS, M.B);
+        else
+            march_tile<S, 2>(M.P, M.ctl, slot, M.my, strip, seg, U0, (const S*)M.RHS, (S*)M.U[cur ^ 1], M.B);
+    }
+}
+
 // Rows per wave segment for this grid: the makespan is about the number of
 // rounds of resident waves times the rows each wave marches (segment + 4
 // halo rows), so pick the height in [8, 64] that minimises it.
@@ -500,6 +526,31 @@ hipError_t launch_march(hipStream_t st, const KParams& P, StepCtl* ctl, int slot
                            RHS1, OUT, B);
     return hipPeekAtLastError();
 }
+template <class S>
+hipError_t launch_march_batch(hipStream_t st, int stage, const BatchMember* mem, const BatchHeader* hdr, int lp,
+                              int blocks) {
+    if (blocks < N_XCD || blocks % N_XCD) return hipErrorInvalidValue;
+    if (stage == 1)
+        hipLaunchKernelGGL((k_march_batch<S, 1>), dim3(blocks), dim3(MW * MWAVES), 0, st, mem, hdr, lp);
+    else
+        hipLaunchKernelGGL((k_march_batch<S, 2>), dim3(blocks), dim3(MW * MWAVES), 0, st, mem, hdr, lp);
+    return hipPeekAtLastError();
+}
+
+int march_batch_blocks_per_cu(int precision, int stage) {
+    int blocks = 0;
+    hipError_t e;
+    if (precision == 0)
+        e = stage == 1 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_march_batch<double, 1>, MW * MWAVES, 0)
+                       : hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_march_batch<double, 2>, MW * MWAVES, 0);
+    else
+        e = stage == 1 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_march_batch<float, 1>, MW * MWAVES, 0)
+                       : hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_march_batch<float, 2>, MW * MWAVES, 0);
+    return (e == hipSuccess && blocks > 0) ? blocks : 2;
+}
+
+template hipError_t launch_march_batch<double>(hipStream_t, int, const BatchMember*, const BatchHeader*, int, int);
+template hipError_t launch_march_batch<float>(hipStream_t, int, const BatchMember*, const BatchHeader*, int, int);
 template hipError_t launch_march<double>(hipStream_t, const KParams&, StepCtl*, int, int, const double*,
                                          const double*, double*, const double*);
 template hipError_t launch_march<float>(hipStream_t, const KParams&, StepCtl*, int, int, const float*, const float*,

@@ -28,4 +28,6 @@ from .simulator import (HIPBackend, IntervalWriter, MultipleCallbacks, Simulator
                         get_available_backends, has_hip_backend, make_hip_backend, name, perform_step,
                         set_current_state, simulate_to_time)
 
+from .batch import Ensemble  # noqa: E402
+
 __all__ = [n for n in dir() if not n.startswith("_")]

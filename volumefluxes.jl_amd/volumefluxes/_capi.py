@@ -153,16 +153,18 @@ def _declare(L) -> None:
     L.sfv_device_count.restype = c_int32
     L.sfv_set_fused.argtypes = [h, c_int32]
     L.sfv_set_fused.restype = c_int32
-    for name, argtypes, restype in _EXTRA_DECLS:
-        fn = getattr(L, name, None)
-        if fn is not None:
-            fn.argtypes = argtypes
-            fn.restype = restype
-
-
-# Declarations of the ensemble / serving entry points, filled in by the
-# modules that use them (kept here so one loader declares every symbol).
-_EXTRA_DECLS: list = []
+    b = c_void_p
+    L.sfv_batch_create.argtypes = [POINTER(c_void_p), c_int32, POINTER(c_void_p)]
+    L.sfv_batch_create.restype = c_int32
+    L.sfv_batch_destroy.argtypes = [b]
+    L.sfv_batch_destroy.restype = None
+    L.sfv_batch_last_error.argtypes = [b]
+    L.sfv_batch_last_error.restype = c_char_p
+    L.sfv_batch_size.argtypes = [b]
+    L.sfv_batch_size.restype = c_int32
+    L.sfv_batch_run.argtypes = [b, POINTER(c_double), c_int32, c_double, c_int64, POINTER(c_double),
+                                POINTER(c_int64)]
+    L.sfv_batch_run.restype = c_int32
 
 
 def lib():
@@ -182,6 +184,12 @@ def lib():
 def check(handle, code: int) -> None:
     if code != SFV_OK:
         msg = lib().sfv_last_error(handle) if handle else lib().sfv_create_error()
+        raise_for(code, (msg or b"").decode(errors="replace"))
+
+
+def check_batch(batch, code: int) -> None:
+    if code != SFV_OK:
+        msg = lib().sfv_batch_last_error(batch)
         raise_for(code, (msg or b"").decode(errors="replace"))
 
 
