@@ -1,0 +1,281 @@
+# HIPBackend.jl — SinFVM backend that runs the 2D shallow-water step on an
+# AMD MI355X (gfx950) through libsinfvm_hip.so (include/sinfvm_hip.h).
+#
+# Load after SinFVM:
+#     using SinFVM
+#     include("HIPBackend.jl"); using .SinFVMHIP
+#     backend = SinFVMHIP.make_hip_backend()            # or make_hip_backend(Float32)
+#     sim = Simulator(backend, system, RungeKutta2(), grid)
+#     set_current_state!(sim, initial); simulate_to_time(sim, T)
+#
+# How it plugs in (src/backends/kernel_abstractions.jl:24-55, src/simulator.jl):
+#   * HIPBackend{T} <: SinFVM.Backend with field `realtype`, like
+#     KernelAbstractionBackend.  The reference's generic create_buffer /
+#     convert_to_backend (src/backends/buffer.jl:22,29-31) give it plain host
+#     arrays: every Volume of a HIP simulator is a host mirror.
+#   * perform_step!(sim::Simulator{<:HIPBackend}, max_dt) is one sfv_step call
+#     (src/simulator.jl:145-177).  simulate_to_time without a callback is one
+#     sfv_run call (a device-resident loop); with a callback the reference's
+#     own loop runs and calls perform_step! per step.
+#   * set_current_state!, current_state, current_interior_state sync the host
+#     mirror (src/simulator.jl:98-135).
+# The device handle of a simulator is kept in a WeakKeyDict keyed by its
+# `t` MVector (the Simulator struct itself is immutable), created on first use.
+module SinFVMHIP
+
+using SinFVM
+using StaticArrays
+
+const libsinfvm = get(ENV, "SINFVM_HIP_LIB", joinpath(@__DIR__, "..", "lib", "libsinfvm_hip.so"))
+
+# ---- status codes and enums (include/sinfvm_hip.h) ------------------------
+const SFV_OK = 0
+const SFV_ERR_DOMAIN = 1
+const SFV_ERR_ARGUMENT = 2
+const SFV_ERR_ASSERTION = 3
+const SFV_ERR_NONFINITE_DT = 5
+const SFV_BC_WALL, SFV_BC_PERIODIC, SFV_BC_NEUMANN = 0, 1, 2
+const SFV_FORWARD_EULER, SFV_RUNGE_KUTTA2 = 1, 2
+const SFV_LAYOUT_FULL, SFV_LAYOUT_INTERIOR = 0, 1
+
+# Field order and types match `sfv_config` exactly (C layout of an isbits struct).
+mutable struct SfvConfig
+    nx::Int32; ny::Int32; gc::Int32
+    dx::Float64; dy::Float64
+    bc::Int32
+    g::Float64; rho::Float64; depth_cutoff::Float64; desingularizing_kappa::Float64
+    reconstruction::Int32
+    theta::Float64
+    stepper::Int32
+    cfl::Float64
+    bottom_source::Int32
+    friction::Int32
+    Cz::Float64
+    rain::Int32
+    rain_rate::Float64
+    rain_table_len::Int32
+    rain_table_rates::Ptr{Float64}
+    rain_table_times::Ptr{Float64}
+    infiltration::Int32
+    infiltration_rate::Float64
+    horton_f0::Float64; horton_fc::Float64; horton_k::Float64
+    infiltration_factor_scalar::Float64
+    infiltration_factor::Ptr{Float64}
+    rain_before_infiltration::Int32
+    precision::Int32
+    SfvConfig() = new()
+end
+
+struct HIPBackend{RealType} <: SinFVM.Backend
+    realtype::Type{RealType}
+end
+
+device_count() = Int(ccall((:sfv_device_count, libsinfvm), Cint, ()))
+
+function make_hip_backend(RealType=Float64)
+    RealType in (Float64, Float32) || throw(ArgumentError("HIPBackend supports Float64 and Float32"))
+    device_count() > 0 || error("no HIP device visible to libsinfvm_hip")
+    return HIPBackend{RealType}(RealType)
+end
+SinFVM.name(::HIPBackend) = "HIP"
+
+"get_available_backends() plus the HIP backend when a device is present (kernel_abstractions.jl:45-55)."
+function available_backends_with_hip()
+    backends = SinFVM.get_available_backends()
+    try
+        push!(backends, make_hip_backend())
+    catch err
+        @show err
+    end
+    return backends
+end
+
+# ---- errors ----------------------------------------------------------------
+last_error(h) = unsafe_string(ccall((:sfv_last_error, libsinfvm), Cstring, (Ptr{Cvoid},), h))
+create_error() = unsafe_string(ccall((:sfv_create_error, libsinfvm), Cstring, ()))
+
+function check(h, rc)
+    rc == SFV_OK && return nothing
+    msg = h == C_NULL ? create_error() : last_error(h)
+    rc == SFV_ERR_DOMAIN && throw(DomainError(msg))
+    rc == SFV_ERR_ARGUMENT && throw(ArgumentError(msg))
+    rc == SFV_ERR_ASSERTION && throw(AssertionError(msg))
+    rc == SFV_ERR_NONFINITE_DT && (@warn msg; return nothing)
+    error("libsinfvm_hip: $msg (status $rc)")
+end
+
+# ---- system -> sfv_config ----------------------------------------------------
+friction_code(f) = f === SinFVM.friction_bh2021 ? 1 : f === SinFVM.friction_bsa2012 ? 2 :
+                   f === SinFVM.friction_fcg2016 ? 3 : throw(ArgumentError("unsupported friction function $f"))
+bc_code(::SinFVM.WallBC) = SFV_BC_WALL
+bc_code(::SinFVM.PeriodicBC) = SFV_BC_PERIODIC
+bc_code(::SinFVM.NeumannBC) = SFV_BC_NEUMANN
+stepper_code(::SinFVM.RungeKutta2) = SFV_RUNGE_KUTTA2
+stepper_code(::SinFVM.ForwardEulerStepper) = SFV_FORWARD_EULER
+
+mutable struct HandleBox
+    h::Ptr{Cvoid}
+    keep::Vector{Any}          # arrays the config points into, alive until sfv_create copies them
+    host_valid::Bool           # substep_outputs[1] mirrors the device state
+end
+
+const HANDLES = WeakKeyDict{Any,HandleBox}()
+
+function build_config(sim)
+    sys, grid, eq = sim.system, sim.grid, sim.system.equation
+    sys.numericalflux isa SinFVM.CentralUpwind || throw(ArgumentError("HIPBackend runs CentralUpwind"))
+    eq isa SinFVM.ShallowWaterEquations || throw(ArgumentError("HIPBackend runs the 2D ShallowWaterEquations"))
+    c = SfvConfig()
+    ccall((:sfv_config_defaults, libsinfvm), Cvoid, (Ref{SfvConfig}, Int32, Int32, Float64, Float64), c,
+          grid.totalcells[1] - 2grid.ghostcells[1], grid.totalcells[2] - 2grid.ghostcells[2],
+          SinFVM.compute_dx(grid, SinFVM.XDIR), SinFVM.compute_dx(grid, SinFVM.YDIR))
+    c.gc = grid.ghostcells[1]
+    c.bc = bc_code(grid.boundary)
+    c.g, c.rho = eq.g, eq.ρ
+    c.depth_cutoff, c.desingularizing_kappa = eq.depth_cutoff, eq.desingularizing_kappa
+    if sys.reconstruction isa SinFVM.LinearReconstruction
+        c.reconstruction, c.theta = 1, sys.reconstruction.theta
+    else
+        c.reconstruction = 0
+    end
+    c.stepper = stepper_code(sim.timestepper)
+    c.cfl = sim.cfl
+    keep = Any[]
+    seen_rain = false
+    c.rain_before_infiltration = 1
+    for st in sys.source_terms
+        if st isa SinFVM.SourceTermBottom
+            c.bottom_source = 1
+        elseif st isa SinFVM.ConstantRain
+            c.rain, c.rain_rate, seen_rain = 1, st.rain_rate, true
+        elseif st isa SinFVM.TimeDependentRain
+            rates, times = collect(Float64, st.rain_rates), collect(Float64, st.time)
+            push!(keep, rates, times)
+            c.rain, c.rain_table_len = 2, length(rates)
+            c.rain_table_rates, c.rain_table_times = pointer(rates), pointer(times)
+            seen_rain = true
+        elseif st isa SinFVM.HortonInfiltration
+            factor = collect(Float64, st.factor)
+            push!(keep, factor)
+            c.infiltration = 2
+            c.horton_f0, c.horton_fc, c.horton_k = st.f0, st.fc, st.k
+            c.infiltration_factor = pointer(factor)
+            seen_rain || (c.rain_before_infiltration = 0)
+        elseif st isa SinFVM.ConstantInfiltration
+            c.infiltration, c.infiltration_rate = 1, st.infiltration_rate
+            seen_rain || (c.rain_before_infiltration = 0)
+        else
+            throw(ArgumentError("HIPBackend does not run source term $(typeof(st))"))
+        end
+    end
+    fr = sys.implicit_source_term
+    if fr !== nothing
+        fr isa SinFVM.ImplicitFriction || throw(ArgumentError("unsupported implicit source term"))
+        c.friction, c.Cz = friction_code(fr.friction_function), fr.Cz
+    end
+    c.precision = sim.backend.realtype === Float32 ? 1 : 0
+    return c, keep
+end
+
+function corner_topography(sim)
+    B = sim.system.equation.B
+    n = (sim.grid.totalcells[1] + 1, sim.grid.totalcells[2] + 1)
+    B isa SinFVM.ConstantBottomTopography && return fill(Float64(B.B), n)
+    return collect(Float64, B.B)
+end
+
+function handle!(sim::SinFVM.Simulator{<:HIPBackend})
+    box = get(HANDLES, sim.t, nothing)
+    box === nothing || return box
+    c, keep = build_config(sim)
+    href = Ref{Ptr{Cvoid}}(C_NULL)
+    GC.@preserve keep check(C_NULL, ccall((:sfv_create, libsinfvm), Cint, (Ref{SfvConfig}, Ref{Ptr{Cvoid}}), c, href))
+    box = HandleBox(href[], Any[], false)
+    finalizer(b -> b.h == C_NULL || ccall((:sfv_destroy, libsinfvm), Cvoid, (Ptr{Cvoid},), b.h), box)
+    HANDLES[sim.t] = box
+    B = corner_topography(sim)
+    check(box.h, ccall((:sfv_set_topography, libsinfvm), Cint, (Ptr{Cvoid}, Ptr{Float64}, Int64, Int64),
+                       box.h, B, size(B, 1), size(B, 2)))
+    # the state the Simulator was built with (zeros, or whatever its mirror holds)
+    upload_full!(sim, box)
+    return box
+end
+
+function upload_full!(sim, box)
+    data = sim.substep_outputs[1]._data
+    check(box.h, ccall((:sfv_set_state, libsinfvm), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Int32),
+                       box.h, data, SFV_LAYOUT_FULL))
+    box.host_valid = true
+end
+
+function sync_host!(sim)
+    box = handle!(sim)
+    box.host_valid && return
+    data = sim.substep_outputs[1]._data
+    check(box.h, ccall((:sfv_get_state, libsinfvm), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Int32),
+                       box.h, data, SFV_LAYOUT_FULL))
+    box.host_valid = true
+end
+
+# ---- the reference API on a HIP simulator ---------------------------------------
+function SinFVM.perform_step!(sim::SinFVM.Simulator{<:HIPBackend}, max_dt)
+    box = handle!(sim)
+    dt = Ref{Float64}(0.0)
+    check(box.h, ccall((:sfv_step, libsinfvm), Cint, (Ptr{Cvoid}, Float64, Float64, Ref{Float64}),
+                       box.h, max_dt, sim.t[1], dt))
+    sim.current_timestep[1] = dt[]
+    box.host_valid = false
+    return nothing
+end
+
+function SinFVM.simulate_to_time(sim::SinFVM.Simulator{<:HIPBackend}, endtime; match_endtime=true,
+                                 callback=nothing, show_progress=true, maximum_timestep=nothing)
+    if callback !== nothing
+        # the reference's own loop: perform_step! per step, callback after each
+        return invoke(SinFVM.simulate_to_time, Tuple{SinFVM.Simulator,Any}, sim, endtime;
+                      match_endtime=match_endtime, callback=callback, show_progress=show_progress,
+                      maximum_timestep=maximum_timestep)
+    end
+    box = handle!(sim)
+    t = Ref{Float64}(sim.t[1])
+    steps = Ref{Int64}(0)
+    mts = maximum_timestep === nothing ? 0.0 : Float64(maximum_timestep)
+    rc = ccall((:sfv_run, libsinfvm), Cint, (Ptr{Cvoid}, Float64, Int32, Float64, Int64, Ref{Float64}, Ref{Int64}),
+               box.h, endtime, match_endtime ? 1 : 0, mts, 0, t, steps)
+    sim.t[1] = t[]
+    if steps[] > 0
+        dt = Ref{Float64}(0.0)
+        check(box.h, ccall((:sfv_current_timestep, libsinfvm), Cint, (Ptr{Cvoid}, Ref{Float64}), box.h, dt))
+        sim.current_timestep[1] = dt[]
+    end
+    box.host_valid = false
+    check(box.h, rc)
+    return nothing
+end
+
+function SinFVM.current_state(sim::SinFVM.Simulator{<:HIPBackend})
+    sync_host!(sim)
+    return sim.substep_outputs[1]
+end
+
+SinFVM.current_interior_state(sim::SinFVM.Simulator{<:HIPBackend}) = SinFVM.interior(SinFVM.current_state(sim))
+
+function SinFVM.set_current_state!(sim::SinFVM.Simulator{<:HIPBackend}, new_state)
+    box = handle!(sim)
+    sync_host!(sim)
+    SinFVM.current_interior_state(sim)[1:end, 1:end] = new_state
+    # interior written into the mirror; the library applies the boundary condition
+    upload_interior!(sim, box)
+    return nothing
+end
+
+function upload_interior!(sim, box)
+    g = sim.grid.ghostcells[1]
+    data = sim.substep_outputs[1]._data
+    inner = Array(data[g+1:end-g, g+1:end-g, :])
+    check(box.h, ccall((:sfv_set_state, libsinfvm), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Int32),
+                       box.h, inner, SFV_LAYOUT_INTERIOR))
+    box.host_valid = false          # ghosts now differ from the mirror until the next read
+end
+
+end # module
