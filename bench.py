@@ -182,6 +182,132 @@ def ensemble_bench(members: int, steps: int, seed: int):
     }
 
 
+def serving_scenarios(rate: float, duration: float, t_range, seed: int):
+    """Serving requests: Poisson arrivals (rate per s over `duration` s); each
+    a square grid of side log-uniform in [64, 2048] (dx = 1 m), half dam
+    breaks (2 m | 1 m, flat bottom) and half terrains (seeded bumps as in the
+    single-grid workload), Manning n = 0.03, f64, RK2, t_end U[t_range]."""
+    rng = np.random.default_rng(seed)
+    reqs, t = [], 0.0
+    while True:
+        t += float(rng.exponential(1.0 / rate))
+        if t > duration:
+            break
+        side = int(round(float(np.exp(rng.uniform(np.log(64), np.log(2048))))))
+        grid = VF.CartesianGrid(side, side, gc=2, boundary=VF.WallBC(), extent=[[0.0, float(side)], [0.0, float(side)]])
+        friction = VF.ImplicitFriction(Cz=0.03 * 0.03)
+        if rng.uniform() < 0.5:
+            eq = VF.ShallowWaterEquations(desingularizing_kappa=1e-4)
+            system = VF.ConservedSystem(None, VF.LinearReconstruction(), VF.CentralUpwind(eq), eq, grid, [], friction)
+            from volumefluxes.scenarios import dam_break_initial_state
+            U0 = dam_break_initial_state(grid, side / 2.0, 2.0, 1.0)
+            kind = "dam_break"
+        else:
+            B = terrain_topography(grid, seed=int(rng.integers(1 << 30)), n_bumps=32, slope=1e-3, amp=(0.5, 5.0),
+                                   width=(20.0, 200.0))
+            eq = VF.ShallowWaterEquations(VF.BottomTopography2D(B, None, grid), desingularizing_kappa=1e-4)
+            system = VF.ConservedSystem(None, VF.LinearReconstruction(), VF.CentralUpwind(eq), eq, grid,
+                                        [VF.SourceTermBottom(), VF.ConstantRain(1e-5 * 3600)], friction)
+            U0 = terrain_initial_state(grid, B, 2.0)
+            kind = "terrain"
+        reqs.append({"arrival": t, "grid": grid, "system": system, "U0": U0, "kind": kind,
+                     "t_end": float(rng.uniform(*t_range))})
+    return reqs
+
+
+def serving_bench(rate: float, duration: float, t_range, budget_gb: float, seed: int):
+    """Continuous batching (Server: arrivals submitted in real time, admitted
+    under the HBM budget, advanced together) against plain one-at-a-time
+    FIFO service of the same requests (each a device-resident
+    simulate_to_time; its queue is replayed from the measured service times
+    and the same arrival times).  Time-to-result = completion - arrival."""
+    import threading
+    reqs = serving_scenarios(rate, duration, t_range, seed)
+    n = len(reqs)
+    backend = VF.make_hip_backend(np.float64)
+    # warm-up of both paths
+    w = reqs[0]
+    with VF.Server(int(budget_gb * (1 << 30))) as sv:
+        rid = sv.submit(w["system"], w["grid"], w["U0"], 0.1)
+        sv.wait(rid, timeout=120)
+        sv.fetch(rid)
+    sim = VF.Simulator(backend, w["system"], VF.RungeKutta2(), w["grid"])
+    VF.set_current_state(sim, w["U0"])
+    VF.simulate_to_time(sim, 0.1)
+    del sim
+
+    # plain: service times of one-at-a-time solo runs
+    service, solo_states, cells = [], {}, []
+    picks = set(range(0, n, max(1, n // 6))) | {n - 1}
+    for k, r in enumerate(reqs):
+        sim = VF.Simulator(backend, r["system"], VF.RungeKutta2(), r["grid"])
+        VF.set_current_state(sim, r["U0"])
+        C.check(sim.handle, C.lib().sfv_synchronize(sim.handle))
+        t0 = time.perf_counter()
+        L0 = C.lib()
+        tt = ctypes.c_double(0.0)
+        steps = ctypes.c_int64()
+        C.check(sim.handle, L0.sfv_run(sim.handle, r["t_end"], 1, 0.0, 0, ctypes.byref(tt), ctypes.byref(steps)))
+        service.append(time.perf_counter() - t0)
+        cells.append(r["grid"].nx * r["grid"].ny * steps.value)
+        if k in picks:
+            g = r["grid"].gc
+            solo_states[k] = sim.get_full_state()[g:-g, g:-g, :]
+        del sim
+    done, clock = [], 0.0
+    for r, sv_t in zip(reqs, service):
+        clock = max(clock, r["arrival"]) + sv_t
+        done.append(clock)
+    plain_lat = [d - r["arrival"] for d, r in zip(done, reqs)]
+    plain_span = done[-1] - reqs[0]["arrival"]
+
+    # served: real-time arrivals into the server
+    ids = [None] * n
+    with VF.Server(int(budget_gb * (1 << 30))) as sv:
+        def submitter():
+            t_start = time.perf_counter()
+            for k, r in enumerate(reqs):
+                delay = r["arrival"] - (time.perf_counter() - t_start)
+                if delay > 0:
+                    time.sleep(delay)
+                ids[k] = sv.submit(r["system"], r["grid"], r["U0"], r["t_end"])
+        th = threading.Thread(target=submitter)
+        th.start()
+        th.join()
+        infos, same = [], True
+        for k, rid in enumerate(ids):
+            sv.wait(rid, timeout=1200)
+            U, info = sv.fetch(rid)
+            infos.append(info)
+            if k in solo_states:
+                same = same and np.array_equal(U, solo_states[k])
+        st = sv.stats()
+    lat = [i["done_s"] - i["submit_s"] for i in infos]
+    wait = [i["admit_s"] - i["submit_s"] for i in infos]
+    span = max(i["done_s"] for i in infos) - min(i["submit_s"] for i in infos)
+    total = float(sum(cells))
+
+    def pct(v, q):
+        return float(np.percentile(np.asarray(v), q))
+    return {
+        "workload": (f"Poisson arrivals {rate}/s for {duration} s ({n} requests), square grids side log-uniform "
+                     f"[64, 2048], half dam breaks / half terrains, f64 RK2, t_end U[{t_range[0]}, {t_range[1]}] s "
+                     f"(the full workload's U[10, 600] s scaled down to keep the run short), HBM budget {budget_gb} GB"),
+        "requests": n,
+        "cell_updates_total": total,
+        "offered_cell_updates_per_s": total / duration,
+        "served": {"p50_s": pct(lat, 50), "p99_s": pct(lat, 99), "queue_wait_p50_s": pct(wait, 50),
+                   "queue_wait_p99_s": pct(wait, 99), "cell_updates_per_s": total / span, "makespan_s": span,
+                   "busy_s": st["busy_s"], "peak_bytes": st["bytes_peak"], "replays": st["replays"]},
+        "plain_fifo": {"p50_s": pct(plain_lat, 50), "p99_s": pct(plain_lat, 99),
+                       "cell_updates_per_s": total / plain_span, "makespan_s": plain_span,
+                       "service_s_total": float(sum(service))},
+        "throughput_ratio": (total / span) / (total / plain_span),
+        "p99_ratio": pct(plain_lat, 99) / pct(lat, 99),
+        "parity_served_vs_alone": ("bitwise" if same else "MISMATCH") + f" ({len(solo_states)} requests compared)",
+    }
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -197,6 +323,10 @@ def main():
     ap.add_argument("--cpu-runs", type=int, default=3)
     ap.add_argument("--ensemble", type=int, default=256, help="ensemble members (0: skip the ensemble leg)")
     ap.add_argument("--ensemble-steps", type=int, default=200)
+    ap.add_argument("--serving-rate", type=float, default=50.0, help="arrivals per second (0: skip serving)")
+    ap.add_argument("--serving-duration", type=float, default=1.0, help="seconds of arrivals")
+    ap.add_argument("--serving-tend", type=float, nargs=2, default=[1.0, 30.0], help="t_end range of a request (s)")
+    ap.add_argument("--serving-budget-gb", type=float, default=64.0)
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -308,6 +438,9 @@ def main():
         result["gpu_over_cpu"] = value / cpu_value
     if rank == 0 and args.ensemble > 0:
         result["ensemble"] = ensemble_bench(args.ensemble, args.ensemble_steps, args.seed)
+    if rank == 0 and args.serving_rate > 0:
+        result["serving"] = serving_bench(args.serving_rate, args.serving_duration, tuple(args.serving_tend),
+                                          args.serving_budget_gb, args.seed)
     print(json.dumps(result))
     return 0
 

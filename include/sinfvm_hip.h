@@ -201,6 +201,64 @@ int32_t sfv_batch_size(const sfv_batch* b);
 int sfv_batch_run(sfv_batch* b, const double* t_end, int32_t match_endtime, double maximum_timestep,
                   int64_t max_steps, double* t_io, int64_t* steps_out);
 
+/* ------------------------------------------------------------------ serving
+ * No counterpart in the reference.  A server runs flood scenarios submitted
+ * at any time (continuous batching): a worker thread admits queued requests
+ * in arrival order while their device bytes (sfv_device_bytes_for) fit the
+ * HBM budget, advances everything admitted with the batched kernels (each
+ * request with its own dt, until its own t_end, as simulate_to_time with
+ * match_endtime does), and retires finished requests: the final interior
+ * state is copied to host memory and the device memory is freed.  A result
+ * is bitwise that of running the request alone.  Requests use RungeKutta2,
+ * the server's precision, and ghost-cell sources inside the grid (WallBC or
+ * PeriodicBC with nx, ny >= gc).  All calls are thread-safe. */
+typedef struct sfv_server sfv_server;
+
+enum sfv_request_status { SFV_REQ_QUEUED = 0, SFV_REQ_RUNNING = 1, SFV_REQ_DONE = 2, SFV_REQ_FAILED = 3 };
+
+typedef struct sfv_request_info {
+    int64_t id;
+    int32_t status;        /* enum sfv_request_status */
+    int32_t error;         /* status code of a failed (or non-finite-dt) request */
+    double t;              /* simulated time reached */
+    int64_t steps;
+    int64_t cell_updates;  /* steps * nx * ny */
+    int64_t device_bytes;  /* what the request holds on the device while it runs */
+    double submit_s, admit_s, done_s;   /* seconds since the server started; -1 = not yet */
+} sfv_request_info;
+
+typedef struct sfv_server_stats {
+    int64_t submitted, admitted, completed, failed, queued, running;
+    int64_t bytes_in_use, bytes_peak, budget_bytes;
+    int64_t cell_updates;  /* of completed requests */
+    int64_t replays;       /* graph replays (8 steps of every running request each) */
+    double busy_s;         /* time spent in replays */
+    double uptime_s;
+} sfv_server_stats;
+
+/* Start a server on the current device with an HBM byte budget. */
+int sfv_server_create(int64_t budget_bytes, int32_t precision, sfv_server** out);
+/* Stop the worker (queued and running requests fail) and free everything. */
+void sfv_server_destroy(sfv_server* s);
+const char* sfv_server_last_error(const sfv_server* s);
+/* Queue a request: configuration (tables it points to are copied), corner
+ * topography (nxT+1, nyT+1) and the initial interior state (nx, ny, 3) in the
+ * server's precision; it runs from t0 to t_end.  Inputs are copied.  A
+ * request larger than the whole budget fails with SFV_ERR_BUDGET. */
+int sfv_server_submit(sfv_server* s, const sfv_config* cfg, const double* B, int64_t nbx, int64_t nby,
+                      const void* U_interior, double t0, double t_end, int32_t match_endtime,
+                      double maximum_timestep, int64_t* id_out);
+/* Status of a request. */
+int sfv_server_poll(sfv_server* s, int64_t id, sfv_request_info* info);
+/* Block until the request is done or failed (timeout_s < 0: no limit);
+ * SFV_ERR_STATE on timeout. */
+int sfv_server_wait(sfv_server* s, int64_t id, double timeout_s, sfv_request_info* info);
+/* Copy a finished request's final interior state (nx, ny, 3) to U_interior_out
+ * (may be NULL) and forget the request.  Returns the request's own status
+ * code for a failed request. */
+int sfv_server_fetch(sfv_server* s, int64_t id, void* U_interior_out, sfv_request_info* info);
+int sfv_server_get_stats(sfv_server* s, sfv_server_stats* out);
+
 #ifdef __cplusplus
 }
 #endif

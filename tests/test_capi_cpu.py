@@ -162,3 +162,15 @@ def test_product_code_never_imports_the_oracle():
                 text = open(os.path.join(dirpath, f), errors="replace").read()
                 assert not re.search(r"^\s*(import|from)\s+oracle\b", text, flags=re.M), f
                 assert "liboracle" not in text and "swe_oracle" not in text, f
+
+
+def test_server_needs_a_device():
+    L = C.lib()
+    s = ctypes.c_void_p()
+    assert L.sfv_server_create(0, 0, ctypes.byref(s)) == C.SFV_ERR_ARGUMENT
+    if L.sfv_device_count() > 0:
+        pytest.skip("a GPU is visible")
+    assert L.sfv_server_create(1 << 30, 0, ctypes.byref(s)) == C.SFV_ERR_HIP
+    assert not s.value
+    with pytest.raises(VF.SfvError if hasattr(VF, "SfvError") else Exception):
+        VF.Server(1 << 30)

@@ -1,0 +1,101 @@
+"""Serving on the GPU: requests submitted while others run are admitted
+under the HBM byte budget, and every result is bitwise the result of
+running the request alone (simulate_to_time with the same arguments)."""
+import time
+
+import numpy as np
+import pytest
+
+import volumefluxes as VF
+
+from helpers import dam_break, make_hip, terrain
+
+pytestmark = pytest.mark.gpu
+
+
+def _interior(sim):
+    g = sim.grid.gc
+    return sim.get_full_state()[g:-g, g:-g, :]
+
+
+def _alone(sc, t_end, maximum_timestep=None):
+    sim = make_hip(sc)
+    VF.simulate_to_time(sim, t_end, maximum_timestep=maximum_timestep)
+    return sim
+
+
+def _requests():
+    return [
+        (terrain(40, 30, seed=1), 2.0),
+        (terrain(130, 64, seed=2, bc=VF.PeriodicBC()), 1.0),
+        (dam_break(96, 10, hr=0.0), 3.5),
+        (terrain(64, 200, seed=3, friction=False), 0.75),
+        (terrain(257, 129, seed=4), 1.25),
+    ]
+
+
+def test_served_results_equal_solo_runs():
+    reqs = _requests()
+    with VF.Server(8 << 30) as sv:
+        ids = [sv.submit(sc.system, sc.grid, sc.U0, te, cfl=sc.cfl) for sc, te in reqs]
+        for (sc, te), rid in zip(reqs, ids):
+            info = sv.wait(rid, timeout=120)
+            assert info["status"] == 2, info
+            U, info = sv.fetch(rid)
+            alone = _alone(sc, te)
+            print(sc.name, info["t"], info["steps"], alone.t)
+            assert info["t"] == alone.t == te
+            assert np.array_equal(U, _interior(alone)), sc.name
+        st = sv.stats()
+        assert st["completed"] == len(reqs) and st["failed"] == 0 and st["bytes_in_use"] == 0
+
+
+def test_budget_limits_what_runs_at_once():
+    reqs = _requests()
+    per = [VF.build_config(sc.system, sc.stepper, sc.grid, sc.cfl, 0) for sc, _ in reqs]
+    from volumefluxes import _capi as C
+    import ctypes
+    sizes = [C.lib().sfv_device_bytes_for(ctypes.byref(c)) for c in per]
+    budget = max(sizes) + min(sizes)          # at most about two at a time
+    with VF.Server(budget) as sv:
+        ids = []
+        for sc, te in reqs:
+            ids.append(sv.submit(sc.system, sc.grid, sc.U0, te, cfl=sc.cfl))
+            time.sleep(0.01)                  # arrivals while others run
+        outs = [sv.fetch(rid) if sv.wait(rid, timeout=120) else None for rid in ids]
+        st = sv.stats()
+    assert st["bytes_peak"] <= budget
+    for (sc, te), (U, info) in zip(reqs, outs):
+        alone = _alone(sc, te)
+        assert np.array_equal(U, _interior(alone)), sc.name
+        assert info["admit_s"] >= info["submit_s"] and info["done_s"] >= info["admit_s"]
+
+
+def test_request_over_budget_fails_with_budget_error():
+    sc = terrain(256, 256, seed=9)
+    with VF.Server(1 << 20) as sv:
+        rid = sv.submit(sc.system, sc.grid, sc.U0, 1.0)
+        info = sv.wait(rid, timeout=60)
+        assert info["status"] == 3
+        with pytest.raises(VF.BudgetError):
+            sv.fetch(rid)
+
+
+def test_unfusable_request_is_rejected_at_submit():
+    sc = terrain(40, 40, seed=1, bc=VF.NeumannBC())
+    with VF.Server(1 << 30) as sv:
+        with pytest.raises(VF.ArgumentError):
+            sv.submit(sc.system, sc.grid, sc.U0, 1.0)
+
+
+def test_maximum_timestep_and_late_arrivals():
+    a, b = terrain(90, 45, seed=11), terrain(45, 90, seed=12)
+    with VF.Server(4 << 30) as sv:
+        ra = sv.submit(a.system, a.grid, a.U0, 3.0, maximum_timestep=0.02)
+        time.sleep(0.05)
+        rb = sv.submit(b.system, b.grid, b.U0, 0.5)
+        Ub, ib = sv.fetch(rb) if sv.wait(rb, timeout=60) else (None, None)
+        Ua, ia = sv.fetch(ra) if sv.wait(ra, timeout=60) else (None, None)
+    sa, sb = _alone(a, 3.0, maximum_timestep=0.02), _alone(b, 0.5)
+    assert np.array_equal(Ua, _interior(sa)) and ia["t"] == 3.0
+    assert np.array_equal(Ub, _interior(sb)) and ib["t"] == 0.5

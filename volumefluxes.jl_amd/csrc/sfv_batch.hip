@@ -17,38 +17,235 @@
 #include <vector>
 
 #include "../../include/sinfvm_hip.h"
-#include "sfv_handle.hpp"
+#include "sfv_engine.hpp"
 
 using namespace sfv;
 
+namespace sfv {
+
+BatchEngine::~BatchEngine() {
+    if (stream_) (void)hipStreamSynchronize(stream_);
+    if (graph_) (void)hipGraphExecDestroy(graph_);
+    if (d_mem_) (void)hipFree(d_mem_);
+    if (d_hdr_) (void)hipFree(d_hdr_);
+    if (d_ctl_) (void)hipFree(d_ctl_);
+    if (h_mem_) (void)hipHostFree(h_mem_);
+    if (h_hdr_) (void)hipHostFree(h_hdr_);
+    if (h_ctl_) (void)hipHostFree(h_ctl_);
+    if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+#define ETRY(expr)                                                                                   \
+    do {                                                                                             \
+        hipError_t e_ = (expr);                                                                      \
+        if (e_ != hipSuccess) {                                                                      \
+            why = std::string(#expr) + ": " + hipGetErrorString(e_);                                 \
+            return SFV_ERR_HIP;                                                                      \
+        }                                                                                            \
+    } while (0)
+
+int BatchEngine::init(int precision, int capacity, std::string& why) {
+    precision_ = precision;
+    ETRY(hipGetDevice(&device_));
+    ETRY(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    ETRY(hipMalloc((void**)&d_hdr_, sizeof(BatchHeader)));
+    ETRY(hipHostMalloc((void**)&h_hdr_, sizeof(BatchHeader), hipHostMallocDefault));
+    hipDeviceProp_t prop;
+    int cus = 256;
+    if (hipGetDeviceProperties(&prop, device_) == hipSuccess && prop.multiProcessorCount > 0)
+        cus = prop.multiProcessorCount;
+    const int per_cu = std::min(march_batch_blocks_per_cu(precision, 1), march_batch_blocks_per_cu(precision, 2));
+    blocks_stage_ = std::max(8, (cus * per_cu + 7) / 8 * 8);
+    blocks_bc_ = std::max(1, cus);
+    return grow(std::max(1, capacity), why);
+}
+
+int BatchEngine::grow(int capacity, std::string& why) {
+    BatchMember* dm = nullptr;
+    StepCtl* dc = nullptr;
+    BatchMember* hm = nullptr;
+    StepCtl* hc = nullptr;
+    if (hipMalloc((void**)&dm, sizeof(BatchMember) * capacity) != hipSuccess ||
+        hipMalloc((void**)&dc, sizeof(StepCtl) * capacity) != hipSuccess ||
+        hipHostMalloc((void**)&hm, sizeof(BatchMember) * capacity, hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void**)&hc, sizeof(StepCtl) * capacity, hipHostMallocDefault) != hipSuccess) {
+        if (dm) (void)hipFree(dm);
+        if (dc) (void)hipFree(dc);
+        if (hm) (void)hipHostFree(hm);
+        if (hc) (void)hipHostFree(hc);
+        why = "allocation of the batch tables failed";
+        return SFV_ERR_OUT_OF_MEMORY;
+    }
+    std::memset(hc, 0, sizeof(StepCtl) * capacity);
+    if (d_ctl_) {   // members keep their step control across the move
+        ETRY(hipMemcpyAsync(dc, d_ctl_, sizeof(StepCtl) * capacity_, hipMemcpyDeviceToDevice, stream_));
+        std::memcpy(hc, h_ctl_, sizeof(StepCtl) * capacity_);
+        ETRY(hipStreamSynchronize(stream_));
+        (void)hipFree(d_mem_);
+        (void)hipFree(d_ctl_);
+        (void)hipHostFree(h_mem_);
+        (void)hipHostFree(h_ctl_);
+    }
+    d_mem_ = dm; d_ctl_ = dc; h_mem_ = hm; h_ctl_ = hc;
+    capacity_ = capacity;
+    slot_handle_.resize(capacity, nullptr);
+    slot_row_.resize(capacity);
+    ran_.resize(capacity, 0);
+    for (int s = 0; s < capacity; ++s) slot_row_[s].ctl = d_ctl_ + s;
+    if (graph_) {   // the graph holds the old table pointer
+        (void)hipGraphExecDestroy(graph_);
+        graph_ = nullptr;
+    }
+    table_dirty_ = true;
+    return SFV_OK;
+}
+
+int BatchEngine::add(sfv_handle* h, const RunParams& rp, int* slot_out, std::string& why) {
+    if (h->cfg.stepper != SFV_RUNGE_KUTTA2 || !h->fusable()) {
+        why = "batched members run the fused RK2 kernels (RungeKutta2, ghost sources inside the grid)";
+        return SFV_ERR_ARGUMENT;
+    }
+    if (h->cfg.precision != precision_) { why = "all members must have the same precision"; return SFV_ERR_ARGUMENT; }
+    if (h->device != device_) { why = "all members must live on one device"; return SFV_ERR_ARGUMENT; }
+    int s = -1;
+    for (int k = 0; k < capacity_ && s < 0; ++k)
+        if (!slot_handle_[k]) s = k;
+    if (s < 0) {
+        s = capacity_;
+        const int rc = grow(2 * capacity_, why);
+        if (rc) return rc;
+    }
+    ETRY(hipStreamSynchronize(h->stream));   // pending set_state / topography uploads
+    BatchMember M;
+    std::memset(&M, 0, sizeof(M));
+    M.P = h->P;
+    M.ctl = d_ctl_ + s;
+    M.U[0] = h->U[0];
+    M.U[1] = h->U[1];
+    M.RHS = h->RHS;
+    M.B = h->B;
+    M.cur0 = (int)(h->cur & 1);
+    M.slot0 = (int)(h->step_parity & 1);
+    M.my = march_rows_per_wave(h->P, 1);
+    const int segs = (h->P.ny + M.my - 1) / M.my;
+    M.strips = (h->P.nx + MARCH_COLS - 1) / MARCH_COLS;
+    M.seggroups = (segs + MARCH_WAVES - 1) / MARCH_WAVES;
+    M.t0 = rp.t0;
+    M.until_endtime = rp.until_endtime;
+    M.endtime = rp.until_endtime ? rp.endtime : NAN;
+    M.match_endtime = rp.match_endtime;
+    M.has_max_timestep = rp.maximum_timestep > 0 ? 1 : 0;
+    M.max_timestep = rp.maximum_timestep > 0 ? rp.maximum_timestep : 0.0;
+    M.max_steps = rp.max_steps;
+    M.precision = precision_;
+    ETRY(launch_prepare(stream_, h->P, M.ctl, M.slot0, M.t0, M.endtime, M.match_endtime, M.has_max_timestep,
+                        M.max_timestep, M.max_steps, M.until_endtime, precision_));
+    slot_row_[s] = M;
+    slot_handle_[s] = h;
+    ran_[s] = 0;
+    rows_.push_back(s);
+    table_dirty_ = true;
+    *slot_out = s;
+    return SFV_OK;
+}
+
+void BatchEngine::remove(int slot) {
+    sfv_handle* h = slot_handle_[slot];
+    if (!h) return;
+    if (ran_[slot]) {
+        const StepCtl& c = h_ctl_[slot];
+        h->cur = (h->cur + c.steps) & 1;
+        h->step_parity = (h->step_parity + c.steps) & 1;
+        if (c.steps > 0) { h->last_dt = c.dt; h->have_dt = true; }
+    }
+    slot_handle_[slot] = nullptr;
+    ran_[slot] = 0;
+    for (size_t r = 0; r < rows_.size(); ++r)
+        if (rows_[r] == slot) { rows_.erase(rows_.begin() + (long)r); break; }
+    table_dirty_ = true;
+}
+
+bool BatchEngine::finished(int slot) const {
+    if (!ran_[slot]) return false;
+    // replays end on launch parity 0, where the member's next slot is slot0
+    return h_ctl_[slot].slot[slot_row_[slot].slot0].active == 0;
+}
+
+int BatchEngine::upload_table(std::string& why) {
+    int tiles = 0, lines = 0;
+    for (size_t r = 0; r < rows_.size(); ++r) {
+        BatchMember& M = slot_row_[rows_[r]];
+        M.tile0 = tiles;
+        M.bc0 = lines;
+        tiles += M.strips * M.seggroups;
+        lines += M.P.nx + M.P.ny;
+        h_mem_[r] = M;
+    }
+    h_hdr_->members = (int)rows_.size();
+    h_hdr_->tiles = tiles;
+    h_hdr_->bc_lines = lines;
+    h_hdr_->pad = 0;
+    if (!rows_.empty())
+        ETRY(hipMemcpyAsync(d_mem_, h_mem_, sizeof(BatchMember) * rows_.size(), hipMemcpyHostToDevice, stream_));
+    ETRY(hipMemcpyAsync(d_hdr_, h_hdr_, sizeof(BatchHeader), hipMemcpyHostToDevice, stream_));
+    table_dirty_ = false;
+    return SFV_OK;
+}
+
+// graph_steps RK2 steps (stage 1, stage 2, boundary condition each), launch
+// parity alternating 0, 1, ... on the engine's stream.
+int BatchEngine::capture(std::string& why) {
+    ETRY(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+    hipError_t e = hipSuccess;
+    for (int s = 0; s < graph_steps_ && e == hipSuccess; ++s) {
+        const int lp = s & 1;
+        if (precision_ == SFV_F64) {
+            e = launch_march_batch<double>(stream_, 1, d_mem_, d_hdr_, lp, blocks_stage_);
+            if (e == hipSuccess) e = launch_march_batch<double>(stream_, 2, d_mem_, d_hdr_, lp, blocks_stage_);
+            if (e == hipSuccess) e = launch_bc_batch<double>(stream_, d_mem_, d_hdr_, lp, blocks_bc_);
+        } else {
+            e = launch_march_batch<float>(stream_, 1, d_mem_, d_hdr_, lp, blocks_stage_);
+            if (e == hipSuccess) e = launch_march_batch<float>(stream_, 2, d_mem_, d_hdr_, lp, blocks_stage_);
+            if (e == hipSuccess) e = launch_bc_batch<float>(stream_, d_mem_, d_hdr_, lp, blocks_bc_);
+        }
+    }
+    hipGraph_t g = nullptr;
+    const hipError_t e2 = hipStreamEndCapture(stream_, &g);
+    if (e != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        why = std::string("batched launch during capture: ") + hipGetErrorString(e);
+        return SFV_ERR_HIP;
+    }
+    if (e2 != hipSuccess) { why = std::string("hipStreamEndCapture: ") + hipGetErrorString(e2); return SFV_ERR_HIP; }
+    const hipError_t e3 = hipGraphInstantiate(&graph_, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e3 != hipSuccess) {
+        graph_ = nullptr;
+        why = std::string("hipGraphInstantiate: ") + hipGetErrorString(e3);
+        return SFV_ERR_HIP;
+    }
+    return SFV_OK;
+}
+
+int BatchEngine::replay(int k, std::string& why) {
+    int rc;
+    if (table_dirty_ && (rc = upload_table(why))) return rc;
+    if (!graph_ && (rc = capture(why))) return rc;
+    for (int r = 0; r < k; ++r) ETRY(hipGraphLaunch(graph_, stream_));
+    ETRY(hipMemcpyAsync(h_ctl_, d_ctl_, sizeof(StepCtl) * capacity_, hipMemcpyDeviceToHost, stream_));
+    ETRY(hipStreamSynchronize(stream_));
+    for (int s : rows_) ran_[s] = 1;
+    return SFV_OK;
+}
+
+#undef ETRY
+
+}  // namespace sfv
+
 struct sfv_batch {
     std::vector<sfv_handle*> members;
-    int device = 0;
-    int precision = SFV_F64;
-    hipStream_t stream = nullptr;
-    BatchMember* d_mem = nullptr;
-    BatchHeader* d_hdr = nullptr;
-    BatchMember* h_mem = nullptr;      // pinned staging
-    BatchHeader* h_hdr = nullptr;
-    StepCtl* d_ctl = nullptr;
-    StepCtl* h_ctl = nullptr;          // pinned mirror
-    int capacity = 0;
-    int blocks_stage = 0, blocks_bc = 0;
-    int graph_steps = 8;               // even: launch parity restarts at 0 on every replay
-    hipGraphExec_t graph = nullptr;
+    sfv::BatchEngine engine;
     std::string err;
-
-    ~sfv_batch() {
-        if (stream) (void)hipStreamSynchronize(stream);
-        if (graph) (void)hipGraphExecDestroy(graph);
-        if (d_mem) (void)hipFree(d_mem);
-        if (d_hdr) (void)hipFree(d_hdr);
-        if (d_ctl) (void)hipFree(d_ctl);
-        if (h_mem) (void)hipHostFree(h_mem);
-        if (h_hdr) (void)hipHostFree(h_hdr);
-        if (h_ctl) (void)hipHostFree(h_ctl);
-        if (stream) (void)hipStreamDestroy(stream);
-    }
 };
 
 namespace {
@@ -59,46 +256,6 @@ int bfail(sfv_batch* b, int code, const std::string& msg) {
     if (b) b->err = msg;
     else g_batch_error = msg;
     return code;
-}
-
-#define BHIP(b, expr)                                                                                \
-    do {                                                                                             \
-        hipError_t e_ = (expr);                                                                      \
-        if (e_ != hipSuccess)                                                                        \
-            return bfail((b), SFV_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));       \
-    } while (0)
-
-// Capture graph_steps RK2 steps (stage 1, stage 2, boundary condition each),
-// launch parity alternating 0, 1, ... on the batch stream.
-int capture(sfv_batch* b) {
-    BHIP(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-    hipError_t e = hipSuccess;
-    for (int s = 0; s < b->graph_steps && e == hipSuccess; ++s) {
-        const int lp = s & 1;
-        if (b->precision == SFV_F64) {
-            e = launch_march_batch<double>(b->stream, 1, b->d_mem, b->d_hdr, lp, b->blocks_stage);
-            if (e == hipSuccess) e = launch_march_batch<double>(b->stream, 2, b->d_mem, b->d_hdr, lp, b->blocks_stage);
-            if (e == hipSuccess) e = launch_bc_batch<double>(b->stream, b->d_mem, b->d_hdr, lp, b->blocks_bc);
-        } else {
-            e = launch_march_batch<float>(b->stream, 1, b->d_mem, b->d_hdr, lp, b->blocks_stage);
-            if (e == hipSuccess) e = launch_march_batch<float>(b->stream, 2, b->d_mem, b->d_hdr, lp, b->blocks_stage);
-            if (e == hipSuccess) e = launch_bc_batch<float>(b->stream, b->d_mem, b->d_hdr, lp, b->blocks_bc);
-        }
-    }
-    hipGraph_t g = nullptr;
-    const hipError_t e2 = hipStreamEndCapture(b->stream, &g);
-    if (e != hipSuccess) {
-        if (g) (void)hipGraphDestroy(g);
-        return bfail(b, SFV_ERR_HIP, std::string("batched launch during capture: ") + hipGetErrorString(e));
-    }
-    if (e2 != hipSuccess) return bfail(b, SFV_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e2));
-    const hipError_t e3 = hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(g);
-    if (e3 != hipSuccess) {
-        b->graph = nullptr;
-        return bfail(b, SFV_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e3));
-    }
-    return SFV_OK;
 }
 
 }  // namespace
@@ -125,33 +282,14 @@ int sfv_batch_create(sfv_handle* const* handles, int32_t n, sfv_batch** out) {
     }
     sfv_batch* b = new sfv_batch();
     b->members.assign(handles, handles + n);
-    b->device = handles[0]->device;
-    b->precision = handles[0]->cfg.precision;
-    b->capacity = n;
-    auto bail = [&](int code, const std::string& msg) {
+    std::string why;
+    int rc = hipSetDevice(handles[0]->device) == hipSuccess ? SFV_OK : SFV_ERR_HIP;
+    if (rc == SFV_OK) rc = b->engine.init(handles[0]->cfg.precision, n, why);
+    else why = "hipSetDevice failed";
+    if (rc) {
         delete b;
-        return bfail(nullptr, code, msg);
-    };
-    if (hipSetDevice(b->device) != hipSuccess) return bail(SFV_ERR_HIP, "hipSetDevice failed");
-    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess)
-        return bail(SFV_ERR_HIP, "hipStreamCreate failed");
-    if (hipMalloc((void**)&b->d_mem, sizeof(BatchMember) * n) != hipSuccess ||
-        hipMalloc((void**)&b->d_hdr, sizeof(BatchHeader)) != hipSuccess ||
-        hipMalloc((void**)&b->d_ctl, sizeof(StepCtl) * n) != hipSuccess)
-        return bail(SFV_ERR_OUT_OF_MEMORY, "hipMalloc of the batch tables failed");
-    if (hipHostMalloc((void**)&b->h_mem, sizeof(BatchMember) * n, hipHostMallocDefault) != hipSuccess ||
-        hipHostMalloc((void**)&b->h_hdr, sizeof(BatchHeader), hipHostMallocDefault) != hipSuccess ||
-        hipHostMalloc((void**)&b->h_ctl, sizeof(StepCtl) * n, hipHostMallocDefault) != hipSuccess)
-        return bail(SFV_ERR_OUT_OF_MEMORY, "hipHostMalloc of the batch staging failed");
-    std::memset(b->h_mem, 0, sizeof(BatchMember) * n);
-    std::memset(b->h_ctl, 0, sizeof(StepCtl) * n);
-    hipDeviceProp_t prop;
-    int cus = 256;
-    if (hipGetDeviceProperties(&prop, b->device) == hipSuccess && prop.multiProcessorCount > 0)
-        cus = prop.multiProcessorCount;
-    const int per_cu = std::min(march_batch_blocks_per_cu(b->precision, 1), march_batch_blocks_per_cu(b->precision, 2));
-    b->blocks_stage = std::max(8, (cus * per_cu + 7) / 8 * 8);
-    b->blocks_bc = std::max(1, cus);
+        return bfail(nullptr, rc, why);
+    }
     *out = b;
     return SFV_OK;
 }
@@ -170,82 +308,51 @@ int sfv_batch_run(sfv_batch* b, const double* t_end, int32_t match_endtime, doub
     const int n = (int)b->members.size();
     for (int i = 0; i < n; ++i)
         if (!b->members[i]->has_state) return bfail(b, SFV_ERR_STATE, "member " + std::to_string(i) + " has no state");
-    BHIP(b, hipSetDevice(b->device));
-    for (sfv_handle* h : b->members) BHIP(b, hipStreamSynchronize(h->stream));   // pending set_state etc.
-
-    // member table in the batch order (tiles of member 0, then member 1, ...)
-    int tiles = 0, lines = 0;
+    if (hipSetDevice(b->members[0]->device) != hipSuccess) return bfail(b, SFV_ERR_HIP, "hipSetDevice failed");
+    std::string why;
+    std::vector<int> slots(n, -1);
+    auto drop_all = [&]() {
+        for (int i = 0; i < n; ++i)
+            if (slots[i] >= 0) b->engine.remove(slots[i]);
+    };
     for (int i = 0; i < n; ++i) {
-        sfv_handle* h = b->members[i];
-        BatchMember& M = b->h_mem[i];
-        std::memset(&M, 0, sizeof(M));
-        M.P = h->P;
-        M.ctl = b->d_ctl + i;
-        M.U[0] = h->U[0];
-        M.U[1] = h->U[1];
-        M.RHS = h->RHS;
-        M.B = h->B;
-        M.cur0 = (int)(h->cur & 1);
-        M.slot0 = (int)(h->step_parity & 1);
-        M.my = march_rows_per_wave(h->P, 1);
-        const int segs = (h->P.ny + M.my - 1) / M.my;
-        M.strips = (h->P.nx + MARCH_COLS - 1) / MARCH_COLS;
-        M.seggroups = (segs + MARCH_WAVES - 1) / MARCH_WAVES;
-        M.tile0 = tiles;
-        M.bc0 = lines;
-        tiles += M.strips * M.seggroups;
-        lines += h->P.nx + h->P.ny;
-        M.t0 = t_io[i];
-        M.until_endtime = t_end ? 1 : 0;
-        M.endtime = t_end ? t_end[i] : NAN;
-        M.match_endtime = match_endtime ? 1 : 0;
-        M.has_max_timestep = maximum_timestep > 0 ? 1 : 0;
-        M.max_timestep = maximum_timestep > 0 ? maximum_timestep : 0.0;
-        M.max_steps = max_steps;
-        M.precision = b->precision;
+        sfv::RunParams rp;
+        rp.t0 = t_io[i];
+        rp.until_endtime = t_end ? 1 : 0;
+        rp.endtime = t_end ? t_end[i] : NAN;
+        rp.match_endtime = match_endtime ? 1 : 0;
+        rp.maximum_timestep = maximum_timestep;
+        rp.max_steps = max_steps;
+        const int rc = b->engine.add(b->members[i], rp, &slots[i], why);
+        if (rc) { drop_all(); return bfail(b, rc, why); }
     }
-    b->h_hdr->members = n;
-    b->h_hdr->tiles = tiles;
-    b->h_hdr->bc_lines = lines;
-    b->h_hdr->pad = 0;
-    BHIP(b, hipMemcpyAsync(b->d_mem, b->h_mem, sizeof(BatchMember) * n, hipMemcpyHostToDevice, b->stream));
-    BHIP(b, hipMemcpyAsync(b->d_hdr, b->h_hdr, sizeof(BatchHeader), hipMemcpyHostToDevice, b->stream));
-    BHIP(b, launch_prepare_batch(b->stream, b->d_mem, b->d_hdr, n));
-    if (!b->graph) {
-        const int rc = capture(b);
-        if (rc) return rc;
-    }
-
     // Replay until every member has finished; a fixed step count needs no
     // look at the device on the way.
-    int64_t launched = 0, replays_per_check = 1;
+    const int gs = b->engine.graph_steps();
+    int64_t launched = 0;
+    int per_check = 1;
     for (;;) {
-        int64_t k = replays_per_check;
-        if (max_steps > 0) k = std::min<int64_t>(k, (max_steps - launched + b->graph_steps - 1) / b->graph_steps);
-        if (!t_end) k = (max_steps - launched + b->graph_steps - 1) / b->graph_steps;
+        int64_t k = per_check;
+        if (max_steps > 0) k = std::min<int64_t>(k, (max_steps - launched + gs - 1) / gs);
+        if (!t_end) k = (max_steps - launched + gs - 1) / gs;
         if (k <= 0) break;
-        for (int64_t r = 0; r < k; ++r) BHIP(b, hipGraphLaunch(b->graph, b->stream));
-        launched += k * b->graph_steps;
-        BHIP(b, hipMemcpyAsync(b->h_ctl, b->d_ctl, sizeof(StepCtl) * n, hipMemcpyDeviceToHost, b->stream));
-        BHIP(b, hipStreamSynchronize(b->stream));
+        const int rc = b->engine.replay((int)k, why);
+        if (rc) { drop_all(); return bfail(b, rc, why); }
+        launched += k * gs;
         bool any = false;
-        for (int i = 0; i < n && !any; ++i) any = b->h_ctl[i].slot[(b->h_mem[i].slot0 + launched) & 1].active != 0;
+        for (int i = 0; i < n && !any; ++i) any = !b->engine.finished(slots[i]);
         if (!any) break;
-        replays_per_check = std::min<int64_t>(replays_per_check * 2, 16);
+        per_check = std::min(per_check * 2, 16);
     }
     int status = SFV_OK;
     for (int i = 0; i < n; ++i) {
-        sfv_handle* h = b->members[i];
-        const StepCtl& c = b->h_ctl[i];
-        h->cur = (h->cur + c.steps) & 1;
-        h->step_parity = (h->step_parity + c.steps) & 1;
-        if (c.steps > 0) { h->last_dt = c.dt; h->have_dt = true; }
+        const sfv::StepCtl& c = b->engine.ctl(slots[i]);
         t_io[i] = c.t;
         if (steps_out) steps_out[i] = c.steps;
-        if (c.nonfinite && status == SFV_OK) {
+        if (c.nonfinite && status == SFV_OK)
             status = bfail(b, SFV_ERR_NONFINITE_DT, "member " + std::to_string(i) + ": a time step produced a non-finite dt");
-        }
     }
+    drop_all();
     return status;
 }
 

@@ -1,0 +1,81 @@
+// sfv_engine.hpp — the batch engine behind sfv_batch_* and sfv_server_*: a
+// device member table, one step-control block per member slot, and a
+// captured graph of graph_steps RK2 steps of the batched kernels.
+//
+// Members are sfv_handle objects; the engine borrows their buffers.  A member
+// lives in a stable control slot (its StepCtl), while its row in the member
+// table may move when membership changes, so members can join and leave
+// between graph replays without disturbing the others.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "sfv_handle.hpp"
+
+namespace sfv {
+
+struct RunParams {
+    double t0 = 0.0;
+    double endtime = 0.0;         // NaN / unused when until_endtime = 0
+    int until_endtime = 0;
+    int match_endtime = 0;
+    double maximum_timestep = 0;  // <= 0: none
+    long long max_steps = 0;      // <= 0: unlimited
+};
+
+class BatchEngine {
+public:
+    ~BatchEngine();
+    // Set up for `precision` members on the current device.  capacity is the
+    // initial number of member slots (grows on demand).
+    int init(int precision, int capacity, std::string& why);
+
+    // Add a member (RK2, fusable, same precision) and prepare its step
+    // control for `rp`, from the member's current buffers and parity.
+    // Returns the member's slot in *slot_out.  Takes effect at the next replay.
+    int add(sfv_handle* h, const RunParams& rp, int* slot_out, std::string& why);
+    // Remove the member in `slot` (finished or cancelled), accounting its
+    // steps into the handle (buffer parity, last dt).
+    void remove(int slot);
+
+    // Replay the graph `k` times (k * graph_steps launched steps), then copy
+    // every slot's step control to the host and wait.
+    int replay(int k, std::string& why);
+
+    int members() const { return (int)rows_.size(); }
+    // After replay: has the member in `slot` finished its run?
+    bool finished(int slot) const;
+    const StepCtl& ctl(int slot) const { return h_ctl_[slot]; }
+    sfv_handle* handle(int slot) const { return slot_handle_[slot]; }
+    hipStream_t stream() const { return stream_; }
+    int graph_steps() const { return graph_steps_; }
+
+private:
+    int grow(int capacity, std::string& why);
+    int upload_table(std::string& why);
+    int capture(std::string& why);
+
+    int precision_ = 0;
+    int device_ = 0;
+    int capacity_ = 0;
+    int graph_steps_ = 8;         // even: launch parity is 0 at every replay start
+    int blocks_stage_ = 0, blocks_bc_ = 0;
+    hipStream_t stream_ = nullptr;
+    BatchMember* d_mem_ = nullptr;
+    BatchHeader* d_hdr_ = nullptr;
+    BatchMember* h_mem_ = nullptr;
+    BatchHeader* h_hdr_ = nullptr;
+    StepCtl* d_ctl_ = nullptr;
+    StepCtl* h_ctl_ = nullptr;
+    hipGraphExec_t graph_ = nullptr;
+    bool table_dirty_ = true;
+    std::vector<int> rows_;                  // slot of each table row
+    std::vector<sfv_handle*> slot_handle_;   // nullptr: free slot
+    std::vector<BatchMember> slot_row_;      // the member's row (host copy)
+    std::vector<char> ran_;                  // slot took part in the last replay
+};
+
+}  // namespace sfv

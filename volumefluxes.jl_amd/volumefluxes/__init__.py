@@ -29,5 +29,6 @@ from .simulator import (HIPBackend, IntervalWriter, MultipleCallbacks, Simulator
                         set_current_state, simulate_to_time)
 
 from .batch import Ensemble  # noqa: E402
+from .serving import Server  # noqa: E402
 
 __all__ = [n for n in dir() if not n.startswith("_")]

@@ -101,6 +101,37 @@ _ERROR_TYPES = {
 }
 
 
+class RequestInfo(ctypes.Structure):
+    """Mirror of ``struct sfv_request_info``."""
+
+    _fields_ = [("id", c_int64), ("status", c_int32), ("error", c_int32), ("t", c_double), ("steps", c_int64),
+                ("cell_updates", c_int64), ("device_bytes", c_int64), ("submit_s", c_double),
+                ("admit_s", c_double), ("done_s", c_double)]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class ServerStats(ctypes.Structure):
+    """Mirror of ``struct sfv_server_stats``."""
+
+    _fields_ = [(k, c_int64) for k in ("submitted", "admitted", "completed", "failed", "queued", "running",
+                                       "bytes_in_use", "bytes_peak", "budget_bytes", "cell_updates", "replays")] + \
+               [("busy_s", c_double), ("uptime_s", c_double)]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+SFV_REQ_QUEUED, SFV_REQ_RUNNING, SFV_REQ_DONE, SFV_REQ_FAILED = 0, 1, 2, 3
+
+
+def check_server(server, code: int) -> None:
+    if code != SFV_OK:
+        msg = lib().sfv_server_last_error(server)
+        raise_for(code, (msg or b"").decode(errors="replace"))
+
+
 def raise_for(code: int, message: str) -> None:
     if code == SFV_OK:
         return
@@ -165,6 +196,24 @@ def _declare(L) -> None:
     L.sfv_batch_run.argtypes = [b, POINTER(c_double), c_int32, c_double, c_int64, POINTER(c_double),
                                 POINTER(c_int64)]
     L.sfv_batch_run.restype = c_int32
+    sv = c_void_p
+    L.sfv_server_create.argtypes = [c_int64, c_int32, POINTER(c_void_p)]
+    L.sfv_server_create.restype = c_int32
+    L.sfv_server_destroy.argtypes = [sv]
+    L.sfv_server_destroy.restype = None
+    L.sfv_server_last_error.argtypes = [sv]
+    L.sfv_server_last_error.restype = c_char_p
+    L.sfv_server_submit.argtypes = [sv, cfgp, POINTER(c_double), c_int64, c_int64, c_void_p, c_double, c_double,
+                                    c_int32, c_double, POINTER(c_int64)]
+    L.sfv_server_submit.restype = c_int32
+    L.sfv_server_poll.argtypes = [sv, c_int64, POINTER(RequestInfo)]
+    L.sfv_server_poll.restype = c_int32
+    L.sfv_server_wait.argtypes = [sv, c_int64, c_double, POINTER(RequestInfo)]
+    L.sfv_server_wait.restype = c_int32
+    L.sfv_server_fetch.argtypes = [sv, c_int64, c_void_p, POINTER(RequestInfo)]
+    L.sfv_server_fetch.restype = c_int32
+    L.sfv_server_get_stats.argtypes = [sv, POINTER(ServerStats)]
+    L.sfv_server_get_stats.restype = c_int32
 
 
 def lib():
