@@ -90,6 +90,10 @@ typedef struct sfv_config {
     const double* infiltration_factor;   /* (nxT, nyT) field or NULL */
     int32_t rain_before_infiltration;    /* 1: rain precedes infiltration in the source list */
     int32_t precision;             /* enum sfv_precision */
+    int32_t embedded_1d;           /* 1: this (nx, 3) grid carries a CartesianGrid{1} run in every row
+                                      (bottom replicated along y), and B_cell is the 1D lookup
+                                      0.5*(B[i] + B[i+1]) (bottom_topography.jl:92) so the middle row
+                                      equals the 1D scheme bitwise; 0 otherwise */
 } sfv_config;
 
 /* Fill *cfg with the reference's defaults (gc=2, WallBC, g=9.81, ρ=1,

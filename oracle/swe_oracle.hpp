@@ -96,7 +96,7 @@ inline double desingularize(H h, double kappa) {
 struct Params {
     int nx, ny, gc, nxT, nyT;
     double dx, dy, g, rho, cutoff, kappa, theta, cfl, Cz;
-    int bc, rec, stepper, bottom, friction, rain, infil, rain_first;
+    int bc, rec, stepper, bottom, friction, rain, infil, rain_first, embedded_1d;
     double rain_rate, infil_rate, f0, fc, hk, factor_scalar;
     std::vector<double> rain_rates, rain_times, factor;
 };
@@ -114,6 +114,7 @@ inline Params make_params(const sfv_config& c) {
     p.rain_rate = c.rain_rate; p.infil_rate = c.infiltration_rate;
     p.f0 = c.horton_f0; p.fc = c.horton_fc; p.hk = c.horton_k;
     p.factor_scalar = c.infiltration_factor_scalar;
+    p.embedded_1d = c.embedded_1d;
     if (c.rain == SFV_RAIN_TIME_TABLE && c.rain_table_len > 0) {
         p.rain_rates.assign(c.rain_table_rates, c.rain_table_rates + c.rain_table_len);
         p.rain_times.assign(c.rain_table_times, c.rain_table_times + c.rain_table_len);
@@ -182,6 +183,8 @@ struct Sim2D {
 
     // bottom_topography.jl:97-101 (0-based corner (i,j) is the reference's B[i+1, j+1])
     inline double B_cell(int i, int j) const {
+        // bottom_topography.jl:97; the 1D lookup (:92) for a 1D run carried by an (nx, 3) grid
+        if (p.embedded_1d) return 0.5 * (Bc(i, j) + Bc(i + 1, j));
         return 0.25 * (Bc(i, j) + Bc(i + 1, j) + Bc(i, j + 1) + Bc(i + 1, j + 1));
     }
     inline double B_face_left(int i, int j, int d) const {

@@ -94,3 +94,57 @@ def dam_break_exact(x, t, x0, hl, hr, g=9.81):
     h[mid] = cm * cm / g
     u[mid] = 2 * (cl - cm)
     return h, u
+
+
+def oracle_1d_for(system, grid, stepper, U0, *, cfl=0.25):
+    """The 1D oracle (the reference's CartesianGrid{1} scheme) for a 1D
+    system description: the configuration of the 2D embedding with the grid
+    put back to 1D (ny = 1) and the 1D bottom and factor field."""
+    g2, s2 = VF.embed_1d(system, grid)
+    cfg = VF.build_config(s2, stepper, g2, cfl, C.SFV_F64)
+    cfg.ny = 1
+    cfg.dy = cfg.dx
+    keep = [cfg]
+    fac = [s for s in system.source_terms if isinstance(s, VF.HortonInfiltration)]
+    if fac:
+        f1 = np.ascontiguousarray(fac[0].factor.reshape(-1), dtype=np.float64)
+        cfg.infiltration_factor = C.dptr(f1)
+        keep.append(f1)
+    o = O.Oracle1D(cfg, np.ascontiguousarray(system.equation.B.corners(grid), dtype=np.float64))
+    o._keep = keep
+    o.set_interior(U0)
+    return o
+
+
+def system_1d(grid, B=None, *, sources=(), friction=None, rec=None, cutoff=1e-5, kappa=1e-5):
+    topo = VF.BottomTopography1D(B, None, grid) if B is not None else VF.ConstantBottomTopography()
+    eq = VF.ShallowWaterEquations1D(topo, depth_cutoff=cutoff, desingularizing_kappa=kappa)
+    return VF.ConservedSystem(None, rec if rec is not None else VF.LinearReconstruction(), VF.CentralUpwind(eq), eq,
+                              grid, list(sources), friction)
+
+
+def scenarios_1d():
+    """1D cases for the 2D-embedding checks: (name, system, grid, stepper, U0, cfl)."""
+    out = []
+    g = VF.CartesianGrid(256, gc=2, boundary=VF.WallBC(), extent=[[0.0, 1000.0]])
+    xc = g.cell_centers()
+    U = np.zeros((256, 2)); U[:, 0] = np.where(xc < 500.0, 2.0, 1.0)
+    out.append(("stoker", system_1d(g), g, VF.RungeKutta2(), U, 0.25))
+    U = np.zeros((256, 2)); U[:, 0] = np.where(xc < 500.0, 2.0, 0.0)
+    out.append(("ritter_fe", system_1d(g, cutoff=1e-7, kappa=1e-7), g, VF.ForwardEulerStepper(), U, 0.2))
+    gb = VF.CartesianGrid(300, gc=2, boundary=VF.WallBC(), extent=[[0.0, 300.0]])
+    xf = gb.cell_faces(interior=False)
+    B = 1e-3 * xf + 1.5 * np.exp(-((xf - 150.0) ** 2) / (2 * 20.0 ** 2))
+    Bc = 0.5 * (B[:-1] + B[1:])[2:-2]
+    U = np.zeros((300, 2)); U[:, 0] = np.maximum(Bc, 1.0)
+    out.append(("bump_rain_friction", system_1d(gb, B, sources=[VF.SourceTermBottom(), VF.ConstantRain(36.0)],
+                                                friction=VF.ImplicitFriction(Cz=0.03 ** 2), kappa=1e-4),
+                gb, VF.RungeKutta2(), U, 0.25))
+    gp = VF.CartesianGrid(200, gc=2, boundary=VF.PeriodicBC(), extent=[[0.0, 100.0]])
+    xcp = gp.cell_centers()
+    fac = np.linspace(0.5, 1.5, 204)
+    U = np.zeros((200, 2)); U[:, 0] = 1.0 + 0.3 * np.sin(2 * np.pi * xcp / 100.0); U[:, 1] = 0.2
+    out.append(("periodic_horton_table", system_1d(gp, sources=[VF.TimeDependentRain([10.0, 50.0, 0.0], [0.0, 2.0, 4.0]),
+                                                                 VF.HortonInfiltration(gp, factor=fac)]),
+                gp, VF.RungeKutta2(), U, 0.25))
+    return out

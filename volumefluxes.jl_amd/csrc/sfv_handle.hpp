@@ -134,6 +134,7 @@ struct sfv_handle {
         p.infil_rate = c.infiltration_rate;
         p.f0 = c.horton_f0; p.fc = c.horton_fc; p.hk = c.horton_k;
         p.factor = d_factor;
+        p.bcell_1d = c.embedded_1d ? 1 : 0;
         p.bc = c.bc;
         ghost_maps_into(c.bc, c.gc, p.nxT, p.src_lo_x, p.src_hi_x, p.flip_lo_x, p.flip_hi_x);
         ghost_maps_into(c.bc, c.gc, p.nyT, p.src_lo_y, p.src_hi_y, p.flip_lo_y, p.flip_hi_y);

@@ -37,6 +37,7 @@ struct KParams {
     // normal momentum negated if flip is set.  Built on the host by
     // replaying bc.jl's per-ghost assignments in order, so order-dependent
     // maps (NeumannBC's high side, grids narrower than gc) are exact.
+    int bcell_1d;   // B_cell = 0.5*(B[i,j] + B[i+1,j]) (a CartesianGrid{1} run embedded in (nx, 3))
     int bc;
     int src_lo_x[MAX_GC], src_hi_x[MAX_GC], src_lo_y[MAX_GC], src_hi_y[MAX_GC];
     int flip_lo_x[MAX_GC], flip_hi_x[MAX_GC], flip_lo_y[MAX_GC], flip_hi_y[MAX_GC];
@@ -117,9 +118,15 @@ __device__ __forceinline__ size_t cid(const KParams& P, int i, int j) { return (
 __device__ __forceinline__ double Bcorner(const KParams& P, const double* B, int i, int j) {
     return B[(size_t)i + (size_t)(P.nxT + 1) * j];
 }
-// bottom_topography.jl:97-101
+// B_cell from the four corners a = B[i,j], b = B[i+1,j], c = B[i,j+1],
+// d = B[i+1,j+1] in the reference's summation order (bottom_topography.jl:97),
+// or the 1D lookup 0.5*(B[i] + B[i+1]) (bottom_topography.jl:92) when the
+// grid carries a 1D run (bottom replicated along y).
+__device__ __forceinline__ double bcell4(const KParams& P, double a, double b, double c, double d) {
+    return P.bcell_1d ? 0.5 * (a + b) : 0.25 * (((a + b) + c) + d);
+}
 __device__ __forceinline__ double B_cell(const KParams& P, const double* B, int i, int j) {
-    return 0.25 * (Bcorner(P, B, i, j) + Bcorner(P, B, i + 1, j) + Bcorner(P, B, i, j + 1) + Bcorner(P, B, i + 1, j + 1));
+    return bcell4(P, Bcorner(P, B, i, j), Bcorner(P, B, i + 1, j), Bcorner(P, B, i, j + 1), Bcorner(P, B, i + 1, j + 1));
 }
 __device__ __forceinline__ double B_face_left(const KParams& P, const double* B, int i, int j, int d) {
     return d == 0 ? 0.5 * (Bcorner(P, B, i, j) + Bcorner(P, B, i, j + 1))

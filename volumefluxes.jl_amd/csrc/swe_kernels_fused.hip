@@ -124,8 +124,8 @@ __device__ __forceinline__ unsigned long long faces(const KParams& P, const Tile
             reconstruct_cell(c[0], c[1], c[2], P.theta, BlA, BrA, La, Ra);
             reconstruct_cell(c[1], c[2], c[3], P.theta, BlB, BrB, Lb, Rb);
         } else {
-            const double bca = 0.25 * (s.b[ay][ax] + s.b[ay][ax + 1] + s.b[ay + 1][ax] + s.b[ay + 1][ax + 1]);
-            const double bcb = 0.25 * (s.b[by][bx] + s.b[by][bx + 1] + s.b[by + 1][bx] + s.b[by + 1][bx + 1]);
+            const double bca = bcell4(P, s.b[ay][ax], s.b[ay][ax + 1], s.b[ay + 1][ax], s.b[ay + 1][ax + 1]);
+            const double bcb = bcell4(P, s.b[by][bx], s.b[by][bx + 1], s.b[by + 1][bx], s.b[by + 1][bx + 1]);
 #pragma unroll
             for (int v = 0; v < 3; ++v) { Ra[v] = c[1][v]; Lb[v] = c[2][v]; }
             Ra[0] = (S)(c[1][0] - bca);
@@ -290,9 +290,7 @@ __global__ void __launch_bounds__(F_THREADS) k_stage2(KParams P, StepCtl* ctl, i
         else if (j < gc) { neg = P.flip_lo_y[j] ? 2 : -1; j = P.src_lo_y[j]; }
         else if (j >= nyT - gc) { neg = P.flip_hi_y[j - (nyT - gc)] ? 2 : -1; j = P.src_hi_y[j - (nyT - gc)]; }
         const size_t k = (size_t)i + (size_t)nxT * j;
-        const double Bm = 0.25 * (B[(size_t)i + (size_t)(nxT + 1) * j] + B[(size_t)(i + 1) + (size_t)(nxT + 1) * j] +
-                                  B[(size_t)i + (size_t)(nxT + 1) * (j + 1)] +
-                                  B[(size_t)(i + 1) + (size_t)(nxT + 1) * (j + 1)]);
+        const double Bm = B_cell(P, B, i, j);
         S o[3];
         finish1<S>(P, U0[k], U0[n + k], U0[2 * n + k], RHS1[k], RHS1[n + k], RHS1[2 * n + k], Bm, dt_d, dt_s, o);
         if (neg >= 0) o[neg] = (S)(-o[neg]);
@@ -328,7 +326,7 @@ __global__ void __launch_bounds__(F_THREADS) k_stage2(KParams P, StepCtl* ctl, i
         o[1] = (S)(0.5 * (U0[n + k] + o[1]));
         o[2] = (S)(0.5 * (U0[2 * n + k] + o[2]));
         const int bx = cx + 2, by = cy + 2;
-        const double Bm = 0.25 * (s.b[by][bx] + s.b[by][bx + 1] + s.b[by + 1][bx] + s.b[by + 1][bx + 1]);
+        const double Bm = bcell4(P, s.b[by][bx], s.b[by][bx + 1], s.b[by + 1][bx], s.b[by + 1][bx + 1]);
         if (P.friction) {
             const double fs = friction_scalar(P.friction, P.Cz, P.phys.kappa, x0, x1, x2, Bm);
             apply_friction(o, dt_s, fs);
@@ -357,9 +355,7 @@ __global__ void __launch_bounds__(F_THREADS) k_stage2(KParams P, StepCtl* ctl, i
             const int i = hi_x ? nxT - gc + r % gc : r % gc;
             const int j = hi_y ? nyT - gc + r / gc : r / gc;
             const size_t k = (size_t)i + (size_t)nxT * j;
-            const double Bm = 0.25 * (B[(size_t)i + (size_t)(nxT + 1) * j] + B[(size_t)(i + 1) + (size_t)(nxT + 1) * j] +
-                                      B[(size_t)i + (size_t)(nxT + 1) * (j + 1)] +
-                                      B[(size_t)(i + 1) + (size_t)(nxT + 1) * (j + 1)]);
+            const double Bm = B_cell(P, B, i, j);
             const S src = apply_sources(P, ctl, S(0), k);
             const S x0 = U0[k], x1 = U0[n + k], x2 = U0[2 * n + k];
             S u1[3];

@@ -180,8 +180,8 @@ __device__ __forceinline__ void march_tile(const KParams& P, StepCtl* ctl, int s
             o.r[0] = RHS1[k]; o.r[1] = RHS1[n + k]; o.r[2] = RHS1[2 * n + k];
             o.remap = sci != i || sj != jr;
             if (o.remap)   // the source cell's bottom
-                o.bsrc = 0.25 * (Bload(P, B, sci, sj) + Bload(P, B, sci + 1, sj) + Bload(P, B, sci, sj + 1) +
-                                 Bload(P, B, sci + 1, sj + 1));
+                o.bsrc = bcell4(P, Bload(P, B, sci, sj), Bload(P, B, sci + 1, sj), Bload(P, B, sci, sj + 1),
+                                Bload(P, B, sci + 1, sj + 1));
         }
     };
     // bottom corner row: own column, and (lane 63 only) the next column
@@ -230,7 +230,7 @@ __device__ __forceinline__ void march_tile(const KParams& P, StepCtl* ctl, int s
         if (STAGE == 1) {
             cC[0] = cur.x[0]; cC[1] = cur.x[1]; cC[2] = cur.x[2];
         } else if (in_dom) {
-            const double Bm = cur.remap ? cur.bsrc : 0.25 * (((b0 + n0) + bp1) + np1);
+            const double Bm = cur.remap ? cur.bsrc : bcell4(P, b0, n0, bp1, np1);
             stage1_cell<S>(P, cur.x, cur.r, Bm, dt_d, dt_s, cC);
             if (fxc) cC[1] = (S)(-cC[1]);
             if (cur.fy) cC[2] = (S)(-cC[2]);
@@ -243,7 +243,7 @@ __device__ __forceinline__ void march_tile(const KParams& P, StepCtl* ctl, int s
         //    neighbours; bottom corner rows jr-1, jr)
         S Ly[3] = {S(0), S(0), S(0)}, Ry[3] = {S(0), S(0), S(0)};
         const double Bly = 0.5 * (bm1 + nm1), Bry = 0.5 * (b0 + n0);
-        const double bcq = 0.25 * (((bm1 + nm1) + b0) + n0);    // B_cell of row jr-1
+        const double bcq = bcell4(P, bm1, nm1, b0, n0);           // B_cell of row jr-1
         S Lx[3] = {S(0), S(0), S(0)}, Rx[3] = {S(0), S(0), S(0)}, Rxl[3] = {S(0), S(0), S(0)};
         const double Blx = 0.5 * (bm1 + b0), Brx = 0.5 * (nm1 + n0);
         if (FULL || r >= 1) {
@@ -392,8 +392,8 @@ __device__ __forceinline__ void march_tile(const KParams& P, StepCtl* ctl, int s
             const int ci = (corner & 1) ? nxT - gc + rr % gc : rr % gc;
             const int cj = (corner >> 1) ? nyT - gc + rr / gc : rr / gc;
             const size_t k = (size_t)ci + (size_t)nxT * cj;
-            const double Bm = 0.25 * (Bload(P, B, ci, cj) + Bload(P, B, ci + 1, cj) + Bload(P, B, ci, cj + 1) +
-                                      Bload(P, B, ci + 1, cj + 1));
+            const double Bm = bcell4(P, Bload(P, B, ci, cj), Bload(P, B, ci + 1, cj), Bload(P, B, ci, cj + 1),
+                                     Bload(P, B, ci + 1, cj + 1));
             const S src = apply_sources(P, ctl, S(0), k);
             const S x[3] = {U0[k], U0[n + k], U0[2 * n + k]};
             const S rs[3] = {src, S(0), S(0)};

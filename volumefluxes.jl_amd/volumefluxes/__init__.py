@@ -17,10 +17,10 @@ from ._capi import (ArgumentError, BudgetError, DomainError, NonFiniteTimestepEr
                     SfvError)
 from .grid import (XDIR, YDIR, CartesianGrid, NeumannBC, PeriodicBC, WallBC, cell_centers, cell_faces,
                    compute_dx)
-from .model import (BottomTopography2D, CentralUpwind, ConservedSystem, ConstantBottomTopography,
+from .model import (BottomTopography1D, BottomTopography2D, CentralUpwind, ConservedSystem, ConstantBottomTopography,
                     ConstantInfiltration, ConstantRain, ForwardEulerStepper, FunctionalRain,
                     HortonInfiltration, ImplicitFriction, LinearReconstruction, NoReconstruction, RungeKutta2,
-                    ShallowWaterEquations, SourceTermBottom, TimeDependentRain, build_config,
+                    ShallowWaterEquations, ShallowWaterEquations1D, SourceTermBottom, TimeDependentRain, build_config, embed_1d,
                     collect_topography_cells, compute_infiltration, compute_rain, friction_bh2021,
                     friction_bsa2012, friction_fcg2016, number_of_substeps, validate_bottom_topography)
 from .simulator import (HIPBackend, IntervalWriter, MultipleCallbacks, Simulator, advance_steps,

@@ -57,6 +57,7 @@ class SfvConfig(ctypes.Structure):
         ("infiltration_factor", POINTER(c_double)),
         ("rain_before_infiltration", c_int32),
         ("precision", c_int32),
+        ("embedded_1d", c_int32),
     ]
 
     def keepalive(self, *arrays):

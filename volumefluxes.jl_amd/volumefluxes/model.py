@@ -38,8 +38,7 @@ class ConstantBottomTopography:
         return self.B == 0.0
 
     def corners(self, grid: CartesianGrid) -> np.ndarray:
-        nxT, nyT = grid.totalcells
-        return np.full((nxT + 1, nyT + 1), self.B, order="F")
+        return np.full(tuple(t + 1 for t in grid.totalcells), self.B, order="F")
 
 
 def validate_bottom_topography(B: np.ndarray, grid: CartesianGrid) -> None:
@@ -47,6 +46,24 @@ def validate_bottom_topography(B: np.ndarray, grid: CartesianGrid) -> None:
     want = tuple(t + 1 for t in grid.totalcells)
     if tuple(B.shape) != want:
         raise _domain_error(f"Bottom topography should be of size {want} but got {tuple(B.shape)}")
+
+
+class BottomTopography1D:
+    """BottomTopography1D(B, backend, grid) (bottom_topography.jl:31-37): B
+    on the nxT+1 cell faces, ghosts included."""
+
+    def __init__(self, B, backend=None, grid: CartesianGrid | None = None):
+        B = np.asarray(B, dtype=np.float64)
+        if grid is not None:
+            validate_bottom_topography(B, grid)
+        self.B = B.copy()
+
+    def is_zero(self) -> bool:
+        return bool(np.all(self.B == 0.0))
+
+    def corners(self, grid: CartesianGrid) -> np.ndarray:
+        validate_bottom_topography(self.B, grid)
+        return self.B
 
 
 class BottomTopography2D:
@@ -104,6 +121,11 @@ class ShallowWaterEquations:
         self.g = float(g)
         self.depth_cutoff = float(depth_cutoff)
         self.desingularizing_kappa = float(desingularizing_kappa)
+
+
+class ShallowWaterEquations1D(ShallowWaterEquations):
+    """ShallowWaterEquations1D(B; ρ, g, depth_cutoff, desingularizing_kappa)
+    (swe_1D.jl:21-29): state (w, hu) on a CartesianGrid{1}."""
 
 
 class LinearReconstruction:
@@ -265,12 +287,57 @@ class ConservedSystem:
         self.grid = grid
         self.source_terms = list(source_terms)
         self.implicit_source_term = implicit_source_term
-        if isinstance(equation.B, BottomTopography2D):
+        if isinstance(equation.B, (BottomTopography1D, BottomTopography2D)):
             validate_bottom_topography(equation.B.B, grid)
         if not equation.B.is_zero() and not any(isinstance(s, SourceTermBottom) for s in self.source_terms):
             raise _argument_error(
                 "Found non-zero bottom topography in equation, but no corresponding source term. "
                 "Did you forget to add a source term to your system?")
+
+
+# ------------------------------------------------------------------ 1D runs
+EMBED_ROWS = 3
+
+
+def embed_1d(system: ConservedSystem, grid: CartesianGrid):
+    """The 2D system on an (nx, 3) grid that reproduces a CartesianGrid{1}
+    system bitwise in its middle row: the reference's own check is that the
+    2D scheme on (n, 3) with WallBC equals the 1D scheme bit for bit, dt
+    included (test_swashes.jl:317-328).  Every row carries the 1D state (hv =
+    0) and the 1D bottom, so all y flux differences vanish exactly; dy is
+    dx * 2^60, so the y wave speeds never set dt (min(dx/a_x, dy/a_y) =
+    dx/a_x, as in 1D).  The reference's identity holds for a flat bottom
+    only: the 2D cell bottom 0.25*(((a+b)+c)+d) rounds differently from the
+    1D 0.5*(a+b) (bottom_topography.jl:92,97), so the embedded system is
+    marked and B_cell takes the 1D lookup.  Returns (grid2d, system2d)."""
+    if grid.dimension != 1:
+        raise _argument_error("embed_1d takes a CartesianGrid{1}")
+    x0, x1 = grid.extent
+    dy = grid.delta[0] * 2.0 ** 60
+    g2 = CartesianGrid(grid.nx, EMBED_ROWS, gc=grid.gc, boundary=grid.boundary,
+                       extent=[[x0, x1], [0.0, EMBED_ROWS * dy]])
+    eq = system.equation
+    nyc = g2.totalcells[1] + 1
+    if isinstance(eq.B, ConstantBottomTopography):
+        topo = ConstantBottomTopography(eq.B.B)
+    else:
+        B1 = eq.B.corners(grid)
+        topo = BottomTopography2D(np.asfortranarray(np.repeat(B1[:, None], nyc, axis=1)), None, g2)
+    eq2 = ShallowWaterEquations(topo, rho=eq.rho, g=eq.g, depth_cutoff=eq.depth_cutoff,
+                                desingularizing_kappa=eq.desingularizing_kappa)
+    sources = []
+    for st in system.source_terms:
+        if isinstance(st, HortonInfiltration):
+            fac = np.asfortranarray(np.repeat(st.factor.reshape(-1)[:, None], g2.totalcells[1], axis=1))
+            sources.append(HortonInfiltration(g2, factor=fac, fc=st.fc, f0=st.f0, k=st.k))
+        elif isinstance(st, FunctionalRain):
+            raise _argument_error("FunctionalRain takes (t, x, y); a 1D run has no y")
+        else:
+            sources.append(st)
+    sys2 = ConservedSystem(None, system.reconstruction, CentralUpwind(eq2), eq2, g2, sources,
+                           system.implicit_source_term)
+    sys2.embedded_1d = True           # B_cell takes the 1D lookup (build_config)
+    return g2, sys2
 
 
 def build_config(system: ConservedSystem, stepper, grid: CartesianGrid, cfl: float, precision: int) -> C.SfvConfig:
@@ -327,5 +394,6 @@ def build_config(system: ConservedSystem, stepper, grid: CartesianGrid, cfl: flo
     order = [s for s in system.source_terms if s in rains or s in infs]
     cfg.rain_before_infiltration = 1 if (not order or not infs or order[0] in rains) else 0
     cfg.precision = precision
+    cfg.embedded_1d = 1 if getattr(system, "embedded_1d", False) else 0
     cfg.keepalive(*keep)
     return cfg

@@ -15,9 +15,10 @@ import numpy as np
 
 from . import _capi as C
 from .grid import CartesianGrid
-from .model import ConservedSystem, RungeKutta2, build_config
+from .model import EMBED_ROWS, ConservedSystem, RungeKutta2, build_config, embed_1d
 
 InteriorState = namedtuple("InteriorState", ["h", "hu", "hv"])
+InteriorState1D = namedtuple("InteriorState1D", ["h", "hu"])
 
 
 class HIPBackend:
@@ -77,12 +78,15 @@ class Simulator:
         self.cfl = float(self.realtype.type(cfl))
         self.t = float(self.realtype.type(t0))
         self.current_timestep = 0.0
-        self._cfg = build_config(system, timestepper, grid, cfl, backend.precision)
+        # A CartesianGrid{1} run is the exactly equivalent 2D run on (nx, 3)
+        # (model.embed_1d); the state the user sees is its middle row.
+        self._grid2, self._system2 = (embed_1d(system, grid) if grid.dimension == 1 else (grid, system))
+        self._cfg = build_config(self._system2, timestepper, self._grid2, cfl, backend.precision)
         L = C.lib()
         h = ctypes.c_void_p()
         C.check(None, L.sfv_create(ctypes.byref(self._cfg), ctypes.byref(h)))
         self._h = h
-        B = np.asfortranarray(system.equation.B.corners(grid), dtype=np.float64)
+        B = np.asfortranarray(self._system2.equation.B.corners(self._grid2), dtype=np.float64)
         C.check(self._h, L.sfv_set_topography(self._h, C.dptr(B), B.shape[0], B.shape[1]))
 
     def __del__(self):
@@ -97,17 +101,26 @@ class Simulator:
         return self._h
 
     def _state_shape(self):
+        if self.grid.dimension == 1:
+            return (self.grid.totalcells[0], 2)
         return (*self.grid.totalcells, 3)
 
     def get_full_state(self) -> np.ndarray:
-        U = np.empty(self._state_shape(), dtype=self.realtype, order="F")
+        """The full state with ghost cells: (nxT, nyT, 3), or (nxT, 2) in 1D."""
+        U = np.empty((*self._grid2.totalcells, 3), dtype=self.realtype, order="F")
         C.check(self._h, C.lib().sfv_get_state(self._h, U.ctypes.data_as(ctypes.c_void_p), C.SFV_LAYOUT_FULL))
+        if self.grid.dimension == 1:
+            return np.array(U[:, self.grid.gc + 1, :2], order="F")
         return U
 
     def set_full_state(self, U) -> None:
         U = np.asfortranarray(U, dtype=self.realtype)
         if U.shape != self._state_shape():
             raise C.DomainError(C.SFV_ERR_DOMAIN, f"state should be of size {self._state_shape()} but got {U.shape}")
+        if self.grid.dimension == 1:
+            U2 = np.zeros((*self._grid2.totalcells, 3), dtype=self.realtype, order="F")
+            U2[:, :, :2] = U[:, None, :]
+            U = U2
         C.check(self._h, C.lib().sfv_set_state(self._h, U.ctypes.data_as(ctypes.c_void_p), C.SFV_LAYOUT_FULL))
 
     KERNELS = {"per-pass": 0, "tiled": 1, "march": 2}
@@ -137,6 +150,9 @@ def current_interior_state(sim: Simulator) -> InteriorState:
     """current_interior_state(sim) (simulator.jl:113), per-variable interior views."""
     U = sim.get_full_state()
     g = sim.grid.gc
+    if sim.grid.dimension == 1:
+        I = U[g:-g, :]
+        return InteriorState1D(np.array(I[:, 0]), np.array(I[:, 1]))
     I = U[g:-g, g:-g, :]
     return InteriorState(np.array(I[:, :, 0]), np.array(I[:, :, 1]), np.array(I[:, :, 2]))
 
@@ -150,6 +166,17 @@ def current_timestep(sim: Simulator) -> float:
 
 
 def _as_interior_array(sim: Simulator, new_state) -> np.ndarray:
+    if sim.grid.dimension == 1:
+        nx = sim.grid.nx
+        if isinstance(new_state, (tuple, list)) and len(new_state) == 2:
+            arr = np.stack([np.asarray(a, dtype=np.float64) for a in new_state], axis=-1)
+        else:
+            arr = np.asarray(new_state, dtype=np.float64)
+        if arr.shape != (nx, 2):
+            raise C.DomainError(C.SFV_ERR_DOMAIN, f"state should be of size {(nx, 2)} but got {arr.shape}")
+        U2 = np.zeros((nx, EMBED_ROWS, 3))
+        U2[:, :, :2] = arr[:, None, :]
+        return np.asfortranarray(U2.astype(sim.realtype))
     nx, ny = sim.grid.interior_size()
     if isinstance(new_state, (tuple, list)) and len(new_state) == 3:
         arr = np.stack([np.asarray(a, dtype=np.float64) for a in new_state], axis=-1)
