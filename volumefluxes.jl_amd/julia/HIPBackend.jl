@@ -63,6 +63,7 @@ mutable struct SfvConfig
     infiltration_factor::Ptr{Float64}
     rain_before_infiltration::Int32
     precision::Int32
+    embedded_1d::Int32
     SfvConfig() = new()
 end
 
@@ -174,6 +175,7 @@ function build_config(sim)
         c.friction, c.Cz = friction_code(fr.friction_function), fr.Cz
     end
     c.precision = sim.backend.realtype === Float32 ? 1 : 0
+    c.embedded_1d = 0     # 2D grids; a CartesianGrid{1} run would use the (nx, 3) embedding with 1
     return c, keep
 end
 
