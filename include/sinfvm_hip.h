@@ -49,7 +49,10 @@ enum sfv_reconstruction { SFV_NO_RECONSTRUCTION = 0, SFV_LINEAR_RECONSTRUCTION =
 enum sfv_friction {                                                                 /* friction.jl:30-43 */
     SFV_FRICTION_NONE = 0, SFV_FRICTION_BH2021 = 1, SFV_FRICTION_BSA2012 = 2, SFV_FRICTION_FCG2016 = 3
 };
-enum sfv_rain { SFV_RAIN_NONE = 0, SFV_RAIN_CONSTANT = 1, SFV_RAIN_TIME_TABLE = 2 }; /* source_term_rain.jl */
+enum sfv_rain {                                                                     /* source_term_rain.jl */
+    SFV_RAIN_NONE = 0, SFV_RAIN_CONSTANT = 1, SFV_RAIN_TIME_TABLE = 2,
+    SFV_RAIN_FIELD = 3    /* FunctionalRain: per-cell rates the caller sets before each step (sfv_set_rain_field) */
+};
 enum sfv_infiltration {                                                             /* source_term_infiltration.jl */
     SFV_INFILTRATION_NONE = 0, SFV_INFILTRATION_CONSTANT = 1, SFV_INFILTRATION_HORTON = 2
 };
@@ -126,6 +129,16 @@ int sfv_set_topography(sfv_handle* h, const double* B, int64_t nbx, int64_t nby)
 int sfv_set_state(sfv_handle* h, const void* U, int32_t layout);
 /* collect(current_state(sim)) / current_interior_state (simulator.jl:98-113). */
 int sfv_get_state(sfv_handle* h, void* U, int32_t layout);
+
+/* FunctionalRain (source_term_rain.jl:57-68): the rates rain_function(t, x, y)
+ * at the cell centres of ALL cells, ghosts included (cell_center,
+ * grid.jl:117-121), as a (nxT, nyT) array, x fastest, added to w as they are
+ * (no /3600).  The reference evaluates the function at the step's start time
+ * t for both stages, so the caller sets the field for t before each sfv_step.
+ * Only for a handle created with rain = SFV_RAIN_FIELD; such a handle steps
+ * with sfv_step only (sfv_run / sfv_advance / batches return
+ * SFV_ERR_ARGUMENT, since the field must change every step). */
+int sfv_set_rain_field(sfv_handle* h, const double* field);
 
 /* perform_step!(sim, max_dt) (simulator.jl:145-177): one full time step with
  * sources evaluated at time t.  Writes the step's dt (current_timestep) to

@@ -50,6 +50,7 @@ def lib():
         L.oracle2d_create.restype = c_void_p
         L.oracle2d_destroy.argtypes = [c_void_p]
         L.oracle2d_set_topography.argtypes = [c_void_p, POINTER(d)]
+        L.oracle2d_set_rain_field.argtypes = [c_void_p, POINTER(d)]
         L.oracle2d_set_interior.argtypes = [c_void_p, POINTER(d)]
         L.oracle2d_set_full.argtypes = [c_void_p, c_void_p]
         L.oracle2d_get_full.argtypes = [c_void_p, c_void_p]
@@ -112,6 +113,11 @@ class Oracle2D:
         assert B.shape == (self.nxT + 1, self.nyT + 1)
         self._B = B
         lib().oracle2d_set_topography(self._h, _dp(B))
+
+    def set_rain_field(self, field) -> None:
+        """FunctionalRain rates of the coming step on all (nxT, nyT) cells."""
+        f = np.asfortranarray(field, dtype=np.float64)
+        lib().oracle2d_set_rain_field(self._h, _dp(f))
 
     def B_lookup(self, which: int, i: int, j: int, d: int = 0) -> float:
         return lib().oracle2d_B_lookup(self._h, which, i, j, d)

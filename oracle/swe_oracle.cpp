@@ -88,6 +88,12 @@ void oracle2d_destroy(void* hp) {
     delete h->f;
     delete h;
 }
+void oracle2d_set_rain_field(void* hp, const double* field) {
+    Handle2D* h = (Handle2D*)hp;
+    auto& f = h->d ? h->d->p.rain_field : h->f->p.rain_field;
+    f.assign(field, field + f.size());
+}
+
 void oracle2d_set_topography(void* hp, const double* B) {
     Handle2D* h = (Handle2D*)hp;
     if (h->d) std::memcpy(h->d->B.data(), B, h->d->B.size() * sizeof(double));

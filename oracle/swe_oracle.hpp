@@ -99,6 +99,7 @@ struct Params {
     int bc, rec, stepper, bottom, friction, rain, infil, rain_first, embedded_1d;
     double rain_rate, infil_rate, f0, fc, hk, factor_scalar;
     std::vector<double> rain_rates, rain_times, factor;
+    std::vector<double> rain_field;   // FunctionalRain: rain_function(t, cell_center) of the current step
 };
 
 inline Params make_params(const sfv_config& c) {
@@ -119,6 +120,7 @@ inline Params make_params(const sfv_config& c) {
         p.rain_rates.assign(c.rain_table_rates, c.rain_table_rates + c.rain_table_len);
         p.rain_times.assign(c.rain_table_times, c.rain_table_times + c.rain_table_len);
     }
+    if (c.rain == SFV_RAIN_FIELD) p.rain_field.assign((size_t)p.nxT * p.nyT, 0.0);
     if (c.infiltration == SFV_INFILTRATION_HORTON) {
         const size_t n = (size_t)p.nxT * p.nyT;
         if (c.infiltration_factor) p.factor.assign(c.infiltration_factor, c.infiltration_factor + n);
@@ -334,6 +336,11 @@ struct Sim2D {
     void nondirectional_sources(std::vector<S>& o, double t) {
         auto rain = [&]() {
             if (p.rain == SFV_RAIN_NONE) return;
+            if (p.rain == SFV_RAIN_FIELD) {   // source_term_rain.jl:64-68, values set by the caller for t
+                #pragma omp parallel for schedule(static)
+                for (long k = 0; k < (long)ncell; ++k) o[k] = (S)(o[k] + p.rain_field[k]);
+                return;
+            }
             const double r = compute_rain(p, t);
             #pragma omp parallel for schedule(static)
             for (long k = 0; k < (long)ncell; ++k) o[k] = (S)(o[k] + r);

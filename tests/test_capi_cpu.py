@@ -122,13 +122,13 @@ def test_infiltration_factor_size_is_domain_error():
     assert VF.HortonInfiltration(grid).factor.shape == grid.size
 
 
-def test_functional_rain_is_rejected_by_the_hip_backend():
+def test_functional_rain_maps_to_a_rate_field():
     grid = VF.CartesianGrid(4, 4, gc=2)
     eq = VF.ShallowWaterEquations()
     system = VF.ConservedSystem(None, VF.LinearReconstruction(), VF.CentralUpwind(eq), eq, grid,
                                 [VF.FunctionalRain(lambda t, x, y: 0.0, grid)])
-    with pytest.raises(VF.ArgumentError):
-        VF.build_config(system, VF.RungeKutta2(), grid, 0.25, C.SFV_F64)
+    cfg = VF.build_config(system, VF.RungeKutta2(), grid, 0.25, C.SFV_F64)
+    assert cfg.rain == C.SFV_RAIN_FIELD
 
 
 def test_build_config_maps_reference_system():

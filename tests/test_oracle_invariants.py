@@ -309,3 +309,62 @@ def test_float32_storage_tracks_float64():
     w64, w32 = o64.state()[2:-2, 2:-2, 0], o32.state()[2:-2, 2:-2, 0].astype(np.float64)
     assert np.max(np.abs(w64 - w32)) / np.max(np.abs(w64)) < 1e-5
     assert abs(o64.t - o32.t) / o64.t < 1e-5
+
+
+# ---------------------------------------------------------- FunctionalRain
+# test/test_rain.jl:98-160: the same lake, rain from a host function f(t, x, y)
+# evaluated on every cell centre at the step's start time.
+def _functional_rain_run(fn, hours, nx=10):
+    grid = VF.CartesianGrid(nx, nx, gc=2, boundary=VF.WallBC(), extent=[[0.0, 1000.0], [0.0, 1000.0]])
+    rain = VF.FunctionalRain(fn, grid)
+    U0 = np.zeros((nx, nx, 3)); U0[:, :, 0] = 1.0
+    o = oracle_2d(grid, system_2d(grid, sources=[rain]), FE, U0)
+    T, t = hours * 3600.0, 0.0
+    while t < T:
+        o.set_rain_field(VF.rain_field_values(rain, grid, t))
+        t += o.step(T - t, t)
+    assert abs(t - T) <= 1e-10
+    return o.state()[2:-2, 2:-2]
+
+
+def test_functional_rain_constant():
+    U = _functional_rain_run(lambda t, x, y: 0.01 / 3600.0, 3)
+    assert np.max(np.abs(U[:, :, 0] - (3 * 0.01 + 1.0))) <= 1e-12
+    assert np.all(U[:, :, 1] == 0.0) and np.all(U[:, :, 2] == 0.0)
+
+
+def test_functional_rain_time_dependent():
+    def fn(t, x, y):
+        rate = 15.0 * 0.001 if t < 3600 else (0.3 * 0.001 if t < 7200 else 20.0 * 0.001)
+        return rate / 3600
+    U = _functional_rain_run(fn, 3)
+    assert np.max(np.abs(U[:, :, 0] - (0.0353 + 1.0))) <= 1e-4
+
+
+def test_functional_rain_spatial_total():
+    def fn(t, x, y):
+        return 3 * (x / 500) * (y / 500) * 0.01 / 3600.0 if 3600.0 < t < 7200.0 else 0.0 * x
+    U = _functional_rain_run(fn, 3)
+    assert abs(U[:, :, 0].sum() - 100 * 1.03) <= 1e-2
+
+
+def test_functional_rain_equals_constant_rain_bitwise():
+    """A FunctionalRain returning rate/3600 is ConstantRain(rate) bit for bit."""
+    Uf = _functional_rain_run(lambda t, x, y: 0.036 / 3600, 1)
+    Uc = _rain_run(VF.ConstantRain(0.036), 1)
+    assert np.array_equal(Uf, Uc)
+
+
+def test_cell_center_coordinates_follow_grid_jl():
+    grid = VF.CartesianGrid(7, 5, gc=2, extent=[[0.5, 7.5], [-1.0, 9.0]])
+    xs, ys = VF.model.cell_center_coordinates(grid)
+    for i in range(grid.totalcells[0]):   # grid.jl:117-121 with 1-based I = i + 1
+        assert xs[i] == 0.5 + grid.delta[0] * ((i + 1) - 0.5 - 2)
+    for j in range(grid.totalcells[1]):
+        assert ys[j] == -1.0 + grid.delta[1] * ((j + 1) - 0.5 - 2)
+    assert np.allclose(xs[2:-2], grid.cell_centers()[:, 0, 0]) and np.allclose(ys[2:-2], grid.cell_centers()[0, :, 1])
+    calls = []
+    f = VF.rain_field_values(VF.FunctionalRain(lambda t, x, y: calls.append(1) or (x + 10 * y if x < 3 else 0.0), grid),
+                             grid, 0.0)   # scalar-only function: evaluated per cell
+    assert f.shape == grid.totalcells and len(calls) >= grid.totalcells[0] * grid.totalcells[1]
+    assert f[0, 0] == xs[0] + 10 * ys[0]

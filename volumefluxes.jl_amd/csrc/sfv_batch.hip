@@ -106,6 +106,10 @@ int BatchEngine::add(sfv_handle* h, const RunParams& rp, int* slot_out, std::str
         return SFV_ERR_ARGUMENT;
     }
     if (h->cfg.precision != precision_) { why = "all members must have the same precision"; return SFV_ERR_ARGUMENT; }
+    if (h->cfg.rain == SFV_RAIN_FIELD) {
+        why = "FunctionalRain needs its rate field set before every step; it cannot run batched";
+        return SFV_ERR_ARGUMENT;
+    }
     if (h->device != device_) { why = "all members must live on one device"; return SFV_ERR_ARGUMENT; }
     int s = -1;
     for (int k = 0; k < capacity_ && s < 0; ++k)
@@ -275,6 +279,9 @@ int sfv_batch_create(sfv_handle* const* handles, int32_t n, sfv_batch** out) {
             return bfail(nullptr, SFV_ERR_ARGUMENT,
                          "member " + std::to_string(i) +
                              ": batches run the fused RK2 kernels (RungeKutta2, ghost sources inside the grid)");
+        if (h->cfg.rain == SFV_RAIN_FIELD)
+            return bfail(nullptr, SFV_ERR_ARGUMENT,
+                         "member " + std::to_string(i) + ": FunctionalRain needs its rate field every step");
         if (h->device != handles[0]->device)
             return bfail(nullptr, SFV_ERR_ARGUMENT, "all members must live on one device");
         if (h->cfg.precision != handles[0]->cfg.precision)

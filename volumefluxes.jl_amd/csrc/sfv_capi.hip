@@ -73,7 +73,7 @@ int validate(const sfv_config* c, std::string& why) {
     if (c->bc < SFV_BC_WALL || c->bc > SFV_BC_NEUMANN) { why = "unknown boundary condition"; return SFV_ERR_ARGUMENT; }
     if (c->stepper != SFV_FORWARD_EULER && c->stepper != SFV_RUNGE_KUTTA2) { why = "unknown stepper"; return SFV_ERR_ARGUMENT; }
     if (c->friction < SFV_FRICTION_NONE || c->friction > SFV_FRICTION_FCG2016) { why = "unknown friction law"; return SFV_ERR_ARGUMENT; }
-    if (c->rain < SFV_RAIN_NONE || c->rain > SFV_RAIN_TIME_TABLE) { why = "unknown rain kind"; return SFV_ERR_ARGUMENT; }
+    if (c->rain < SFV_RAIN_NONE || c->rain > SFV_RAIN_FIELD) { why = "unknown rain kind"; return SFV_ERR_ARGUMENT; }
     if (c->rain == SFV_RAIN_TIME_TABLE && (c->rain_table_len < 1 || !c->rain_table_rates || !c->rain_table_times)) {
         why = "TimeDependentRain needs rates and times of equal, non-zero length (source_term_rain.jl:36-38)";
         return SFV_ERR_DOMAIN;
@@ -276,6 +276,16 @@ int sfv_set_topography(sfv_handle* h, const double* B, int64_t nbx, int64_t nby)
     return SFV_OK;
 }
 
+int sfv_set_rain_field(sfv_handle* h, const double* field) {
+    if (!h || !field) return fail(h, SFV_ERR_ARGUMENT, "NULL argument");
+    if (h->cfg.rain != SFV_RAIN_FIELD || !h->d_rain_field)
+        return fail(h, SFV_ERR_ARGUMENT, "the handle was not created with rain = SFV_RAIN_FIELD");
+    HIP_TRY(h, hipSetDevice(h->device));
+    HIP_TRY(h, hipMemcpyAsync(h->d_rain_field, field, h->ncell * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    HIP_TRY(h, hipStreamSynchronize(h->stream));
+    return SFV_OK;
+}
+
 int sfv_set_state(sfv_handle* h, const void* U, int32_t layout) {
     if (!h || !U) return fail(h, SFV_ERR_ARGUMENT, "NULL argument");
     HIP_TRY(h, hipSetDevice(h->device));
@@ -341,6 +351,8 @@ int sfv_run(sfv_handle* h, double t_end, int32_t match_endtime, double maximum_t
     int rc = ensure_ready(h);
     if (rc) return rc;
     if (!t_io) return fail(h, SFV_ERR_ARGUMENT, "t_io is NULL");
+    if (h->cfg.rain == SFV_RAIN_FIELD)
+        return fail(h, SFV_ERR_ARGUMENT, "FunctionalRain needs its rate field set before every step: use sfv_step");
     HIP_TRY(h, hipSetDevice(h->device));
     const int has_max = maximum_timestep > 0 ? 1 : 0;
     return run_loop(h, *t_io, t_end, match_endtime ? 1 : 0, has_max, has_max ? maximum_timestep : 0.0,
@@ -351,6 +363,8 @@ int sfv_advance(sfv_handle* h, int64_t nsteps, double max_dt, double* t_io) {
     int rc = ensure_ready(h);
     if (rc) return rc;
     if (!t_io) return fail(h, SFV_ERR_ARGUMENT, "t_io is NULL");
+    if (h->cfg.rain == SFV_RAIN_FIELD)
+        return fail(h, SFV_ERR_ARGUMENT, "FunctionalRain needs its rate field set before every step: use sfv_step");
     if (nsteps <= 0) return SFV_OK;
     HIP_TRY(h, hipSetDevice(h->device));
     int64_t steps = 0;

@@ -30,6 +30,7 @@ struct sfv_handle {
     double* B = nullptr;               // corners (nxT+1, nyT+1)
     double* d_rain = nullptr;          // rates then times
     double* d_factor = nullptr;        // Horton factor field
+    double* d_rain_field = nullptr;    // FunctionalRain rates per cell
     sfv::StepCtl* ctl = nullptr;       // device step-control block
     sfv::StepCtl* ctl_host = nullptr;  // pinned mirror
     int64_t cur = 0;                   // index of the current state buffer
@@ -90,6 +91,10 @@ struct sfv_handle {
             (void)hipMemcpyAsync(d_rain + rain_rates.size(), rain_times.data(), rain_times.size() * sizeof(double),
                            hipMemcpyHostToDevice, stream);
         }
+        if (c.rain == SFV_RAIN_FIELD) {
+            if ((rc = alloc((void**)&d_rain_field, ncell * sizeof(double), why))) return rc;
+            (void)hipMemsetAsync(d_rain_field, 0, ncell * sizeof(double), stream);
+        }
         if (!factor.empty()) {
             if ((rc = alloc((void**)&d_factor, ncell * sizeof(double), why))) return rc;
             (void)hipMemcpyAsync(d_factor, factor.data(), ncell * sizeof(double), hipMemcpyHostToDevice, stream);
@@ -134,6 +139,7 @@ struct sfv_handle {
         p.infil_rate = c.infiltration_rate;
         p.f0 = c.horton_f0; p.fc = c.horton_fc; p.hk = c.horton_k;
         p.factor = d_factor;
+        p.rain_field = d_rain_field;
         p.bcell_1d = c.embedded_1d ? 1 : 0;
         p.bc = c.bc;
         ghost_maps_into(c.bc, c.gc, p.nxT, p.src_lo_x, p.src_hi_x, p.flip_lo_x, p.flip_hi_x);
@@ -163,6 +169,7 @@ struct sfv_handle {
         if (B) (void)hipFree(B);
         if (d_rain) (void)hipFree(d_rain);
         if (d_factor) (void)hipFree(d_factor);
+        if (d_rain_field) (void)hipFree(d_rain_field);
         if (ctl) (void)hipFree(ctl);
         if (ctl_host) (void)hipHostFree(ctl_host);
         if (stream) (void)hipStreamDestroy(stream);

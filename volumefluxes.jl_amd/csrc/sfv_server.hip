@@ -310,6 +310,7 @@ int sfv_server_submit(sfv_server* s, const sfv_config* cfg, const double* B, int
     if (!s || !cfg || !B || !U_interior || !id_out) return sfail(s, SFV_ERR_ARGUMENT, "NULL argument");
     if (cfg->precision != s->precision) return sfail(s, SFV_ERR_ARGUMENT, "request precision differs from the server's");
     if (cfg->stepper != SFV_RUNGE_KUTTA2) return sfail(s, SFV_ERR_ARGUMENT, "served requests use RungeKutta2");
+    if (cfg->rain == SFV_RAIN_FIELD) return sfail(s, SFV_ERR_ARGUMENT, "FunctionalRain cannot be served");
     if (cfg->nx < 1 || cfg->ny < 1 || cfg->gc < 1 || cfg->gc > sfv::MAX_GC) return sfail(s, SFV_ERR_DOMAIN, "bad grid size");
     const int64_t nxT = cfg->nx + 2 * (int64_t)cfg->gc, nyT = cfg->ny + 2 * (int64_t)cfg->gc;
     if (nbx != nxT + 1 || nby != nyT + 1)

@@ -21,7 +21,7 @@ struct KParams {
     int rec;        // 0 none, 1 linear
     int friction;   // 0 none, 1 bh2021, 2 bsa2012, 3 fcg2016
     int bottom;     // SourceTermBottom present
-    int rain_kind;  // 0 none, 1 constant, 2 table
+    int rain_kind;  // 0 none, 1 constant, 2 table, 3 per-cell field (FunctionalRain)
     int infil_kind; // 0 none, 1 constant, 2 horton
     int rain_first;
     int stepper;    // 1 FE, 2 RK2
@@ -31,6 +31,7 @@ struct KParams {
     double rain_const;         // ConstantRain rate / 3600
     double infil_rate, f0, fc, hk;
     const double* factor;      // device (nxT*nyT) Horton factor field
+    const double* rain_field;  // device (nxT*nyT) FunctionalRain rates of the current step
     // Ghost-cell source maps for the boundary condition, by side: low-side
     // ghost g is cell g, high-side ghost g is cell N-gc+g (N = nxT or nyT);
     // it takes the pre-BC value of cell src (same row/column), with the
@@ -160,7 +161,10 @@ __device__ __forceinline__ S apply_sources(const KParams& P, const StepCtl* ctl,
         if (P.infil_kind == 1) return (S)(x - ctl->infil_tfac);
         return x;
     };
-    auto rain = [&](S x) -> S { return P.rain_kind ? (S)(x + ctl->rain_now) : x; };
+    auto rain = [&](S x) -> S {
+        if (P.rain_kind == 3) return (S)(x + P.rain_field[k]);
+        return P.rain_kind ? (S)(x + ctl->rain_now) : x;
+    };
     return P.rain_first ? infil(rain(w)) : rain(infil(w));
 }
 

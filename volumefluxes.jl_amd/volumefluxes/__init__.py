@@ -20,7 +20,7 @@ from .grid import (XDIR, YDIR, CartesianGrid, NeumannBC, PeriodicBC, WallBC, cel
 from .model import (BottomTopography1D, BottomTopography2D, CentralUpwind, ConservedSystem, ConstantBottomTopography,
                     ConstantInfiltration, ConstantRain, ForwardEulerStepper, FunctionalRain,
                     HortonInfiltration, ImplicitFriction, LinearReconstruction, NoReconstruction, RungeKutta2,
-                    ShallowWaterEquations, ShallowWaterEquations1D, SourceTermBottom, TimeDependentRain, build_config, embed_1d,
+                    ShallowWaterEquations, ShallowWaterEquations1D, SourceTermBottom, TimeDependentRain, build_config, embed_1d, rain_field_values,
                     collect_topography_cells, compute_infiltration, compute_rain, friction_bh2021,
                     friction_bsa2012, friction_fcg2016, number_of_substeps, validate_bottom_topography)
 from .simulator import (HIPBackend, IntervalWriter, MultipleCallbacks, Simulator, advance_steps,
@@ -28,6 +28,7 @@ from .simulator import (HIPBackend, IntervalWriter, MultipleCallbacks, Simulator
                         get_available_backends, has_hip_backend, make_hip_backend, name, perform_step,
                         set_current_state, simulate_to_time)
 
+from . import model  # noqa: E402
 from .batch import Ensemble  # noqa: E402
 from .serving import Server  # noqa: E402
 

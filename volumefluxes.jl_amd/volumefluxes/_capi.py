@@ -29,7 +29,7 @@ SFV_BC_WALL, SFV_BC_PERIODIC, SFV_BC_NEUMANN = 0, 1, 2
 SFV_FORWARD_EULER, SFV_RUNGE_KUTTA2 = 1, 2
 SFV_NO_RECONSTRUCTION, SFV_LINEAR_RECONSTRUCTION = 0, 1
 SFV_FRICTION_NONE, SFV_FRICTION_BH2021, SFV_FRICTION_BSA2012, SFV_FRICTION_FCG2016 = 0, 1, 2, 3
-SFV_RAIN_NONE, SFV_RAIN_CONSTANT, SFV_RAIN_TIME_TABLE = 0, 1, 2
+SFV_RAIN_NONE, SFV_RAIN_CONSTANT, SFV_RAIN_TIME_TABLE, SFV_RAIN_FIELD = 0, 1, 2, 3
 SFV_INFILTRATION_NONE, SFV_INFILTRATION_CONSTANT, SFV_INFILTRATION_HORTON = 0, 1, 2
 SFV_F64, SFV_F32_STORAGE = 0, 1
 SFV_LAYOUT_FULL, SFV_LAYOUT_INTERIOR = 0, 1
@@ -162,6 +162,8 @@ def _declare(L) -> None:
     L.sfv_set_topography.argtypes = [h, POINTER(c_double), c_int64, c_int64]
     L.sfv_set_topography.restype = c_int32
     L.sfv_set_state.argtypes = [h, c_void_p, c_int32]
+    L.sfv_set_rain_field.argtypes = [h, POINTER(c_double)]
+    L.sfv_set_rain_field.restype = c_int32
     L.sfv_set_state.restype = c_int32
     L.sfv_get_state.argtypes = [h, c_void_p, c_int32]
     L.sfv_get_state.restype = c_int32

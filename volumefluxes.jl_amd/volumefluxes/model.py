@@ -183,12 +183,51 @@ class TimeDependentRain:
 
 
 class FunctionalRain:
-    """FunctionalRain(f, grid): an arbitrary host closure.  Not available on
-    the HIP backend (a device kernel cannot call a host function)."""
+    """FunctionalRain(rain_function, grid) (source_term_rain.jl:57-68): w gains
+    rain_function(t, x, y) on every cell, ghosts included, at the cell centre
+    (cell_center, grid.jl:117-121), with t the step's start time.  The HIP
+    backend evaluates the host function before each step and hands the
+    rates to the kernels as a per-cell field (rain_field_values)."""
 
     def __init__(self, rain_function, grid):
         self.rain_function = rain_function
         self.grid = grid
+
+
+def cell_center_coordinates(grid: CartesianGrid):
+    """x, y of cell_center(grid, I) for every cell, ghosts included
+    (grid.jl:117-121): start + Δ*(I - 0.5 - gc) with 1-based I, the
+    half-integer formed exactly before the product."""
+    x0, _, y0, _ = grid.extent
+    dx, dy = grid.delta
+    i1 = np.arange(grid.totalcells[0], dtype=np.float64) + 1.0
+    j1 = np.arange(grid.totalcells[1], dtype=np.float64) + 1.0
+    xs = x0 + dx * ((i1 - 0.5) - grid.gc)
+    ys = y0 + dy * ((j1 - 0.5) - grid.gc)
+    return xs, ys
+
+
+def rain_field_values(rain: FunctionalRain, grid: CartesianGrid, t: float) -> np.ndarray:
+    """compute_rain(rain, t, I) (source_term_rain.jl:64-68) for every cell, as
+    a (nxT, nyT) float64 array.  The function is called once with array
+    arguments when it accepts them, else once per cell."""
+    xs, ys = cell_center_coordinates(grid)
+    X = np.broadcast_to(xs[:, None], grid.totalcells)
+    Y = np.broadcast_to(ys[None, :], grid.totalcells)
+    try:
+        v = np.asarray(rain.rain_function(t, X, Y), dtype=np.float64)
+        if v.shape == ():
+            return np.full(grid.totalcells, float(v), order="F")
+        if v.shape == tuple(grid.totalcells):
+            return np.asfortranarray(v)
+    except (TypeError, ValueError):
+        pass
+    out = np.empty(grid.totalcells, order="F")
+    f = rain.rain_function
+    for j in range(grid.totalcells[1]):
+        for i in range(grid.totalcells[0]):
+            out[i, j] = float(f(t, float(xs[i]), float(ys[j])))
+    return out
 
 
 class HortonInfiltration:
@@ -371,8 +410,8 @@ def build_config(system: ConservedSystem, stepper, grid: CartesianGrid, cfl: flo
     if rains:
         r = rains[0]
         if isinstance(r, FunctionalRain):
-            raise _argument_error("FunctionalRain calls a host closure and cannot run on the HIP backend")
-        if isinstance(r, ConstantRain):
+            cfg.rain = C.SFV_RAIN_FIELD
+        elif isinstance(r, ConstantRain):
             cfg.rain, cfg.rain_rate = C.SFV_RAIN_CONSTANT, r.rain_rate
         else:
             cfg.rain = C.SFV_RAIN_TIME_TABLE

@@ -15,7 +15,8 @@ import numpy as np
 
 from . import _capi as C
 from .grid import CartesianGrid
-from .model import EMBED_ROWS, ConservedSystem, RungeKutta2, build_config, embed_1d
+from .model import (EMBED_ROWS, ConservedSystem, FunctionalRain, RungeKutta2, build_config, embed_1d,
+                    rain_field_values)
 
 InteriorState = namedtuple("InteriorState", ["h", "hu", "hv"])
 InteriorState1D = namedtuple("InteriorState1D", ["h", "hu"])
@@ -88,6 +89,14 @@ class Simulator:
         self._h = h
         B = np.asfortranarray(self._system2.equation.B.corners(self._grid2), dtype=np.float64)
         C.check(self._h, L.sfv_set_topography(self._h, C.dptr(B), B.shape[0], B.shape[1]))
+        fr = [s for s in self._system2.source_terms if isinstance(s, FunctionalRain)]
+        self._functional_rain = fr[0] if fr else None
+
+    def _refresh_rain(self) -> None:
+        """FunctionalRain: the rates at this step's start time on every cell."""
+        if self._functional_rain is not None:
+            F = rain_field_values(self._functional_rain, self._grid2, self.t)
+            C.check(self._h, C.lib().sfv_set_rain_field(self._h, C.dptr(F)))
 
     def __del__(self):
         h = getattr(self, "_h", None)
@@ -199,6 +208,7 @@ def perform_step(sim: Simulator, max_dt: float) -> float:
     """perform_step!(sim, max_dt) (simulator.jl:145-177): one step with the
     sources at sim.t.  Like the reference it does not advance sim.t."""
     dt = ctypes.c_double()
+    sim._refresh_rain()
     sim._step_status(C.lib().sfv_step(sim.handle, float(max_dt), sim.t, ctypes.byref(dt)))
     sim.current_timestep = dt.value
     return dt.value
@@ -219,7 +229,7 @@ def simulate_to_time(sim: Simulator, endtime: float, *, match_endtime: bool = Tr
     if callback is not None and not callable(callback):
         callback = MultipleCallbacks(callback)
     L = C.lib()
-    if callback is None:
+    if callback is None and sim._functional_rain is None:
         t = ctypes.c_double(sim.t)
         steps = ctypes.c_int64()
         mts = float(maximum_timestep) if maximum_timestep is not None else 0.0
@@ -238,11 +248,18 @@ def simulate_to_time(sim: Simulator, endtime: float, *, match_endtime: bool = Tr
             max_dt = min(max_dt, maximum_timestep)
         perform_step(sim, max_dt)
         sim.t = _realtype_add(sim, sim.t, sim.current_timestep)
-        callback(sim.t, sim)
+        if callback is not None:
+            callback(sim.t, sim)
 
 
 def advance_steps(sim: Simulator, nsteps: int, max_dt: float = float("inf")) -> None:
-    """nsteps of perform_step!(sim, max_dt) with t += dt, device-resident."""
+    """nsteps of perform_step!(sim, max_dt) with t += dt, device-resident
+    (host-driven per step when a FunctionalRain needs its rates each step)."""
+    if sim._functional_rain is not None:
+        for _ in range(int(nsteps)):
+            perform_step(sim, max_dt)
+            sim.t = _realtype_add(sim, sim.t, sim.current_timestep)
+        return
     t = ctypes.c_double(sim.t)
     sim._step_status(C.lib().sfv_advance(sim.handle, int(nsteps), float(max_dt), ctypes.byref(t)))
     sim.t = t.value
