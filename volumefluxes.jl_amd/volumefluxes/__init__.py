@@ -28,7 +28,8 @@ from .simulator import (HIPBackend, IntervalWriter, MultipleCallbacks, Simulator
                         get_available_backends, has_hip_backend, make_hip_backend, name, perform_step,
                         set_current_state, simulate_to_time)
 
-from . import model  # noqa: E402
+from . import io, model  # noqa: E402
+from .io import SnapshotWriter, TotalWaterVolume, bottom_per_cell, coarsen, load_grid  # noqa: E402
 from .batch import Ensemble  # noqa: E402
 from .serving import Server  # noqa: E402
 
