@@ -188,6 +188,24 @@ int sfv_device_count(void);
  * All give bitwise identical results.  No counterpart in the reference. */
 int sfv_set_fused(sfv_handle* h, int32_t fused);
 
+/* ------------------------------------------------------------------ adjoint
+ * Reverse-mode gradient of J = sum over interior cells of (h_T - h_target)^2,
+ * h = w - B_cell, where h_T is the state after nsteps calls of
+ * perform_step!(sim, Inf) (simulator.jl:145-177) from the handle's current
+ * state and *t_io.  Gradients (each may be NULL): dJ_dB over the corners
+ * (nxT+1, nyT+1), dJ_dCz (ImplicitFriction's Cz; Cz = n^2 gives
+ * dJ/dn = 2n dJ/dCz), dJ_dU0 over the interior initial state (nx, ny, 3).
+ * The derivative is the one forward-mode differentiation of the reference
+ * (ForwardDiff through simulate_to_time) gives: every branch the run took is
+ * differentiated as taken, and dt is differentiated through its maximum
+ * wave speed.  Checkpoints every checkpoint_every steps bound the memory.
+ * Float64, RungeKutta2, LinearReconstruction, WallBC or PeriodicBC with
+ * nx, ny >= gc, no FunctionalRain.  On return the handle holds the final
+ * state, *t_io its time and *J_out the objective.  No counterpart in the
+ * reference, whose optimisation example uses ForwardDiff. */
+int sfv_adjoint(sfv_handle* h, int64_t nsteps, int32_t checkpoint_every, double* t_io, const double* h_target,
+                double* J_out, double* dJ_dB, double* dJ_dCz, double* dJ_dU0);
+
 /* ---------------------------------------------------------------- ensembles
  * No counterpart in the reference (a reference user runs one Simulator at a
  * time).  A batch groups existing handles — grids of any size, each with its

@@ -51,6 +51,11 @@ def lib():
         L.oracle2d_destroy.argtypes = [c_void_p]
         L.oracle2d_set_topography.argtypes = [c_void_p, POINTER(d)]
         L.oracle2d_set_rain_field.argtypes = [c_void_p, POINTER(d)]
+        L.oracle_ad_run.argtypes = [c_void_p, POINTER(d), POINTER(d), ctypes.c_int64, POINTER(d), POINTER(d), d,
+                                    POINTER(d), POINTER(d), POINTER(d), POINTER(d)]
+        L.oracle_ad_run.restype = d
+        L.oracle_ad_forward.argtypes = [c_void_p, POINTER(d), POINTER(d), ctypes.c_int64, POINTER(d), POINTER(d)]
+        L.oracle_ad_forward.restype = None
         L.oracle2d_set_interior.argtypes = [c_void_p, POINTER(d)]
         L.oracle2d_set_full.argtypes = [c_void_p, c_void_p]
         L.oracle2d_get_full.argtypes = [c_void_p, c_void_p]
@@ -87,7 +92,7 @@ def set_threads(n: int) -> None:
 
 
 def _dp(a):
-    return a.ctypes.data_as(POINTER(c_double))
+    return None if a is None else a.ctypes.data_as(POINTER(c_double))
 
 
 class Oracle2D:
@@ -216,3 +221,31 @@ class Oracle1D:
                                             int(max_steps), ctypes.byref(t))
         self.t = t.value
         return n
+
+
+# ---- forward-mode derivatives (oracle_ad.hpp)
+def _f(a):
+    return None if a is None else np.asfortranarray(a, dtype=np.float64)
+
+
+def ad_run(cfg, B, U0, nsteps, target, dB=None, dCz=0.0, dU0=None):
+    """J = sum (h_T - target)^2 after nsteps steps (max_dt = Inf) and its
+    derivative along (dB, dCz, dU0).  Returns (J, dJ, U_T full, dts)."""
+    B, U0, target, dB, dU0 = _f(B), _f(U0), _f(target), _f(dB), _f(dU0)
+    nxT, nyT = cfg.nx + 2 * cfg.gc, cfg.ny + 2 * cfg.gc
+    U = np.empty((nxT, nyT, 3), order="F")
+    dts = np.empty(int(nsteps))
+    dJ = c_double()
+    J = lib().oracle_ad_run(ctypes.byref(cfg), _dp(B), _dp(U0), int(nsteps), _dp(target), _dp(dB), float(dCz),
+                            _dp(dU0), ctypes.byref(dJ), _dp(U), _dp(dts))
+    return J, dJ.value, U, dts
+
+
+def ad_forward(cfg, B, U0, nsteps):
+    """The same run on plain doubles: (U_T full, dts)."""
+    B, U0 = _f(B), _f(U0)
+    nxT, nyT = cfg.nx + 2 * cfg.gc, cfg.ny + 2 * cfg.gc
+    U = np.empty((nxT, nyT, 3), order="F")
+    dts = np.empty(int(nsteps))
+    lib().oracle_ad_forward(ctypes.byref(cfg), _dp(B), _dp(U0), int(nsteps), _dp(U), _dp(dts))
+    return U, dts

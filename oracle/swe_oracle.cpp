@@ -2,6 +2,7 @@
 // Loaded with ctypes by tests/, __graft_entry__.smoke() and bench.py's
 // cpu_baseline leg only.  See swe_oracle.hpp for what each function restates.
 #include "swe_oracle.hpp"
+#include "oracle_ad.hpp"
 
 #ifdef _OPENMP
 #include <omp.h>
@@ -211,5 +212,66 @@ int64_t oracle1d_simulate_to_time(void* hp, double endtime, int match_endtime, d
     return steps;
 }
 double oracle1d_current_timestep(void* hp) { return ((Sim1D*)hp)->current_timestep; }
+
+// ---- forward-mode derivative of a fixed-step run (oracle_ad.hpp)
+// J = sum over interior cells of (w - B_cell - target)^2 after nsteps steps
+// of perform_step!(sim, Inf) from the interior state U0 (nx, ny, 3); with
+// directions dB (corners), dCz and dU0 (interior, may be NULL) returns J and
+// writes dJ along that direction.  U_out (nxT, nyT, 3) may be NULL.
+double oracle_ad_run(const sfv_config* cfg, const double* B, const double* U0, int64_t nsteps, const double* target,
+                     const double* dB, double dCz, const double* dU0, double* dJ_out, double* U_out, double* dts) {
+    using D1 = oracle_ad::Dual<1>;
+    oracle_ad::Step2D<D1> s(*cfg);
+    const size_t nB = (size_t)(s.nxT + 1) * (s.nyT + 1);
+    for (size_t k = 0; k < nB; ++k) { s.B[k] = D1(B[k]); s.B[k].d[0] = dB ? dB[k] : 0.0; }
+    s.Cz.d[0] = dCz;
+    const int nx = cfg->nx, ny = cfg->ny, gc = cfg->gc;
+    for (int v = 0; v < 3; ++v)
+        for (int j = 0; j < ny; ++j)
+            for (int i = 0; i < nx; ++i) {
+                const size_t src = (size_t)v * nx * ny + (size_t)j * nx + i;
+                D1 x(U0[src]);
+                x.d[0] = dU0 ? dU0[src] : 0.0;
+                s.U[0][v * s.n + s.id(i + gc, j + gc)] = x;
+            }
+    s.bc(s.U[0]);
+    double t = 0.0;
+    for (int64_t k = 0; k < nsteps; ++k) {
+        const D1 dt = s.step(INFINITY, t);
+        if (dts) dts[k] = dt.v;
+        t += dt.v;
+    }
+    D1 J(0.0);
+    for (int j = gc; j < s.nyT - gc; ++j)
+        for (int i = gc; i < s.nxT - gc; ++i) {
+            const D1 e = (s.U[0][s.id(i, j)] - s.B_cell(i, j)) - target[(size_t)(i - gc) + (size_t)nx * (j - gc)];
+            J = J + e * e;
+        }
+    if (U_out)
+        for (size_t k = 0; k < 3 * s.n; ++k) U_out[k] = s.U[0][k].v;
+    if (dJ_out) *dJ_out = J.d[0];
+    return J.v;
+}
+
+// The same run on plain doubles (pins oracle_ad.hpp against Sim2D).
+void oracle_ad_forward(const sfv_config* cfg, const double* B, const double* U0, int64_t nsteps, double* U_out,
+                       double* dts) {
+    oracle_ad::Step2D<double> s(*cfg);
+    const size_t nB = (size_t)(s.nxT + 1) * (s.nyT + 1);
+    for (size_t k = 0; k < nB; ++k) s.B[k] = B[k];
+    const int nx = cfg->nx, ny = cfg->ny, gc = cfg->gc;
+    for (int v = 0; v < 3; ++v)
+        for (int j = 0; j < ny; ++j)
+            for (int i = 0; i < nx; ++i)
+                s.U[0][v * s.n + s.id(i + gc, j + gc)] = U0[(size_t)v * nx * ny + (size_t)j * nx + i];
+    s.bc(s.U[0]);
+    double t = 0.0;
+    for (int64_t k = 0; k < nsteps; ++k) {
+        const double dt = s.step(INFINITY, t);
+        if (dts) dts[k] = dt;
+        t += dt;
+    }
+    for (size_t k = 0; k < 3 * s.n; ++k) U_out[k] = s.U[0][k];
+}
 
 }  // extern "C"

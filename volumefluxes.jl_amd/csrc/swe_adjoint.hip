@@ -1,0 +1,760 @@
+// swe_adjoint.hip — reverse-mode gradient of a time-stepped shallow-water run.
+//
+// J = sum over interior cells of (h_T - h_target)^2, h = w - B_cell, after
+// nsteps RK2 steps (perform_step!(sim, Inf) each, simulator.jl:145-177).
+// Gradients: dJ/dB (every bottom corner), dJ/dCz (ImplicitFriction, Cz =
+// Manning n^2), dJ/dU0 (interior initial state).  The derivative is the one
+// forward-mode differentiation of the reference gives: every branch the
+// forward step took (wet/dry cut-offs, minmod, desingularisation, the dry
+// clamp, the argmax face of the CFL time step) is taken again and
+// differentiated; dt is differentiated through its maximum wave speed.
+//
+// Memory: the forward pass keeps a checkpoint every `checkpoint_every`
+// steps.  The reverse pass recomputes one segment from its checkpoint (the
+// per-pass kernels, bitwise the forward run), then walks it backwards; one
+// reverse step recomputes that step's stage-1 state U1, the right-hand sides
+// R1 = L(U0) + sources, R2 = L(U1) + sources and dt, and applies, per stage
+// in reverse order: boundary-condition fold, dry clamp, implicit friction,
+// the RK combination, then per direction the face fluxes (dual-number
+// Jacobian of the central-upwind flux at each face) and the reconstruction
+// (dual-number Jacobian of each cell's face states).  All kernels are
+// gathers: every output element is written by one thread.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <type_traits>
+#include <vector>
+
+#include "../../include/sinfvm_hip.h"
+#include "sfv_handle.hpp"
+#include "swe_dual.hpp"
+
+namespace sfv {
+namespace ad {
+
+// ---- the step's arithmetic on a generic number type T (double or Dual<N>),
+// value-for-value the same operations as swe_physics.hpp.
+
+template <class T>
+__device__ T desingularize(const T& h, double kappa) {
+    const double ah = __builtin_fabs(val(h));
+    if (!(ah < kappa)) return h;
+    const T h2 = h * h;
+    const double sgn = __builtin_signbit(val(h)) ? -1.0 : 1.0;
+    const T inner = h2 / (2 * kappa) + kappa / 2.0;
+    const T mn = djl_min(inner, T(kappa));
+    const T mx = djl_max(dabs(h), mn);
+    return sgn * mx;
+}
+
+template <class T>
+__device__ void phys(int d, const T& h, const T& hu, const T& hv, const PhysParams& p, T f[3], T e[2]) {
+    const T hs = ad::desingularize(h, p.kappa);
+    const T u = hu / hs;
+    const T v = hv / hs;
+    const T h2 = h * h;
+    const T c = dsqrt(p.g * h);
+    if (d == 0) {
+        f[0] = p.rho * h * u;
+        f[1] = p.rho * h * u * u + 0.5 * p.rho * p.g * h2;
+        f[2] = p.rho * h * u * v;
+        e[0] = u + c;
+        e[1] = u - c;
+    } else {
+        f[0] = p.rho * h * v;
+        f[1] = p.rho * h * u * v;
+        f[2] = p.rho * h * v * v + 0.5 * p.rho * p.g * h2;
+        e[0] = v + c;
+        e[1] = v - c;
+    }
+}
+
+// centralupwind.jl:48-77; returns the face speed, F the flux.
+template <class T>
+__device__ T central_upwind(int d, const T m[3], const T q[3], const PhysParams& p, T F[3]) {
+    T fm[3] = {T(0.0), T(0.0), T(0.0)}, em[2] = {T(0.0), T(0.0)};
+    T fp[3] = {T(0.0), T(0.0), T(0.0)}, ep[2] = {T(0.0), T(0.0)};
+    if (val(m[0]) > p.cutoff) ad::phys(d, m[0], m[1], m[2], p, fm, em);
+    if (val(q[0]) > p.cutoff) ad::phys(d, q[0], q[1], q[2], p, fp, ep);
+    const T aplus = djl_max(djl_max(ep[0], em[0]), T(0.0));
+    const T aminus = djl_min(djl_min(ep[1], em[1]), T(0.0));
+    const T da = aplus - aminus;
+    if (__builtin_fabs(val(da)) < p.kappa) {
+        F[0] = F[1] = F[2] = T(0.0);
+        return T(0.0);
+    }
+    const T pm = (aplus * aminus) / da;
+    for (int v = 0; v < 3; ++v) {
+        const T diff = q[v] - m[v];
+        F[v] = (aplus * fm[v] - aminus * fp[v]) / da + pm * diff;
+    }
+    if (val(m[0]) < p.cutoff && val(q[0]) < p.cutoff) return T(0.0);
+    return djl_max(dabs(aplus), dabs(aminus));
+}
+
+// linearreconstruction.jl:28-35: the argument minmod selects (0 if none), ties
+// resolved as Julia's min/max resolve them.
+template <class T>
+__device__ T minmod(const T& a, const T& b, const T& c) {
+    const double x = val(a), y = val(b), z = val(c);
+    if (x != x || y != y || z != z) return T(0.0);
+    if (x > 0 && y > 0 && z > 0) return djl_min(djl_min(a, b), c);
+    if (x < 0 && y < 0 && z < 0) return djl_max(djl_max(a, b), c);
+    return T(0.0);
+}
+
+// minmod_slope + dry fix + face states (linearreconstruction.jl:37-43, 73-100).
+template <class T>
+__device__ void reconstruct(const T l[3], const T c[3], const T r[3], double theta, const T& Bl, const T& Br, T L[3],
+                            T R[3]) {
+    T slope[3];
+    for (int v = 0; v < 3; ++v) {
+        const T fwd = r[v] - c[v];
+        const T bwd = c[v] - l[v];
+        const T central = (fwd + bwd) / 2.0;
+        slope[v] = ad::minmod(theta * fwd, central, theta * bwd);
+    }
+    const T w = c[0];
+    if (val(w - 0.5 * slope[0]) < val(Bl)) slope[0] = 2.0 * (w - Bl);
+    else if (val(w + 0.5 * slope[0]) < val(Br)) slope[0] = 2.0 * (Br - w);
+    for (int v = 0; v < 3; ++v) {
+        L[v] = c[v] - 0.5 * slope[v];
+        R[v] = c[v] + 0.5 * slope[v];
+    }
+    L[0] = L[0] - Bl;
+    R[0] = R[0] - Br;
+}
+
+// friction.jl:30-43, 62-78: the implicit friction scalar from the stage input.
+template <class T>
+__device__ T friction_scalar(int law, const T& Cz, double kappa, const T& w, const T& hu, const T& hv, const T& Bm) {
+    const T hs = ad::desingularize(w - Bm, kappa);
+    const T u = hu / hs, v = hv / hs;
+    T speed = dsqrt(u * u + v * v);
+    if (val(speed) == 0.0) speed = T(0.0);
+    if (law == 2) { const T denom = dcbrt(hs) * hs; return -Cz * speed / denom; }
+    if (law == 3) { const T denom = dcbrt(hs) * hs * hs; return -Cz * speed / denom; }
+    const T denom = hs * hs;
+    return -Cz * speed / denom;
+}
+
+__device__ inline double Bc4(const KParams& P, const double* B, int i, int j) {
+    return 0.25 * (((Bcorner(P, B, i, j) + Bcorner(P, B, i + 1, j)) + Bcorner(P, B, i, j + 1)) + Bcorner(P, B, i + 1, j + 1));
+}
+
+// cell k's face states in direction d, with duals seeded on cell `seed`'s
+// three variables (seed = -1: none) or on the two face bottoms (seedB).
+template <class T>
+__device__ void recon_at(const KParams& P, const double* U, const double* B, int d, int i, int j, int seed_off,
+                         bool seedB, T L[3], T R[3]) {
+    const size_t n = (size_t)P.nxT * P.nyT;
+    const int di = d == 0 ? 1 : 0, dj = d == 0 ? 0 : 1;
+    T cl[3], cc[3], cr[3];
+    for (int v = 0; v < 3; ++v) {
+        cl[v] = T(U[v * n + cid(P, i - di, j - dj)]);
+        cc[v] = T(U[v * n + cid(P, i, j)]);
+        cr[v] = T(U[v * n + cid(P, i + di, j + dj)]);
+    }
+    if constexpr (std::is_same<T, Dual<3>>::value) {
+        if (seed_off >= -1 && seed_off <= 1) {
+            T* tgt = seed_off == -1 ? cl : (seed_off == 0 ? cc : cr);
+            for (int v = 0; v < 3; ++v) tgt[v].d[v] = 1.0;
+        }
+    }
+    T Bl(B_face_left(P, B, i, j, d)), Br(B_face_right(P, B, i, j, d));
+    if constexpr (std::is_same<T, Dual<2>>::value) {
+        if (seedB) { Bl.d[0] = 1.0; Br.d[1] = 1.0; }
+    }
+    ad::reconstruct(cl, cc, cr, P.theta, Bl, Br, L, R);
+}
+
+// ---------------------------------------------------------------- kernels
+
+// Boundary-condition fold (adjoint of update_bc!): ghost adjoints go to the
+// cells the ghosts were copied from (sign flipped with the momentum), then
+// ghosts are zeroed.  axis 0: x ghosts, one thread per interior row; axis 1:
+// y ghosts, one thread per interior column.  The gc x gc corners are never
+// read by the step and are zeroed too.
+__global__ void k_adj_bc(KParams P, double* __restrict__ L, int axis) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t n = (size_t)P.nxT * P.nyT;
+    const int gc = P.gc;
+    if (axis == 0) {
+        if (t >= P.ny) return;
+        const int j = gc + t;
+        for (int g = 0; g < gc; ++g) {
+            const size_t klo = cid(P, g, j), khi = cid(P, P.nxT - gc + g, j);
+            const size_t slo = cid(P, P.src_lo_x[g], j), shi = cid(P, P.src_hi_x[g], j);
+            for (int v = 0; v < 3; ++v) {
+                const double flo = (P.flip_lo_x[g] && v == 1) ? -1.0 : 1.0;
+                const double fhi = (P.flip_hi_x[g] && v == 1) ? -1.0 : 1.0;
+                L[v * n + slo] += flo * L[v * n + klo];
+                L[v * n + klo] = 0.0;
+                L[v * n + shi] += fhi * L[v * n + khi];
+                L[v * n + khi] = 0.0;
+            }
+        }
+    } else {
+        if (t >= P.nx) {
+            // corners
+            const int c = t - P.nx;
+            if (c < 4 * gc * gc) {
+                const int corner = c / (gc * gc), rr = c % (gc * gc);
+                const int ci = (corner & 1) ? P.nxT - gc + rr % gc : rr % gc;
+                const int cj = (corner >> 1) ? P.nyT - gc + rr / gc : rr / gc;
+                for (int v = 0; v < 3; ++v) L[v * n + cid(P, ci, cj)] = 0.0;
+            }
+            return;
+        }
+        const int i = gc + t;
+        for (int g = 0; g < gc; ++g) {
+            const size_t klo = cid(P, i, g), khi = cid(P, i, P.nyT - gc + g);
+            const size_t slo = cid(P, i, P.src_lo_y[g]), shi = cid(P, i, P.src_hi_y[g]);
+            for (int v = 0; v < 3; ++v) {
+                const double flo = (P.flip_lo_y[g] && v == 2) ? -1.0 : 1.0;
+                const double fhi = (P.flip_hi_y[g] && v == 2) ? -1.0 : 1.0;
+                L[v * n + slo] += flo * L[v * n + klo];
+                L[v * n + klo] = 0.0;
+                L[v * n + shi] += fhi * L[v * n + khi];
+                L[v * n + khi] = 0.0;
+            }
+        }
+    }
+}
+
+// Block sum of one value into part[blockIdx]; deterministic (fixed tree).
+__device__ void block_sum(double v, double* part, int slot) {
+    __shared__ double red[256];
+    const int tid = threadIdx.x + blockDim.x * threadIdx.y;
+    red[tid] = v;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (tid < s) red[tid] += red[tid + s];
+        __syncthreads();
+    }
+    if (tid == 0) part[slot] = red[0];
+    __syncthreads();
+}
+
+// Adjoint of: pre = in + dt*R (stage 2: pre = 0.5*(U0 + in + dt*R)),
+// friction (scalar from `in`), dry clamp.  Lout: adjoint of the stage
+// output (ghosts already folded).  Adds to Lin (and, stage 2, to LU0),
+// writes LR = adjoint of R, adds the B_cell adjoint, and block partial sums
+// of the dt and Cz adjoints.
+__global__ void __launch_bounds__(256) k_adj_tail(KParams P, const StepCtl* ctl, int stage, const double* __restrict__ U0,
+                                                  const double* __restrict__ Uin, const double* __restrict__ R,
+                                                  const double* __restrict__ B, const double* __restrict__ Lout,
+                                                  double* __restrict__ Lin, double* __restrict__ LU0,
+                                                  double* __restrict__ LR, double* __restrict__ LBc,
+                                                  double* __restrict__ part_dt, double* __restrict__ part_cz) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int nblk = gridDim.x * gridDim.y, bid = blockIdx.x + gridDim.x * blockIdx.y;
+    double gdt = 0.0, gcz = 0.0;
+    if (i < P.nxT && j < P.nyT) {
+        const size_t n = (size_t)P.nxT * P.nyT;
+        const size_t k = cid(P, i, j);
+        const double dt = ctl->dt_stage1;
+        double x[3], r[3], ax[3], pre[3], lo[3];
+        for (int v = 0; v < 3; ++v) {
+            x[v] = Uin[v * n + k];
+            r[v] = R[v * n + k];
+            ax[v] = x[v] + dt * r[v];
+            pre[v] = stage == 2 ? 0.5 * (U0[v * n + k] + ax[v]) : ax[v];
+            lo[v] = Lout[v * n + k];
+        }
+        const double Bm = Bc4(P, B, i, j);
+        double fs = 0.0, D = 1.0;
+        if (P.friction) {
+            fs = ad::friction_scalar<double>(P.friction, P.Cz, P.phys.kappa, x[0], x[1], x[2], Bm);
+            D = 1 - dt * fs;
+        }
+        double o[3] = {pre[0] * (1 - dt * 0.0), pre[1] / D, pre[2] / D};
+        double lBc = 0.0;
+        // dry clamp (timestepper.jl:53-63)
+        if (o[0] - Bm < P.phys.cutoff) {
+            // jl_max(o0, Bm): o0 if larger, Bm if larger, on a tie Bm when o0 is -0.0
+            const bool takes_w = o[0] > Bm || (!(Bm > o[0]) && !__builtin_signbit(o[0]));
+            if (!takes_w) lBc += lo[0];
+            lo[0] = takes_w ? lo[0] : 0.0;
+            lo[1] = 0.0;
+            lo[2] = 0.0;
+        }
+        // friction: o1 = pre1 / D, o2 = pre2 / D, D = 1 - dt*fs(x, Bm, Cz)
+        double lpre[3] = {lo[0], lo[1] / D, lo[2] / D};
+        if (P.friction) {
+            const double lD = -(lo[1] * pre[1] + lo[2] * pre[2]) / (D * D);
+            const double lfs = lD * (-dt);
+            gdt += lD * (-fs);
+            using D5 = Dual<5>;
+            const D5 fsd = ad::friction_scalar<D5>(P.friction, D5::var(P.Cz, 4), P.phys.kappa, D5::var(x[0], 0),
+                                               D5::var(x[1], 1), D5::var(x[2], 2), D5::var(Bm, 3));
+            for (int v = 0; v < 3; ++v) Lin[v * n + k] += lfs * fsd.d[v];
+            lBc += lfs * fsd.d[3];
+            gcz += lfs * fsd.d[4];
+        }
+        // RK combination
+        double lax[3];
+        for (int v = 0; v < 3; ++v) {
+            lax[v] = stage == 2 ? 0.5 * lpre[v] : lpre[v];
+            if (stage == 2) LU0[v * n + k] += 0.5 * lpre[v];
+            Lin[v * n + k] += lax[v];
+            LR[v * n + k] = dt * lax[v];
+            gdt += lax[v] * r[v];
+        }
+        LBc[k] += lBc;
+    }
+    block_sum(gdt, part_dt, (stage == 2 ? 0 : nblk) + bid);
+    block_sum(gcz, part_cz, (stage == 2 ? 0 : nblk) + bid);
+}
+
+// Face fluxes of direction d (numericalflux.jl:21-33): face between cell c
+// and c+1 along d, c from gc-1 to nT-gc-1, other index interior.  The flux
+// adjoint is (1/Δ)(LR(c+1) - LR(c)) over interior cells; its transpose
+// through the central-upwind Jacobian gives the adjoints of c's right face
+// state and (c+1)'s left face state.
+__global__ void __launch_bounds__(256) k_adj_faces(KParams P, int d, const double* __restrict__ U,
+                                                   const double* __restrict__ B, const double* __restrict__ LR,
+                                                   double* __restrict__ LfL, double* __restrict__ LfR) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nT = d == 0 ? P.nxT : P.nyT, nO = d == 0 ? P.ny : P.nx;
+    const int nf = nT - 2 * P.gc + 1;
+    if (t >= nf * nO) return;
+    const int s = P.gc - 1 + t % nf, o = P.gc + t / nf;
+    const int i = d == 0 ? s : o, j = d == 0 ? o : s;
+    const int i1 = d == 0 ? i + 1 : i, j1 = d == 0 ? j : j + 1;
+    const size_t n = (size_t)P.nxT * P.nyT;
+    const size_t kc = cid(P, i, j), k1 = cid(P, i1, j1);
+    const double inv = d == 0 ? P.inv_dx : P.inv_dy;
+    const bool c_in = s >= P.gc, c1_in = s + 1 < nT - P.gc;
+    double lF[3];
+    for (int v = 0; v < 3; ++v) lF[v] = inv * ((c1_in ? LR[v * n + k1] : 0.0) - (c_in ? LR[v * n + kc] : 0.0));
+    double Lc[3], Rc[3], L1[3], R1[3];
+    recon_at<double>(P, U, B, d, i, j, -2, false, Lc, Rc);
+    recon_at<double>(P, U, B, d, i1, j1, -2, false, L1, R1);
+    using D6 = Dual<6>;
+    D6 m[3], q[3], F[3];
+    for (int v = 0; v < 3; ++v) { m[v] = D6::var(Rc[v], v); q[v] = D6::var(L1[v], 3 + v); }
+    (void)ad::central_upwind<D6>(d, m, q, P.phys, F);
+    for (int a = 0; a < 3; ++a) {
+        double gm = 0.0, gq = 0.0;
+        for (int v = 0; v < 3; ++v) { gm += lF[v] * F[v].d[a]; gq += lF[v] * F[v].d[3 + a]; }
+        LfR[a * n + kc] = gm;
+        LfL[a * n + k1] = gq;
+    }
+}
+
+// Per cell of direction d: adjoint of the reconstruction (and of the bottom
+// source, source_term_bottom.jl:24-41) gathered from the face-state adjoints
+// of the cell and its two neighbours along d.  Adds to LU; adds the face
+// bottom adjoints of the cell to LBf (planes: x left, x right, y left, y right).
+__global__ void __launch_bounds__(256) k_adj_recon(KParams P, int d, const double* __restrict__ U,
+                                                   const double* __restrict__ B, const double* __restrict__ LR,
+                                                   const double* __restrict__ LfL, const double* __restrict__ LfR,
+                                                   double* __restrict__ LU, double* __restrict__ LBf) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    if (i >= P.nxT || j >= P.nyT) return;
+    const int s = d == 0 ? i : j, o = d == 0 ? j : i;
+    const int nT = d == 0 ? P.nxT : P.nyT, oT = d == 0 ? P.nyT : P.nxT;
+    if (o < P.gc || o >= oT - P.gc) return;
+    const size_t n = (size_t)P.nxT * P.nyT;
+    const double g = P.phys.g;
+    const double Dd = d == 0 ? P.dx : P.dy, inv = d == 0 ? P.inv_dx : P.inv_dy;
+    const int pow2 = d == 0 ? P.dx_pow2 : P.dy_pow2;
+    // total adjoint of cell k's face states: flux part + bottom-source part
+    auto face_adj = [&](int kk, int ki, int kj, double aL[3], double aR[3]) {
+        const size_t q = cid(P, ki, kj);
+        for (int v = 0; v < 3; ++v) { aL[v] = LfL[v * n + q]; aR[v] = LfR[v * n + q]; }
+        if (P.bottom && kk >= P.gc && kk < nT - P.gc) {
+            const double slope = slope_div(B_face_right(P, B, ki, kj, d) - B_face_left(P, B, ki, kj, d), Dd, inv, pow2);
+            const double c = LR[(1 + d) * n + q] * (-g * slope) * 0.5;
+            aL[0] += c;
+            aR[0] += c;
+        }
+    };
+    using D3 = Dual<3>;
+    double acc[3] = {0.0, 0.0, 0.0};
+    for (int off = -1; off <= 1; ++off) {
+        const int kk = s + off;                       // the reconstructed cell along d
+        if (kk < P.gc - 1 || kk > nT - P.gc) continue; // its face states are used only in this range
+        const int ki = d == 0 ? kk : i, kj = d == 0 ? j : kk;
+        double aL[3], aR[3];
+        face_adj(kk, ki, kj, aL, aR);
+        D3 L[3], R[3];
+        recon_at<D3>(P, U, B, d, ki, kj, -off, false, L, R);   // this cell sits at offset -off in kk's stencil
+        for (int v = 0; v < 3; ++v)
+            for (int m = 0; m < 3; ++m) acc[v] += aL[m] * L[m].d[v] + aR[m] * R[m].d[v];
+    }
+    const size_t k = cid(P, i, j);
+    for (int v = 0; v < 3; ++v) LU[v * n + k] += acc[v];
+    // the cell's own face bottoms
+    if (s >= P.gc - 1 && s <= nT - P.gc) {
+        double aL[3], aR[3];
+        face_adj(s, i, j, aL, aR);
+        using D2 = Dual<2>;
+        D2 L[3], R[3];
+        recon_at<D2>(P, U, B, d, i, j, -2, true, L, R);
+        double lBl = 0.0, lBr = 0.0;
+        for (int m = 0; m < 3; ++m) {
+            lBl += aL[m] * L[m].d[0] + aR[m] * R[m].d[0];
+            lBr += aL[m] * L[m].d[1] + aR[m] * R[m].d[1];
+        }
+        if (P.bottom && s >= P.gc && s < nT - P.gc) {   // bottom source: -g ((Br - Bl)/Δ) ((hR + hL)/2)
+            double Lc[3], Rc[3];
+            recon_at<double>(P, U, B, d, i, j, -2, false, Lc, Rc);
+            const double c = LR[(1 + d) * n + k] * (-g) * inv * ((Rc[0] + Lc[0]) / 2.0);
+            lBr += c;
+            lBl -= c;
+        }
+        LBf[(2 * d) * n + k] += lBl;
+        LBf[(2 * d + 1) * n + k] += lBr;
+    }
+}
+
+// The CFL dt differentiated through its maximum wave speed (simulator.jl:147-150).
+// The maximum is maximum(wavespeeds) over cells in memory order, each cell's
+// wavespeed being max(right face speed, left face speed) (numericalflux.jl:29-32);
+// with equal speeds on several faces the derivative depends on which one the
+// maximum returns, so the same one is taken: the first cell in memory order
+// whose wavespeed is the maximum, and its right face unless the left one is
+// strictly faster.  One thread per interior cell; best = its memory index.
+__device__ inline void face_speeds(const KParams& P, const double* U, const double* B, int d, int i, int j, double& sr,
+                                   double& sl) {
+    const int di = d == 0 ? 1 : 0, dj = d == 0 ? 0 : 1;
+    double Lm[3], Rm[3], Ll[3], Rl[3], Lr[3], Rr[3], F[3];
+    recon_at<double>(P, U, B, d, i, j, -2, false, Lm, Rm);
+    recon_at<double>(P, U, B, d, i - di, j - dj, -2, false, Ll, Rl);
+    recon_at<double>(P, U, B, d, i + di, j + dj, -2, false, Lr, Rr);
+    sr = ad::central_upwind<double>(d, Rm, Lr, P.phys, F);
+    sl = ad::central_upwind<double>(d, Rl, Lm, P.phys, F);
+}
+
+__global__ void __launch_bounds__(256) k_adj_argmax(KParams P, const StepCtl* ctl, int dsel, const double* __restrict__ U,
+                                                    const double* __restrict__ B, unsigned long long* __restrict__ best) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= P.nx * P.ny) return;
+    const int i = P.gc + t % P.nx, j = P.gc + t / P.nx;
+    double sr, sl;
+    face_speeds(P, U, B, dsel, i, j, sr, sl);
+    const double ws = djl_max(sr, sl);
+    if ((unsigned long long)__double_as_longlong(ws) == ctl->slot[0].wsmax[dsel])
+        atomicMin(best, (unsigned long long)cid(P, i, j));
+}
+
+__global__ void k_adj_dtface(KParams P, const StepCtl* ctl, int d, int dsel, const double* __restrict__ U,
+                             const double* __restrict__ B, const unsigned long long* __restrict__ best,
+                             const double* __restrict__ sums, double* __restrict__ LfL, double* __restrict__ LfR) {
+    if (threadIdx.x != 0 || blockIdx.x != 0 || d != dsel) return;
+    const unsigned long long kbest = *best;
+    if (kbest >= (unsigned long long)P.nxT * P.nyT) return;
+    const int i = (int)(kbest % P.nxT), j = (int)(kbest / P.nxT);
+    const double ax = __longlong_as_double((long long)ctl->slot[0].wsmax[d]);
+    if (!(ax > 0.0) || !__builtin_isfinite(ax)) return;
+    double sr, sl;
+    face_speeds(P, U, B, d, i, j, sr, sl);
+    const bool right = !(sl > sr);                 // jl_max(sr, sl) returns sr unless sl is larger
+    const int di = d == 0 ? 1 : 0, dj = d == 0 ? 0 : 1;
+    const int ia = right ? i : i - di, ja = right ? j : j - dj;   // the face lies between (ia,ja) and the next cell
+    const int ib = ia + di, jb = ja + dj;
+    const double Dd = d == 0 ? P.dx : P.dy;
+    // dt = cfl * Δ_d / a_d on the selected direction: d dt / d a = -cfl Δ / a^2
+    const double lspeed = sums[0] * P.cfl * (-Dd / (ax * ax));
+    const size_t n = (size_t)P.nxT * P.nyT;
+    double La[3], Ra[3], Lb[3], Rb[3];
+    recon_at<double>(P, U, B, d, ia, ja, -2, false, La, Ra);
+    recon_at<double>(P, U, B, d, ib, jb, -2, false, Lb, Rb);
+    using D6 = Dual<6>;
+    D6 m[3], q[3], F[3];
+    for (int v = 0; v < 3; ++v) { m[v] = D6::var(Ra[v], v); q[v] = D6::var(Lb[v], 3 + v); }
+    const D6 sp = ad::central_upwind<D6>(d, m, q, P.phys, F);
+    for (int a = 0; a < 3; ++a) {
+        LfR[a * n + cid(P, ia, ja)] += lspeed * sp.d[a];
+        LfL[a * n + cid(P, ib, jb)] += lspeed * sp.d[3 + a];
+    }
+}
+
+// Sum the two stages' block partials (dt) and the Cz partials, in order.
+__global__ void k_adj_sum(const double* __restrict__ part_dt, const double* __restrict__ part_cz, int count,
+                          double* __restrict__ sums, double* __restrict__ cz_total) {
+    __shared__ double red[2][256];
+    double a = 0.0, b = 0.0;
+    for (int k = threadIdx.x; k < count; k += blockDim.x) { a += part_dt[k]; b += part_cz[k]; }
+    red[0][threadIdx.x] = a;
+    red[1][threadIdx.x] = b;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) { red[0][threadIdx.x] += red[0][threadIdx.x + s]; red[1][threadIdx.x] += red[1][threadIdx.x + s]; }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        sums[0] = red[0][0];
+        *cz_total += red[1][0];
+    }
+}
+
+// J = sum (w - B_cell - h_target)^2 over interior cells: block partials, the
+// final-state adjoint (w only) and the B_cell adjoint of J itself.
+__global__ void __launch_bounds__(256) k_adj_objective(KParams P, const double* __restrict__ U, const double* __restrict__ B,
+                                                       const double* __restrict__ target, double* __restrict__ L,
+                                                       double* __restrict__ LBc, double* __restrict__ part) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    double e2 = 0.0;
+    if (i >= P.gc && i < P.nxT - P.gc && j >= P.gc && j < P.nyT - P.gc) {
+        const size_t k = cid(P, i, j);
+        const double h = U[k] - Bc4(P, B, i, j);
+        const double e = h - target[(size_t)(i - P.gc) + (size_t)P.nx * (j - P.gc)];
+        e2 = e * e;
+        L[k] = 2.0 * e;
+        LBc[k] += -2.0 * e;
+    }
+    block_sum(e2, part, blockIdx.x + gridDim.x * blockIdx.y);
+}
+
+// Corner gather: B_cell weights 0.25 on its 4 corners, face bottoms 0.5 on 2.
+__global__ void k_adj_corners(KParams P, const double* __restrict__ LBc, const double* __restrict__ LBf,
+                              double* __restrict__ LB) {
+    const int ci = blockIdx.x * blockDim.x + threadIdx.x;
+    const int cj = blockIdx.y * blockDim.y + threadIdx.y;
+    if (ci > P.nxT || cj > P.nyT) return;
+    const size_t n = (size_t)P.nxT * P.nyT;
+    auto in = [&](int i, int j) { return i >= 0 && j >= 0 && i < P.nxT && j < P.nyT; };
+    double a = 0.0;
+    for (int dj = -1; dj <= 0; ++dj)
+        for (int di = -1; di <= 0; ++di)
+            if (in(ci + di, cj + dj)) a += 0.25 * LBc[cid(P, ci + di, cj + dj)];
+    // x faces: left face of (ci, cj) and (ci, cj-1); right face of (ci-1, cj) and (ci-1, cj-1)
+    for (int dj = -1; dj <= 0; ++dj) {
+        if (in(ci, cj + dj)) a += 0.5 * LBf[0 * n + cid(P, ci, cj + dj)];
+        if (in(ci - 1, cj + dj)) a += 0.5 * LBf[1 * n + cid(P, ci - 1, cj + dj)];
+    }
+    // y faces: bottom face of (ci, cj) and (ci-1, cj); top face of (ci, cj-1) and (ci-1, cj-1)
+    for (int di = -1; di <= 0; ++di) {
+        if (in(ci + di, cj)) a += 0.5 * LBf[2 * n + cid(P, ci + di, cj)];
+        if (in(ci + di, cj - 1)) a += 0.5 * LBf[3 * n + cid(P, ci + di, cj - 1)];
+    }
+    LB[(size_t)ci + (size_t)(P.nxT + 1) * cj] = a;
+}
+
+}  // namespace ad
+}  // namespace sfv
+
+// ------------------------------------------------------------------- host
+namespace {
+
+using namespace sfv;
+
+struct AdjBuffers {
+    std::vector<void*> owned;
+    double* alloc(size_t count, std::string& why, int& rc) {
+        void* p = nullptr;
+        if (rc) return nullptr;
+        if (hipMalloc(&p, count * sizeof(double)) != hipSuccess) {
+            rc = SFV_ERR_OUT_OF_MEMORY;
+            why = "hipMalloc failed for the adjoint buffers";
+            return nullptr;
+        }
+        owned.push_back(p);
+        (void)hipMemset(p, 0, count * sizeof(double));
+        return (double*)p;
+    }
+    ~AdjBuffers() {
+        for (void* p : owned) (void)hipFree(p);
+    }
+};
+
+#define ATRY(expr)                                                                                   \
+    do {                                                                                             \
+        hipError_t e_ = (expr);                                                                      \
+        if (e_ != hipSuccess) {                                                                      \
+            h->err = std::string(#expr) + ": " + hipGetErrorString(e_);                              \
+            return SFV_ERR_HIP;                                                                      \
+        }                                                                                            \
+    } while (0)
+
+// One forward step U0 -> U2 with the per-pass kernels (bitwise the fused
+// path), keeping the stage-1 state and both right-hand sides.
+int forward_step(sfv_handle* h, double t, const double* U0, double* U1, double* R1, double* R2, double* U2) {
+    const KParams& P = h->P;
+    hipStream_t st = h->stream;
+    ATRY(launch_prepare(st, P, h->ctl, 0, t, NAN, 0, 0, 0.0, 1, 0, SFV_F64));
+    ATRY(launch_flux<double>(st, P, h->ctl, 0, U0, R1, h->B, 1));
+    ATRY(launch_finish<double>(st, P, h->ctl, 0, 1, U0, U0, R1, U1, h->B));
+    ATRY(launch_bc<double>(st, P, h->ctl, 0, U1, BC_IN_STEP));
+    ATRY(launch_flux<double>(st, P, h->ctl, 0, U1, R2, h->B, 0));
+    ATRY(launch_finish<double>(st, P, h->ctl, 0, 2, U0, U1, R2, U2, h->B));
+    ATRY(launch_bc<double>(st, P, h->ctl, 0, U2, BC_LAST_STAGE));
+    return SFV_OK;
+}
+
+}  // namespace
+
+extern "C" int sfv_adjoint(sfv_handle* h, int64_t nsteps, int32_t checkpoint_every, double* t_io,
+                           const double* h_target, double* J_out, double* dJ_dB, double* dJ_dCz, double* dJ_dU0) {
+    if (!h || !t_io || !h_target) return SFV_ERR_ARGUMENT;
+    const sfv_config& c = h->cfg;
+    if (c.precision != SFV_F64 || c.stepper != SFV_RUNGE_KUTTA2 || c.reconstruction != SFV_LINEAR_RECONSTRUCTION ||
+        c.rain == SFV_RAIN_FIELD || c.embedded_1d || !h->fusable()) {
+        h->err = "the adjoint runs Float64 RungeKutta2 with LinearReconstruction, ghost sources inside the grid "
+                 "(WallBC or PeriodicBC), and no FunctionalRain";
+        return SFV_ERR_ARGUMENT;
+    }
+    if (nsteps < 1 || checkpoint_every < 1) { h->err = "nsteps and checkpoint_every must be >= 1"; return SFV_ERR_ARGUMENT; }
+    if (!h->has_state) { h->err = "set the state before the adjoint"; return SFV_ERR_STATE; }
+    ATRY(hipSetDevice(h->device));
+    const KParams& P = h->P;
+    hipStream_t st = h->stream;
+    const size_t n = h->ncell, n3 = 3 * n, nB = (size_t)(P.nxT + 1) * (P.nyT + 1);
+    const int64_t ce = checkpoint_every;
+    const int64_t nseg = (nsteps + ce - 1) / ce;
+    std::string why;
+    int rc = SFV_OK;
+    AdjBuffers buf;
+    double* ck = buf.alloc(n3 * (size_t)nseg, why, rc);         // checkpoint states
+    double* seg = buf.alloc(n3 * (size_t)(ce + 1), why, rc);    // the states of one segment
+    double* U1 = buf.alloc(n3, why, rc);
+    double* R1 = buf.alloc(n3, why, rc);
+    double* R2 = buf.alloc(n3, why, rc);
+    double* Ut = buf.alloc(n3, why, rc);                        // scratch output
+    double* L = buf.alloc(n3, why, rc);                         // adjoint of the current step's output
+    double* Lnew = buf.alloc(n3, why, rc);                      // adjoint of its input
+    double* L1 = buf.alloc(n3, why, rc);                        // adjoint of U1
+    double* LR = buf.alloc(n3, why, rc);
+    double* LfL = buf.alloc(n3, why, rc);
+    double* LfR = buf.alloc(n3, why, rc);
+    double* LBc = buf.alloc(n, why, rc);
+    double* LBf = buf.alloc(4 * n, why, rc);
+    double* LB = buf.alloc(nB, why, rc);
+    double* tgt = buf.alloc((size_t)P.nx * P.ny, why, rc);
+    const dim3 cb(64, 4), cg((P.nxT + 63) / 64, (P.nyT + 3) / 4);
+    const int nblk = (int)(cg.x * cg.y);
+    double* part_dt = buf.alloc(2 * (size_t)nblk, why, rc);
+    double* part_cz = buf.alloc(2 * (size_t)nblk, why, rc);
+    double* sums = buf.alloc(4, why, rc);                      // [0] dt adjoint of the step, [1] Cz total, [2] J
+    double* part_j = buf.alloc((size_t)nblk, why, rc);
+    unsigned long long* best = (unsigned long long*)buf.alloc(1, why, rc);
+    if (rc) { h->err = why; return rc; }
+    ATRY(hipMemcpyAsync(tgt, h_target, sizeof(double) * P.nx * P.ny, hipMemcpyHostToDevice, st));
+
+    // forward pass with checkpoints; step start times kept on the host
+    std::vector<double> times((size_t)nsteps + 1);
+    double t = *t_io;
+    double* cur = (double*)h->U[h->cur & 1];
+    ATRY(hipMemcpyAsync(seg, cur, n3 * sizeof(double), hipMemcpyDeviceToDevice, st));
+    for (int64_t s = 0; s < nsteps; ++s) {
+        double* a = seg + (size_t)(s & 1) * n3;
+        double* b = seg + (size_t)((s + 1) & 1) * n3;
+        if (s % ce == 0) ATRY(hipMemcpyAsync(ck + (size_t)(s / ce) * n3, a, n3 * sizeof(double), hipMemcpyDeviceToDevice, st));
+        times[(size_t)s] = t;
+        if ((rc = forward_step(h, t, a, U1, R1, R2, b))) return rc;
+        ATRY(hipMemcpyAsync(h->ctl_host, h->ctl, sizeof(StepCtl), hipMemcpyDeviceToHost, st));
+        ATRY(hipStreamSynchronize(st));
+        t = h->ctl_host->t;
+    }
+    times[(size_t)nsteps] = t;
+    const double last_dt = h->ctl_host->dt;
+    double* UT = seg + (size_t)(nsteps & 1) * n3;
+    // the final state becomes the handle's state
+    ATRY(hipMemcpyAsync(cur, UT, n3 * sizeof(double), hipMemcpyDeviceToDevice, st));
+
+    // objective and its adjoint
+    hipLaunchKernelGGL(ad::k_adj_objective, cg, cb, 0, st, P, UT, h->B, tgt, L, LBc, part_j);
+    ATRY(hipPeekAtLastError());
+    std::vector<double> pj((size_t)nblk);
+    ATRY(hipMemcpyAsync(pj.data(), part_j, sizeof(double) * nblk, hipMemcpyDeviceToHost, st));
+    ATRY(hipStreamSynchronize(st));
+    double J = 0.0;
+    for (double v : pj) J += v;
+
+    const int bcx = (P.ny + 255) / 256, bcy = (P.nx + 4 * P.gc * P.gc + 255) / 256;
+    auto faces_grid = [&](int d) {
+        const int nT = d == 0 ? P.nxT : P.nyT, nO = d == 0 ? P.ny : P.nx;
+        return dim3((unsigned)(((nT - 2 * P.gc + 1) * nO + 255) / 256));
+    };
+    for (int64_t sg = nseg - 1; sg >= 0; --sg) {
+        const int64_t s0 = sg * ce, s1 = std::min<int64_t>(nsteps, s0 + ce);
+        ATRY(hipMemcpyAsync(seg, ck + (size_t)sg * n3, n3 * sizeof(double), hipMemcpyDeviceToDevice, st));
+        for (int64_t s = s0; s < s1 - 1; ++s) {
+            double* a = seg + (size_t)(s - s0) * n3;
+            if ((rc = forward_step(h, times[(size_t)s], a, U1, R1, R2, a + n3))) return rc;
+        }
+        for (int64_t s = s1 - 1; s >= s0; --s) {
+            const double* U0 = seg + (size_t)(s - s0) * n3;
+            if ((rc = forward_step(h, times[(size_t)s], U0, U1, R1, R2, Ut))) return rc;
+            // fold the output's ghosts (its boundary condition)
+            hipLaunchKernelGGL(ad::k_adj_bc, dim3(bcx), dim3(256), 0, st, P, L, 0);
+            hipLaunchKernelGGL(ad::k_adj_bc, dim3(bcy), dim3(256), 0, st, P, L, 1);
+            ATRY(hipMemsetAsync(Lnew, 0, n3 * sizeof(double), st));
+            ATRY(hipMemsetAsync(L1, 0, n3 * sizeof(double), st));
+            // stage 2: U2 = clamp(friction(0.5 (U0 + U1 + dt R2)))
+            hipLaunchKernelGGL(ad::k_adj_tail, cg, cb, 0, st, P, h->ctl, 2, U0, U1, R2, h->B, L, L1, Lnew, LR, LBc,
+                               part_dt, part_cz);
+            for (int d = 0; d < 2; ++d) {
+                hipLaunchKernelGGL(ad::k_adj_faces, faces_grid(d), dim3(256), 0, st, P, d, U1, h->B, LR, LfL, LfR);
+                hipLaunchKernelGGL(ad::k_adj_recon, cg, cb, 0, st, P, d, U1, h->B, LR, LfL, LfR, L1, LBf);
+            }
+            hipLaunchKernelGGL(ad::k_adj_bc, dim3(bcx), dim3(256), 0, st, P, L1, 0);
+            hipLaunchKernelGGL(ad::k_adj_bc, dim3(bcy), dim3(256), 0, st, P, L1, 1);
+            // stage 1: U1 = clamp(friction(U0 + dt R1))
+            hipLaunchKernelGGL(ad::k_adj_tail, cg, cb, 0, st, P, h->ctl, 1, U0, U0, R1, h->B, L1, Lnew, nullptr, LR,
+                               LBc, part_dt, part_cz);
+            hipLaunchKernelGGL(ad::k_adj_sum, dim3(1), dim3(256), 0, st, part_dt, part_cz, 2 * nblk, sums, sums + 1);
+            // the direction that set dt (jl_min picks y on a tie)
+            ATRY(hipMemcpyAsync(h->ctl_host, h->ctl, sizeof(StepCtl), hipMemcpyDeviceToHost, st));
+            ATRY(hipStreamSynchronize(st));
+            const double ax = __builtin_bit_cast(double, (long long)h->ctl_host->slot[0].wsmax[0]);
+            const double ay = __builtin_bit_cast(double, (long long)h->ctl_host->slot[0].wsmax[1]);
+            const int dsel = (P.dx / ax < P.dy / ay) ? 0 : 1;
+            const unsigned long long none = ~0ull;
+            ATRY(hipMemcpyAsync(best, &none, sizeof(none), hipMemcpyHostToDevice, st));
+            hipLaunchKernelGGL(ad::k_adj_argmax, dim3((P.nx * P.ny + 255) / 256), dim3(256), 0, st, P, h->ctl, dsel, U0,
+                               h->B, best);
+            if (std::isfinite(ax) && std::isfinite(ay) && (ax > 0.0 || ay > 0.0)) {
+                unsigned long long found = none;
+                ATRY(hipMemcpyAsync(&found, best, sizeof(found), hipMemcpyDeviceToHost, st));
+                ATRY(hipStreamSynchronize(st));
+                if (found == none) {
+                    h->err = "adjoint: no face reproduces the step's maximum wave speed";
+                    return SFV_ERR_STATE;
+                }
+            }
+            for (int d = 0; d < 2; ++d) {
+                hipLaunchKernelGGL(ad::k_adj_faces, faces_grid(d), dim3(256), 0, st, P, d, U0, h->B, LR, LfL, LfR);
+                hipLaunchKernelGGL(ad::k_adj_dtface, dim3(1), dim3(64), 0, st, P, h->ctl, d, dsel, U0, h->B, best, sums,
+                                   LfL, LfR);
+                hipLaunchKernelGGL(ad::k_adj_recon, cg, cb, 0, st, P, d, U0, h->B, LR, LfL, LfR, Lnew, LBf);
+            }
+            ATRY(hipPeekAtLastError());
+            std::swap(L, Lnew);
+        }
+    }
+    // the initial state's own boundary condition
+    hipLaunchKernelGGL(ad::k_adj_bc, dim3(bcx), dim3(256), 0, st, P, L, 0);
+    hipLaunchKernelGGL(ad::k_adj_bc, dim3(bcy), dim3(256), 0, st, P, L, 1);
+    hipLaunchKernelGGL(ad::k_adj_corners, dim3((P.nxT + 1 + 63) / 64, (P.nyT + 1 + 3) / 4), cb, 0, st, P, LBc, LBf, LB);
+    ATRY(hipPeekAtLastError());
+    double czsum = 0.0;
+    ATRY(hipMemcpyAsync(&czsum, sums + 1, sizeof(double), hipMemcpyDeviceToHost, st));
+    if (dJ_dB) ATRY(hipMemcpyAsync(dJ_dB, LB, nB * sizeof(double), hipMemcpyDeviceToHost, st));
+    std::vector<double> Lh;
+    if (dJ_dU0) {
+        Lh.resize(n3);
+        ATRY(hipMemcpyAsync(Lh.data(), L, n3 * sizeof(double), hipMemcpyDeviceToHost, st));
+    }
+    ATRY(hipStreamSynchronize(st));
+    if (dJ_dU0)
+        for (int v = 0; v < 3; ++v)
+            for (int j = 0; j < P.ny; ++j)
+                for (int i = 0; i < P.nx; ++i)
+                    dJ_dU0[(size_t)i + (size_t)P.nx * (j + (size_t)P.ny * v)] =
+                        Lh[v * n + (size_t)(i + P.gc) + (size_t)P.nxT * (j + P.gc)];
+    if (dJ_dCz) *dJ_dCz = czsum;
+    if (J_out) *J_out = J;
+    *t_io = t;
+    h->last_dt = last_dt;
+    h->have_dt = true;
+    return SFV_OK;
+}

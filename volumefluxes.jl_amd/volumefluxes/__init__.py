@@ -30,6 +30,7 @@ from .simulator import (HIPBackend, IntervalWriter, MultipleCallbacks, Simulator
 
 from . import io, model  # noqa: E402
 from .io import SnapshotWriter, TotalWaterVolume, bottom_per_cell, coarsen, load_grid  # noqa: E402
+from .adjoint import adjoint_gradient, manning_gradient  # noqa: E402
 from .batch import Ensemble  # noqa: E402
 from .serving import Server  # noqa: E402
 

@@ -164,6 +164,9 @@ def _declare(L) -> None:
     L.sfv_set_state.argtypes = [h, c_void_p, c_int32]
     L.sfv_set_rain_field.argtypes = [h, POINTER(c_double)]
     L.sfv_set_rain_field.restype = c_int32
+    L.sfv_adjoint.argtypes = [h, c_int64, c_int32, POINTER(c_double), POINTER(c_double), POINTER(c_double),
+                              POINTER(c_double), POINTER(c_double), POINTER(c_double)]
+    L.sfv_adjoint.restype = c_int32
     L.sfv_set_state.restype = c_int32
     L.sfv_get_state.argtypes = [h, c_void_p, c_int32]
     L.sfv_get_state.restype = c_int32
