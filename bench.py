@@ -308,6 +308,106 @@ def serving_bench(rate: float, duration: float, t_range, budget_gb: float, seed:
     }
 
 
+def adjoint_bench(seed: int, n: int = 512, steps: int = 200, checkpoint_every: int = 20):
+    """Gradient of J = sum (h_T - h_target)^2 with respect to every bottom
+    corner and Manning's n on the n x n terrain (f64, RK2, `steps` steps),
+    h_target from a forward run on a perturbed topography.  Checked by
+    central finite differences on seeded corners and on n, and against the
+    oracle's forward-mode derivative along a random direction."""
+    from volumefluxes.scenarios import cell_bottom
+    grid, system, _, _, B = build_scenario(n, "f64", seed)
+    backend = VF.make_hip_backend(np.float64)
+    g = grid.gc
+    # The bumps of the single-grid workload crowd a smaller domain: at 2 m the
+    # 512 m terrain is dry everywhere.  The initial surface is set so the wet
+    # fraction matches the 2048 m workload's (35%).
+    w0 = float(np.quantile(cell_bottom(B)[g:-g, g:-g], 0.35))
+    U0 = terrain_initial_state(grid, B, w0)
+    manning = 0.03
+    xf, yf = grid.cell_faces_dir(0, interior=False), grid.cell_faces_dir(1, interior=False)
+    # the target's bottom perturbation sits in the deepest water, away from dry ground
+    depth = U0[:, :, 0] - cell_bottom(B)[g:-g, g:-g]
+    ci, cj = np.unravel_index(np.argmax(depth), depth.shape)
+    cx, cy = xf[ci + g] + 0.5, yf[cj + g] + 0.5
+    bump = np.exp(-((xf[:, None] - cx) ** 2 + (yf[None, :] - cy) ** 2) / (2 * (0.04 * n) ** 2))
+
+    def system_for(Bx, Cz):
+        eq = VF.ShallowWaterEquations(VF.BottomTopography2D(Bx, None, grid), desingularizing_kappa=1e-4)
+        return VF.ConservedSystem(None, VF.LinearReconstruction(), VF.CentralUpwind(eq), eq, grid,
+                                  [VF.SourceTermBottom(), VF.ConstantRain(1e-5 * 3600)], VF.ImplicitFriction(Cz=Cz))
+
+    def forward_J(Bx, Cz, target):
+        sim = VF.Simulator(backend, system_for(Bx, Cz), VF.RungeKutta2(), grid, cfl=0.25)
+        VF.set_current_state(sim, U0)
+        VF.advance_steps(sim, steps)
+        U = sim.get_full_state()
+        h = U[g:-g, g:-g, 0] - cell_bottom(Bx)[g:-g, g:-g]
+        return float(np.sum((h - target) ** 2)) if target is not None else h
+
+    Bp = np.asfortranarray(B + 0.2 * bump)
+    target = forward_J(Bp, manning ** 2, None)
+
+    sim = VF.Simulator(backend, system_for(B, manning ** 2), VF.RungeKutta2(), grid, cfl=0.25)
+    VF.set_current_state(sim, U0)
+    t0 = time.perf_counter()
+    VF.advance_steps(sim, steps)
+    C.check(sim.handle, C.lib().sfv_synchronize(sim.handle))
+    fwd_s = time.perf_counter() - t0
+    VF.set_current_state(sim, U0)
+    sim.t = 0.0
+    VF.adjoint_gradient(VF.Simulator(backend, system_for(B, manning ** 2), VF.RungeKutta2(), grid), 2, target)  # warm-up
+    t0 = time.perf_counter()
+    res = VF.adjoint_gradient(sim, steps, target, checkpoint_every=checkpoint_every)
+    adj_s = time.perf_counter() - t0
+    gB, gn = res["dJ_dB"], VF.manning_gradient(res["dJ_dCz"], manning)
+
+    rng = np.random.default_rng(seed)
+    corner_depth = np.full(B.shape, -1.0)
+    corner_depth[g + 1:-g - 1, g + 1:-g - 1] = np.minimum.reduce(
+        [depth[:-1, :-1], depth[1:, :-1], depth[:-1, 1:], depth[1:, 1:]])   # all four cells of the corner
+    wet = np.argwhere((bump > 0.3) & (corner_depth > 0.5))
+    picks = [tuple(wet[k]) for k in rng.choice(len(wet), 3, replace=False)]
+    checks = []
+    eps = 1e-3
+    for (i, j) in picks:
+        Bq = B.copy(order="F"); Bq[i, j] += eps
+        Bm = B.copy(order="F"); Bm[i, j] -= eps
+        fd = (forward_J(Bq, manning ** 2, target) - forward_J(Bm, manning ** 2, target)) / (2 * eps)
+        checks.append({"corner": [int(i), int(j)], "adjoint": float(gB[i, j]), "fd": fd,
+                       "rel": abs(gB[i, j] - fd) / max(abs(fd), 1e-300)})
+    hn = 1e-3 * manning
+    fdn = (forward_J(B, (manning + hn) ** 2, target) - forward_J(B, (manning - hn) ** 2, target)) / (2 * hn)
+    out = {
+        "workload": (f"{n}x{n} terrain (seeded bumps, rain, Manning n = {manning}, initial surface "
+                     f"w = max(B, {w0:.3f} m) for 35% wet cells), f64 RK2, {steps} steps, "
+                     f"checkpoints every {checkpoint_every}; J = sum (h_T - h_target)^2, h_target from a run on "
+                     f"B + 0.2 m Gaussian bump; gradients for all {gB.size} bottom corners and n"),
+        "J": res["J"], "forward_s": fwd_s, "adjoint_s": adj_s, "adjoint_over_forward": adj_s / fwd_s,
+        "dJ_dn": gn,
+        "fd_corners": checks,
+        "fd_dJ_dn": {"fd": fdn, "step": hn,
+                     "note": ("J(n) jumps wherever a cell's depth crosses the dry cut-off (the clamp zeroes its "
+                              "momentum), so central differences in n straddle jumps and do not converge; the "
+                              "oracle's forward mode below is the check for dJ/dn")},
+    }
+    try:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O  # the checker
+        dB = np.asfortranarray(rng.normal(size=B.shape) * bump)
+        cfg = VF.build_config(system_for(B, manning ** 2), VF.RungeKutta2(), grid, 0.25, C.SFV_F64)
+        t1 = time.perf_counter()
+        _, (jvp, jvc) = O.ad_run2(cfg, B, U0, steps, target, (dB, 0.0, None), (None, 1.0, None))
+        got = float(np.sum(gB * dB))
+        jvn = VF.manning_gradient(jvc, manning)
+        out["oracle_forward_mode"] = {"random_dB": {"adjoint": got, "oracle": jvp,
+                                                    "rel": abs(got - jvp) / max(abs(jvp), 1e-300)},
+                                      "dJ_dn": {"adjoint": gn, "oracle": jvn, "rel": abs(gn - jvn) / max(abs(jvn), 1e-300)},
+                                      "oracle_s": time.perf_counter() - t1}
+    except (ImportError, OSError) as e:      # the oracle library is not built
+        out["oracle_forward_mode"] = {"skipped": str(e)}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -327,6 +427,7 @@ def main():
     ap.add_argument("--serving-duration", type=float, default=1.0, help="seconds of arrivals")
     ap.add_argument("--serving-tend", type=float, nargs=2, default=[1.0, 30.0], help="t_end range of a request (s)")
     ap.add_argument("--serving-budget-gb", type=float, default=64.0)
+    ap.add_argument("--adjoint-size", type=int, default=512, help="adjoint leg grid side (0: skip)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -441,6 +542,8 @@ def main():
     if rank == 0 and args.serving_rate > 0:
         result["serving"] = serving_bench(args.serving_rate, args.serving_duration, tuple(args.serving_tend),
                                           args.serving_budget_gb, args.seed)
+    if rank == 0 and args.adjoint_size > 0:
+        result["adjoint"] = adjoint_bench(args.seed, args.adjoint_size)
     print(json.dumps(result))
     return 0
 

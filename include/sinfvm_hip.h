@@ -169,12 +169,14 @@ int64_t sfv_device_bytes(const sfv_handle* h);
 /* Bytes of device memory a handle with this configuration would hold. */
 int64_t sfv_device_bytes_for(const sfv_config* cfg);
 
-/* Kernel-timing hooks for measurement: launch only the named kernel of one
- * RK2 step on the current state ("stage1", "stage2", "bc", or "step" for all
- * three) n times and report the mean device time in milliseconds from HIP
- * events on the handle's stream.  "copy" times a plain device copy of one
- * state (3 planes), the known-traffic kernel for calibrating HBM byte
- * counters.  The state is left unchanged.  No counterpart in the reference. */
+/* Kernel-timing hook for measurement: run n RK2 steps' kernels from the
+ * current state (stage 1, stage 2 with the dt stage 1's wave speeds give,
+ * boundary condition) into a scratch output, with HIP events on the handle's
+ * stream between consecutive kernels, after one warm-up step, and report the
+ * named kernel's mean device time in milliseconds ("stage1", "stage2", "bc",
+ * or "step" for the three).  "copy" times a plain device copy of one state (3
+ * planes), the known-traffic kernel for calibrating HBM byte counters.  The
+ * state is left unchanged.  No counterpart in the reference. */
 int sfv_time_kernel(sfv_handle* h, const char* which, int32_t n, double* ms_out);
 
 /* Number of visible HIP devices (0 when none).  Needs no handle. */

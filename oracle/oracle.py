@@ -54,6 +54,9 @@ def lib():
         L.oracle_ad_run.argtypes = [c_void_p, POINTER(d), POINTER(d), ctypes.c_int64, POINTER(d), POINTER(d), d,
                                     POINTER(d), POINTER(d), POINTER(d), POINTER(d)]
         L.oracle_ad_run.restype = d
+        L.oracle_ad_run2.argtypes = [c_void_p, POINTER(d), POINTER(d), ctypes.c_int64, POINTER(d), POINTER(d), d,
+                                     POINTER(d), POINTER(d), d, POINTER(d), POINTER(d)]
+        L.oracle_ad_run2.restype = d
         L.oracle_ad_forward.argtypes = [c_void_p, POINTER(d), POINTER(d), ctypes.c_int64, POINTER(d), POINTER(d)]
         L.oracle_ad_forward.restype = None
         L.oracle2d_set_interior.argtypes = [c_void_p, POINTER(d)]
@@ -249,3 +252,15 @@ def ad_forward(cfg, B, U0, nsteps):
     dts = np.empty(int(nsteps))
     lib().oracle_ad_forward(ctypes.byref(cfg), _dp(B), _dp(U0), int(nsteps), _dp(U), _dp(dts))
     return U, dts
+
+
+def ad_run2(cfg, B, U0, nsteps, target, dir1, dir2):
+    """Two directional derivatives in one pass; dir = (dB, dCz, dU0).
+    Returns (J, [dJ1, dJ2])."""
+    B, U0, target = _f(B), _f(U0), _f(target)
+    (b1, c1, u1), (b2, c2, u2) = dir1, dir2
+    b1, u1, b2, u2 = _f(b1), _f(u1), _f(b2), _f(u2)
+    out = (c_double * 2)()
+    J = lib().oracle_ad_run2(ctypes.byref(cfg), _dp(B), _dp(U0), int(nsteps), _dp(target), _dp(b1), float(c1),
+                             _dp(u1), _dp(b2), float(c2), _dp(u2), out)
+    return J, [out[0], out[1]]

@@ -215,42 +215,70 @@ double oracle1d_current_timestep(void* hp) { return ((Sim1D*)hp)->current_timest
 
 // ---- forward-mode derivative of a fixed-step run (oracle_ad.hpp)
 // J = sum over interior cells of (w - B_cell - target)^2 after nsteps steps
-// of perform_step!(sim, Inf) from the interior state U0 (nx, ny, 3); with
-// directions dB (corners), dCz and dU0 (interior, may be NULL) returns J and
-// writes dJ along that direction.  U_out (nxT, nyT, 3) may be NULL.
-double oracle_ad_run(const sfv_config* cfg, const double* B, const double* U0, int64_t nsteps, const double* target,
-                     const double* dB, double dCz, const double* dU0, double* dJ_out, double* U_out, double* dts) {
-    using D1 = oracle_ad::Dual<1>;
-    oracle_ad::Step2D<D1> s(*cfg);
+// of perform_step!(sim, Inf) from the interior state U0 (nx, ny, 3); with N
+// directions (dB[k] over the corners, dCz[k], dU0[k] over the interior; any
+// array may be NULL) returns J and writes the N directional derivatives.
+}  // extern "C"
+template <int N>
+static double ad_run(const sfv_config* cfg, const double* B, const double* U0, int64_t nsteps, const double* target,
+                     const double* const* dB, const double* dCz, const double* const* dU0, double* dJ_out,
+                     double* U_out, double* dts) {
+    using DN = oracle_ad::Dual<N>;
+    oracle_ad::Step2D<DN> s(*cfg);
     const size_t nB = (size_t)(s.nxT + 1) * (s.nyT + 1);
-    for (size_t k = 0; k < nB; ++k) { s.B[k] = D1(B[k]); s.B[k].d[0] = dB ? dB[k] : 0.0; }
-    s.Cz.d[0] = dCz;
+    for (size_t k = 0; k < nB; ++k) {
+        s.B[k] = DN(B[k]);
+        for (int q = 0; q < N; ++q) s.B[k].d[q] = dB[q] ? dB[q][k] : 0.0;
+    }
+    for (int q = 0; q < N; ++q) s.Cz.d[q] = dCz[q];
     const int nx = cfg->nx, ny = cfg->ny, gc = cfg->gc;
     for (int v = 0; v < 3; ++v)
         for (int j = 0; j < ny; ++j)
             for (int i = 0; i < nx; ++i) {
                 const size_t src = (size_t)v * nx * ny + (size_t)j * nx + i;
-                D1 x(U0[src]);
-                x.d[0] = dU0 ? dU0[src] : 0.0;
+                DN x(U0[src]);
+                for (int q = 0; q < N; ++q) x.d[q] = dU0[q] ? dU0[q][src] : 0.0;
                 s.U[0][v * s.n + s.id(i + gc, j + gc)] = x;
             }
     s.bc(s.U[0]);
     double t = 0.0;
     for (int64_t k = 0; k < nsteps; ++k) {
-        const D1 dt = s.step(INFINITY, t);
+        const DN dt = s.step(INFINITY, t);
         if (dts) dts[k] = dt.v;
         t += dt.v;
     }
-    D1 J(0.0);
+    DN J(0.0);
     for (int j = gc; j < s.nyT - gc; ++j)
         for (int i = gc; i < s.nxT - gc; ++i) {
-            const D1 e = (s.U[0][s.id(i, j)] - s.B_cell(i, j)) - target[(size_t)(i - gc) + (size_t)nx * (j - gc)];
+            const DN e = (s.U[0][s.id(i, j)] - s.B_cell(i, j)) - target[(size_t)(i - gc) + (size_t)nx * (j - gc)];
             J = J + e * e;
         }
     if (U_out)
         for (size_t k = 0; k < 3 * s.n; ++k) U_out[k] = s.U[0][k].v;
-    if (dJ_out) *dJ_out = J.d[0];
+    for (int q = 0; q < N; ++q) dJ_out[q] = J.d[q];
     return J.v;
+}
+
+extern "C" {
+double oracle_ad_run(const sfv_config* cfg, const double* B, const double* U0, int64_t nsteps, const double* target,
+                     const double* dB, double dCz, const double* dU0, double* dJ_out, double* U_out, double* dts) {
+    const double* b[1] = {dB};
+    const double c[1] = {dCz};
+    const double* u[1] = {dU0};
+    double out[1];
+    const double J = ad_run<1>(cfg, B, U0, nsteps, target, b, c, u, out, U_out, dts);
+    if (dJ_out) *dJ_out = out[0];
+    return J;
+}
+
+// Two directions in one pass.
+double oracle_ad_run2(const sfv_config* cfg, const double* B, const double* U0, int64_t nsteps, const double* target,
+                      const double* dB1, double dCz1, const double* dU1, const double* dB2, double dCz2,
+                      const double* dU2, double* dJ_out) {
+    const double* b[2] = {dB1, dB2};
+    const double c[2] = {dCz1, dCz2};
+    const double* u[2] = {dU1, dU2};
+    return ad_run<2>(cfg, B, U0, nsteps, target, b, c, u, dJ_out, nullptr, nullptr);
 }
 
 // The same run on plain doubles (pins oracle_ad.hpp against Sim2D).

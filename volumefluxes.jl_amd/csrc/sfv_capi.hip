@@ -401,46 +401,55 @@ int sfv_time_kernel(sfv_handle* h, const char* which, int32_t n, double* ms_out)
         if (rc2) return fail(h, rc2, why);
     }
     const int slot = (int)(h->step_parity & 1), cur = (int)(h->cur & 1);
-    // Time the kernels against a scratch output so the state is left as is.
+    // The kernels run as in a step — stage 1, stage 2 (with the dt stage 1's
+    // wave speeds give), boundary condition — against a scratch output so the
+    // state is left as is, with an event between consecutive kernels; the
+    // named kernel's mean duration is returned.  The first step is a warm-up.
     HIP_TRY(h, launch_prepare(h->stream, h->P, h->ctl, slot, 0.0, NAN, 0, 0, 0.0, -1, 0, h->cfg.precision));
-    hipEvent_t e0, e1;
-    HIP_TRY(h, hipEventCreate(&e0));
-    HIP_TRY(h, hipEventCreate(&e1));
     auto stage = [&](int which, auto* U0, auto* R, auto* Us) -> hipError_t {
-        using S = std::remove_pointer_t<decltype(U0)>;
         const KParams& P = h->P;
         if (h->fused == KV_TILED)
-            return which == 1 ? launch_stage1<S>(h->stream, P, h->ctl, slot, U0, R, h->B)
-                              : launch_stage2<S>(h->stream, P, h->ctl, slot, U0, R, Us, h->B);
-        return which == 1 ? launch_march<S>(h->stream, P, h->ctl, slot, 1, U0, nullptr, R, h->B)
-                          : launch_march<S>(h->stream, P, h->ctl, slot, 2, U0, R, Us, h->B);
+            return which == 1 ? launch_stage1(h->stream, P, h->ctl, slot, U0, R, h->B)
+                              : launch_stage2(h->stream, P, h->ctl, slot, U0, R, Us, h->B);
+        return which == 1 ? launch_march(h->stream, P, h->ctl, slot, 1, U0, (decltype(U0))nullptr, R, h->B)
+                          : launch_march(h->stream, P, h->ctl, slot, 2, U0, R, Us, h->B);
     };
-    auto run = [&](auto* U0, auto* R, auto* Us) -> hipError_t {
-        using S = std::remove_pointer_t<decltype(U0)>;
-        if (w == "stage1") return stage(1, U0, R, Us);
-        if (w == "stage2") return stage(2, U0, R, Us);
-        if (w == "bc") return launch_bc<S>(h->stream, h->P, h->ctl, slot, Us, BC_IN_STEP);
-        hipError_t e = stage(1, U0, R, Us);
-        if (e == hipSuccess) e = stage(2, U0, R, Us);
-        if (e == hipSuccess) e = launch_bc<S>(h->stream, h->P, h->ctl, slot, Us, BC_IN_STEP);
-        return e;
+    auto kernel = [&](int k) -> hipError_t {      // 0 stage 1, 1 stage 2, 2 boundary condition, 3 copy
+        if (k == 3) return launch_copy(h->stream, h->U[cur], h->U[2], 3 * h->ncell * h->elem);
+        if (h->cfg.precision == SFV_F64) {
+            double *U0 = (double*)h->U[cur], *R = (double*)h->RHS, *Us = (double*)h->U[2];
+            return k == 2 ? launch_bc<double>(h->stream, h->P, h->ctl, slot, Us, BC_IN_STEP) : stage(k + 1, U0, R, Us);
+        }
+        float *U0 = (float*)h->U[cur], *R = (float*)h->RHS, *Us = (float*)h->U[2];
+        return k == 2 ? launch_bc<float>(h->stream, h->P, h->ctl, slot, Us, BC_IN_STEP) : stage(k + 1, U0, R, Us);
     };
-    auto one = [&]() -> hipError_t {
-        if (w == "copy") return launch_copy(h->stream, h->U[cur], h->U[2], 3 * h->ncell * h->elem);
-        if (h->cfg.precision == SFV_F64)
-            return run((double*)h->U[cur], (double*)h->RHS, (double*)h->U[2]);
-        return run((float*)h->U[cur], (float*)h->RHS, (float*)h->U[2]);
-    };
-    HIP_TRY(h, one());   // warm-up
-    HIP_TRY(h, hipEventRecord(e0, h->stream));
-    for (int r = 0; r < n; ++r) HIP_TRY(h, one());
-    HIP_TRY(h, hipEventRecord(e1, h->stream));
-    HIP_TRY(h, hipEventSynchronize(e1));
-    float ms = 0;
-    HIP_TRY(h, hipEventElapsedTime(&ms, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    *ms_out = (double)ms / n;
+    const bool copy = w == "copy";
+    const int nk = copy ? 1 : 3;
+    const int runs = n + 1;
+    std::vector<hipEvent_t> ev((size_t)runs * (nk + 1));
+    for (auto& e : ev) HIP_TRY(h, hipEventCreate(&e));
+    hipError_t le = hipSuccess;
+    for (int r = 0; r < runs && le == hipSuccess; ++r)
+        for (int k = 0; k <= nk && le == hipSuccess; ++k) {
+            le = hipEventRecord(ev[(size_t)r * (nk + 1) + k], h->stream);
+            if (k < nk && le == hipSuccess) le = kernel(copy ? 3 : k);
+        }
+    if (le == hipSuccess) le = hipEventSynchronize(ev.back());
+    double tot[4] = {0, 0, 0, 0};
+    for (int r = 1; r < runs && le == hipSuccess; ++r)
+        for (int k = 0; k < nk && le == hipSuccess; ++k) {
+            float ms = 0;
+            le = hipEventElapsedTime(&ms, ev[(size_t)r * (nk + 1) + k], ev[(size_t)r * (nk + 1) + k + 1]);
+            tot[k] += ms;
+        }
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    HIP_TRY(h, le);
+    double ms = 0;
+    if (copy || w == "stage1") ms = tot[0];
+    else if (w == "stage2") ms = tot[1];
+    else if (w == "bc") ms = tot[2];
+    else ms = tot[0] + tot[1] + tot[2];
+    *ms_out = ms / n;
     return SFV_OK;
 }
 
