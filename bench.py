@@ -91,6 +91,18 @@ def cpu_baseline(grid, system, realtype, U0, B, steps_per_run: int, runs: int, t
     return first, grid.nx * grid.ny * steps_per_run / med, med
 
 
+def oracle_first_step(grid, system, realtype, U0, B):
+    """The oracle's state after one step from U0 (the parity probe's reference)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O  # the checker
+    O.set_threads(os.cpu_count() or 1)
+    cfg = VF.build_config(system, VF.RungeKutta2(), grid, 0.25, C.SFV_F64 if realtype == np.float64 else 1)
+    o = O.Oracle2D(cfg, B)
+    o.set_interior(U0)
+    o.step(float("inf"), 0.0)
+    return o.state()
+
+
 ENSEMBLE_SIDES = (128, 256, 512, 768, 1024)
 
 
@@ -298,7 +310,8 @@ def serving_bench(rate: float, duration: float, t_range, budget_gb: float, seed:
         "offered_cell_updates_per_s": total / duration,
         "served": {"p50_s": pct(lat, 50), "p99_s": pct(lat, 99), "queue_wait_p50_s": pct(wait, 50),
                    "queue_wait_p99_s": pct(wait, 99), "cell_updates_per_s": total / span, "makespan_s": span,
-                   "busy_s": st["busy_s"], "peak_bytes": st["bytes_peak"], "replays": st["replays"]},
+                   "busy_s": st["busy_s"], "host_admit_s": st["host_admit_s"], "host_retire_s": st["host_retire_s"],
+                   "peak_bytes": st["bytes_peak"], "replays": st["replays"]},
         "plain_fifo": {"p50_s": pct(plain_lat, 50), "p99_s": pct(plain_lat, 99),
                        "cell_updates_per_s": total / plain_span, "makespan_s": plain_span,
                        "service_s_total": float(sum(service))},
@@ -408,6 +421,71 @@ def adjoint_bench(seed: int, n: int = 512, steps: int = 200, checkpoint_every: i
     return out
 
 
+def grid_leg(size: int, precision: str, seed: int, steps: int, warmup: int, kernels: str = "march") -> dict:
+    """One terrain run on the GPU: the first step kept for the parity probe,
+    then `warmup` untimed and `steps` timed device-resident steps, and each
+    kernel's mean device time from HIP events inside full steps."""
+    grid, system, realtype, U0, B = build_scenario(size, precision, seed)
+    backend = VF.make_hip_backend(realtype)
+    sim = VF.Simulator(backend, system, VF.RungeKutta2(), grid, cfl=0.25)
+    sim.set_kernels(kernels)
+    L = C.lib()
+
+    # parity probe: the first step, compared bitwise with the oracle's
+    VF.set_current_state(sim, U0)
+    VF.perform_step(sim, float("inf"))
+    first_gpu = sim.get_full_state()
+
+    VF.set_current_state(sim, U0)
+    sim.t = 0.0
+    if warmup > 0:
+        VF.advance_steps(sim, warmup)
+    C.check(sim.handle, L.sfv_synchronize(sim.handle))
+    t0 = time.perf_counter()
+    VF.advance_steps(sim, steps)
+    C.check(sim.handle, L.sfv_synchronize(sim.handle))
+    elapsed = time.perf_counter() - t0
+    value = grid.nx * grid.ny * steps / elapsed
+    finite = bool(np.isfinite(sim.get_full_state()).all())
+
+    ms = {}
+    timing_sim = sim
+    if kernels == "per-pass":       # the stage kernels timed are the fused march ones
+        timing_sim = VF.Simulator(backend, system, VF.RungeKutta2(), grid, cfl=0.25)
+        VF.set_current_state(timing_sim, U0)
+    for k in ("stage1", "stage2", "bc", "step", "copy"):
+        out = ctypes.c_double()
+        C.check(timing_sim.handle, L.sfv_time_kernel(timing_sim.handle, k.encode(), 20, ctypes.byref(out)))
+        ms[k] = out.value
+    elem = 8 if precision == "f64" else 4
+    copy_bytes = 2 * 3 * (grid.nx + 2 * grid.gc) * (grid.ny + 2 * grid.gc) * elem   # read + write of one state
+    return {"scenario": (grid, system, realtype, U0, B), "value": value, "elapsed": elapsed, "ms": ms,
+            "march_rows": sim.march_rows,
+            "dominant": max(("stage1", "stage2"), key=lambda k: ms[k]), "first_gpu": first_gpu,
+            "finite": finite, "copy_gbs": copy_bytes / (ms["copy"] * 1e-3) / 1e9}
+
+
+def f32_leg(args) -> dict:
+    """The same terrain workload with Float32 storage (make_cpu_backend(Float32)
+    semantics: f64 arithmetic rounded to f32 on every store, B and dt f64):
+    throughput, kernel times and the first step compared with the oracle."""
+    leg = grid_leg(args.size, "f32", args.seed, args.steps, args.warmup, args.kernels)
+    grid, system, realtype, U0, B = leg["scenario"]
+    cells = grid.nx * grid.ny
+    dom = leg["dominant"]
+    achieved = BYTES_PER_CELL_STAGE["f32"][dom] * cells / (leg["ms"][dom] * 1e-3) / 1e9
+    out = {"value": leg["value"], "unit": "cell-updates/s", "ms_per_step": leg["elapsed"] / args.steps * 1e3,
+           "steps": args.steps, "kernel_ms": leg["ms"],
+           "roofline": {"kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": achieved / HBM_PEAK_GBS,
+                        "algorithmic_bytes_per_launch": BYTES_PER_CELL_STAGE["f32"][dom] * cells},
+           "state_finite": leg["finite"]}
+    if not args.no_cpu:
+        first_cpu = oracle_first_step(grid, system, realtype, U0, B)
+        out["parity_first_step"] = "bitwise" if np.array_equal(leg["first_gpu"], first_cpu) else "MISMATCH"
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -427,6 +505,8 @@ def main():
     ap.add_argument("--serving-duration", type=float, default=1.0, help="seconds of arrivals")
     ap.add_argument("--serving-tend", type=float, nargs=2, default=[1.0, 30.0], help="t_end range of a request (s)")
     ap.add_argument("--serving-budget-gb", type=float, default=64.0)
+    ap.add_argument("--no-f32", dest="f32_leg", action="store_false",
+                    help="skip the Float32-storage run of the same workload")
     ap.add_argument("--adjoint-size", type=int, default=512, help="adjoint leg grid side (0: skip)")
     args = ap.parse_args()
 
@@ -436,50 +516,16 @@ def main():
         return 2
     rank = int(os.environ.get("RANK", "0"))
 
-    grid, system, realtype, U0, B = build_scenario(args.size, args.precision, args.seed)
-    backend = VF.make_hip_backend(realtype)
-    sim = VF.Simulator(backend, system, VF.RungeKutta2(), grid, cfl=0.25)
-    sim.set_kernels(args.kernels)
-    L = C.lib()
-
-    # parity probe: the first step, compared bitwise with the oracle's below
-    VF.set_current_state(sim, U0)
-    VF.perform_step(sim, float("inf"))
-    first_gpu = sim.get_full_state()
-
-    VF.set_current_state(sim, U0)
-    sim.t = 0.0
-    if args.warmup > 0:
-        VF.advance_steps(sim, args.warmup)
-    C.check(sim.handle, L.sfv_synchronize(sim.handle))
-    t0 = time.perf_counter()
-    VF.advance_steps(sim, args.steps)
-    C.check(sim.handle, L.sfv_synchronize(sim.handle))
-    elapsed = time.perf_counter() - t0
+    leg = grid_leg(args.size, args.precision, args.seed, args.steps, args.warmup, args.kernels)
+    grid, system, realtype, U0, B = leg["scenario"]
+    value, elapsed, ms, dominant = leg["value"], leg["elapsed"], leg["ms"], leg["dominant"]
     cells = grid.nx * grid.ny
-    value = cells * args.steps / elapsed
-    U = sim.get_full_state()
-    finite = bool(np.isfinite(U).all())
-
-    # per-kernel device time from HIP events on the handle's stream
-    ms = {}
-    timing_sim = sim
-    if args.kernels == "per-pass":       # the stage kernels timed are the fused march ones
-        timing_sim = VF.Simulator(backend, system, VF.RungeKutta2(), grid, cfl=0.25)
-        VF.set_current_state(timing_sim, U0)
-    for k in ("stage1", "stage2", "bc", "step", "copy"):
-        out = ctypes.c_double()
-        C.check(timing_sim.handle, L.sfv_time_kernel(timing_sim.handle, k.encode(), 20, ctypes.byref(out)))
-        ms[k] = out.value
-    elem = 8 if args.precision == "f64" else 4
-    copy_bytes = 2 * 3 * (grid.nx + 2 * grid.gc) * (grid.ny + 2 * grid.gc) * elem   # read + write of one state
-    copy_gbs = copy_bytes / (ms["copy"] * 1e-3) / 1e9
-    dominant = max(("stage1", "stage2"), key=lambda k: ms[k])
     alg_bytes = BYTES_PER_CELL_STAGE[args.precision][dominant] * cells
     achieved = alg_bytes / (ms[dominant] * 1e-3) / 1e9
     workload = f"terrain{args.size}_{args.precision}"
     traffic = read_traffic(dominant, workload)
     step_gbs = value * BYTES_PER_CELL_STEP[args.precision] / 1e9
+    first_gpu, finite, copy_gbs = leg["first_gpu"], leg["finite"], leg["copy_gbs"]
 
     result = {
         "metric": METRIC,
@@ -500,6 +546,7 @@ def main():
                          f"{args.steps} steps"),
             "nx": args.size, "ny": args.size, "precision": args.precision,
             "kernels": args.kernels,
+            "march_rows": leg["march_rows"],
             "parallelism": "single-gpu",
         },
         "roofline": {
@@ -537,6 +584,8 @@ def main():
         }
         result["parity_first_step"] = "bitwise" if np.array_equal(first_gpu, first_cpu) else "MISMATCH"
         result["gpu_over_cpu"] = value / cpu_value
+    if rank == 0 and args.precision == "f64" and args.f32_leg:
+        result["f32"] = f32_leg(args)
     if rank == 0 and args.ensemble > 0:
         result["ensemble"] = ensemble_bench(args.ensemble, args.ensemble_steps, args.seed)
     if rank == 0 and args.serving_rate > 0:

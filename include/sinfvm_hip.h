@@ -190,6 +190,17 @@ int sfv_device_count(void);
  * All give bitwise identical results.  No counterpart in the reference. */
 int sfv_set_fused(sfv_handle* h, int32_t fused);
 
+/* Rows per wave segment of the march kernels (sfv_set_fused(h, 2)).  0 (the
+ * default) lets the library choose on the first step: candidate heights are
+ * timed on the handle's own state and the fastest is kept for grids of that
+ * size class (quarter octaves of nx and ny) on that device; the environment
+ * variable SFV_AUTOTUNE=0 turns the measurement off (a build default is used).
+ * rows > 0 forces a height.  Results are bitwise the same for every height.
+ * No counterpart in the reference. */
+int sfv_set_march_rows(sfv_handle* h, int32_t rows);
+/* The height in use (0 while not yet chosen). */
+int32_t sfv_get_march_rows(const sfv_handle* h);
+
 /* ------------------------------------------------------------------ adjoint
  * Reverse-mode gradient of J = sum over interior cells of (h_T - h_target)^2,
  * h = w - B_cell, where h_T is the state after nsteps calls of
@@ -269,8 +280,10 @@ typedef struct sfv_server_stats {
     int64_t bytes_in_use, bytes_peak, budget_bytes;
     int64_t cell_updates;  /* of completed requests */
     int64_t replays;       /* graph replays (8 steps of every running request each) */
-    double busy_s;         /* time spent in replays */
+    double busy_s;         /* time with a replay in flight */
     double uptime_s;
+    double host_admit_s;   /* worker time creating admitted requests' handles and uploading their inputs */
+    double host_retire_s;  /* worker time downloading finished requests' results */
 } sfv_server_stats;
 
 /* Start a server on the current device with an HBM byte budget. */

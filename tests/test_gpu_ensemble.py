@@ -108,3 +108,18 @@ def test_ensemble_rejects_unfusable_members():
     sc2 = terrain(40, 40, seed=1, stepper=VF.ForwardEulerStepper())
     with pytest.raises(VF.ArgumentError):
         VF.Ensemble([make_hip(sc2)])
+
+
+def test_members_with_different_segment_heights():
+    """Each member keeps its own march segment height inside the batch."""
+    scs = [terrain(70, 45, seed=41), terrain(33, 90, seed=42), terrain(128, 20, seed=43, bc=VF.PeriodicBC())]
+    sims = [make_hip(sc) for sc in scs]
+    for sim, rows in zip(sims, (1, 3, 17)):
+        sim.set_march_rows(rows)
+    ens = VF.Ensemble(sims)
+    ens.advance_steps(10)
+    for sc, sim in zip(scs, sims):
+        solo = _solo(sc, 10)
+        assert sim.t == solo.t, sc.name
+        assert np.array_equal(sim.get_full_state(), solo.get_full_state()), sc.name
+    ens.close()

@@ -7,7 +7,7 @@ import pytest
 
 import volumefluxes as VF
 
-from helpers import dam_break, max_rel_diff, run_both_compare, terrain
+from helpers import dam_break, make_hip, make_oracle, max_rel_diff, run_both_compare, terrain
 
 pytestmark = pytest.mark.gpu
 
@@ -83,3 +83,37 @@ def test_terrain_f32_storage(kernels):
 @pytest.mark.parametrize("kernels", KERNELS)
 def test_terrain_256_20_steps(kernels):
     _assert_bitwise(terrain(256, 256, seed=21), 20, kernels=kernels)
+
+
+@pytest.mark.parametrize("rows", [1, 2, 5, 12, 40, 1024])
+def test_segment_height_keeps_results_bitwise(rows):
+    """The march kernels' segment height (autotuned or forced) changes which
+    wave evaluates what, never a result."""
+    sc = terrain(125, 37, seed=23)
+    print(f"[parity] rows={rows}", flush=True)
+    o = make_oracle(sc)
+    sim = make_hip(sc, "march")
+    sim.set_march_rows(rows)
+    t = 0.0
+    for k in range(8):
+        dr = o.step(float("inf"), t)
+        dh = VF.perform_step(sim, float("inf"))
+        assert dh == dr, (k, dh, dr)
+        t += dr
+        sim.t = t
+        assert np.array_equal(sim.get_full_state(), o.state()), (rows, k)
+    assert sim.march_rows == rows
+
+
+def test_autotuned_height_is_chosen_once_per_size_class():
+    a = make_hip(terrain(90, 60, seed=29), "march")
+    assert a.march_rows == 0
+    VF.perform_step(a, float("inf"))
+    r = a.march_rows
+    assert 1 <= r <= 60
+    b = make_hip(terrain(92, 61, seed=30), "march")       # same quarter-octave class
+    VF.perform_step(b, float("inf"))
+    assert b.march_rows == r
+    sc = terrain(90, 60, seed=29)
+    dts, U_hip, U_ref = run_both_compare(sc, 6, kernels="march")
+    assert all(dh == dr for dh, dr in dts) and np.array_equal(U_hip, U_ref)

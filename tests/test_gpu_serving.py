@@ -56,7 +56,9 @@ def test_budget_limits_what_runs_at_once():
     from volumefluxes import _capi as C
     import ctypes
     sizes = [C.lib().sfv_device_bytes_for(ctypes.byref(c)) for c in per]
-    budget = max(sizes) + min(sizes)          # at most about two at a time
+    # at most about two at a time; the server holds each buffer rounded up to
+    # its pool size class (at most 1/8 more, plus 4 KiB)
+    budget = int(1.25 * max(sizes)) + min(sizes)
     with VF.Server(budget) as sv:
         ids = []
         for sc, te in reqs:
@@ -99,3 +101,20 @@ def test_maximum_timestep_and_late_arrivals():
     sa, sb = _alone(a, 3.0, maximum_timestep=0.02), _alone(b, 0.5)
     assert np.array_equal(Ua, _interior(sa)) and ia["t"] == 3.0
     assert np.array_equal(Ub, _interior(sb)) and ib["t"] == 0.5
+
+
+def test_recycled_device_buffers_give_the_same_result():
+    """A finished request's buffers go back to the server's pool and the next
+    request of the same size takes them: its result must not see the old data."""
+    a = terrain(120, 70, seed=31)
+    b = dam_break(120, 70, hr=0.0)
+    with VF.Server(2 << 30) as sv:
+        outs = []
+        for sc, te in [(a, 1.5), (b, 1.0), (a, 0.5)]:
+            rid = sv.submit(sc.system, sc.grid, sc.U0, te, cfl=sc.cfl)
+            assert sv.wait(rid, timeout=60)["status"] == 2
+            outs.append(sv.fetch(rid)[0])
+        st = sv.stats()
+    assert st["host_admit_s"] >= 0 and st["host_retire_s"] >= 0 and st["busy_s"] > 0
+    for (sc, te), U in zip([(a, 1.5), (b, 1.0), (a, 0.5)], outs):
+        assert np.array_equal(U, _interior(_alone(sc, te))), sc.name

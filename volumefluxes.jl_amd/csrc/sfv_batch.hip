@@ -120,6 +120,12 @@ int BatchEngine::add(sfv_handle* h, const RunParams& rp, int* slot_out, std::str
         if (rc) return rc;
     }
     ETRY(hipStreamSynchronize(h->stream));   // pending set_state / topography uploads
+    if (tune_) {
+        const int rt = tune_march_rows(h, why);
+        if (rt) return rt;
+    } else {
+        (void)cached_march_rows(h);
+    }
     BatchMember M;
     std::memset(&M, 0, sizeof(M));
     M.P = h->P;
@@ -130,7 +136,7 @@ int BatchEngine::add(sfv_handle* h, const RunParams& rp, int* slot_out, std::str
     M.B = h->B;
     M.cur0 = (int)(h->cur & 1);
     M.slot0 = (int)(h->step_parity & 1);
-    M.my = march_rows_per_wave(h->P, 1);
+    M.my = h->P.rows > 0 ? h->P.rows : march_rows_per_wave(h->P, 1);
     const int segs = (h->P.ny + M.my - 1) / M.my;
     M.strips = (h->P.nx + MARCH_COLS - 1) / MARCH_COLS;
     M.seggroups = (segs + MARCH_WAVES - 1) / MARCH_WAVES;
@@ -175,7 +181,45 @@ bool BatchEngine::finished(int slot) const {
     return h_ctl_[slot].slot[slot_row_[slot].slot0].active == 0;
 }
 
+// Segment heights of the members for the current membership.  When the
+// batch fills the device's resident wave slots at the build default height,
+// every member takes it (fewest halo instructions).  Otherwise the device is
+// underfilled and the step time is a wave's serial path: each member takes
+// its own measured height (short segments for small grids), raised to the
+// tallest height at which the batch still fills the slots.  Heights set with
+// sfv_set_march_rows are kept.  No height changes any result.
+void BatchEngine::choose_rows() {
+    const int R0 = SFV_BATCH_DEFAULT_ROWS;
+    const long slots = (long)blocks_stage_ * MARCH_WAVES;
+    auto waves = [&](int r) {
+        long w = 0;
+        for (int s : rows_) {
+            const sfv_handle* h = slot_handle_[s];
+            const int rr = h->rows_forced ? h->march_rows : std::min(r, h->P.ny);
+            w += (long)((h->P.nx + MARCH_COLS - 1) / MARCH_COLS) * ((h->P.ny + rr - 1) / rr);
+        }
+        return w;
+    };
+    const bool filled = waves(R0) >= slots;
+    int r_fill = 1;
+    if (!filled)
+        for (int r : {8, 6, 4, 3, 2})
+            if (waves(r) >= slots) { r_fill = r; break; }
+    for (int s : rows_) {
+        const sfv_handle* h = slot_handle_[s];
+        BatchMember& M = slot_row_[s];
+        int my = R0;
+        if (h->rows_forced) my = h->march_rows;
+        else if (!filled) my = std::max(h->march_rows > 0 ? h->march_rows : 1, r_fill);
+        M.my = my;
+        M.P.rows = my;
+        const int segs = (h->P.ny + my - 1) / my;
+        M.seggroups = (segs + MARCH_WAVES - 1) / MARCH_WAVES;
+    }
+}
+
 int BatchEngine::upload_table(std::string& why) {
+    choose_rows();
     int tiles = 0, lines = 0;
     for (size_t r = 0; r < rows_.size(); ++r) {
         BatchMember& M = slot_row_[rows_[r]];
@@ -231,15 +275,25 @@ int BatchEngine::capture(std::string& why) {
     return SFV_OK;
 }
 
-int BatchEngine::replay(int k, std::string& why) {
+int BatchEngine::replay_begin(int k, std::string& why) {
     int rc;
     if (table_dirty_ && (rc = upload_table(why))) return rc;
     if (!graph_ && (rc = capture(why))) return rc;
     for (int r = 0; r < k; ++r) ETRY(hipGraphLaunch(graph_, stream_));
     ETRY(hipMemcpyAsync(h_ctl_, d_ctl_, sizeof(StepCtl) * capacity_, hipMemcpyDeviceToHost, stream_));
-    ETRY(hipStreamSynchronize(stream_));
-    for (int s : rows_) ran_[s] = 1;
     return SFV_OK;
+}
+
+int BatchEngine::replay_end(std::string& why) {
+    ETRY(hipStreamSynchronize(stream_));
+    for (int s : rows_) ran_[s] = 1;      // membership cannot change between begin and end
+    return SFV_OK;
+}
+
+int BatchEngine::replay(int k, std::string& why) {
+    const int rc = replay_begin(k, why);
+    if (rc) return rc;
+    return replay_end(why);
 }
 
 #undef ETRY

@@ -7,7 +7,10 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <string>
 #include <type_traits>
 #include <vector>
@@ -149,10 +152,152 @@ int ensure_ready(sfv_handle* h) {
     return SFV_OK;
 }
 
+// ---- autotuning of the march kernels' segment height
+//
+// A wave marches `rows` output rows plus 4 halo rows.  Tall segments waste
+// fewer instructions on halo rows, short ones give more waves and a shorter
+// serial path per wave, which is what a small grid needs: the best height
+// depends on the grid and the device, so it is measured.  Candidates are
+// timed on the handle itself (full steps from its current state into the
+// spare ping-pong buffer, which the next step overwrites anyway), and the
+// winner is cached per device, precision and grid size class (quarter
+// octaves of nx and ny).  Every height gives bitwise the same results.
+// SFV_AUTOTUNE=0 in the environment keeps the build default.
+struct TuneKey {
+    int dev, prec, bx, by;
+    bool operator<(const TuneKey& o) const {
+        return dev != o.dev ? dev < o.dev : prec != o.prec ? prec < o.prec : bx != o.bx ? bx < o.bx : by < o.by;
+    }
+};
+std::mutex g_tune_mu;
+std::map<TuneKey, int> g_tuned;
+
+int size_bucket(int n) { return (int)std::lround(4.0 * std::log2((double)std::max(n, 1))); }
+
+bool autotune_enabled() {
+    const char* e = std::getenv("SFV_AUTOTUNE");
+    return !(e && e[0] == '0');
+}
+
+template <class S>
+hipError_t tune_steps(sfv_handle* h, int slot, int n) {
+    const KParams& P = h->P;
+    const int cur = (int)(h->cur & 1);
+    const S* U0 = (const S*)h->U[cur];
+    S* R = (S*)h->RHS;
+    S* Us = (S*)h->U[cur ^ 1];
+    hipError_t e = hipSuccess;
+    for (int k = 0; k < n && e == hipSuccess; ++k) {
+        e = launch_march<S>(h->stream, P, h->ctl, slot, 1, U0, nullptr, R, h->B);
+        if (e == hipSuccess) e = launch_march<S>(h->stream, P, h->ctl, slot, 2, U0, R, Us, h->B);
+        if (e == hipSuccess) e = launch_bc<S>(h->stream, P, h->ctl, slot, Us, BC_IN_STEP);
+    }
+    return e;
+}
+
+}  // namespace
+
+namespace sfv {
+int cached_march_rows(sfv_handle* h) {
+    if (h->march_rows > 0) return h->march_rows;
+    const TuneKey key{h->device, h->cfg.precision, size_bucket(h->cfg.nx), size_bucket(h->cfg.ny)};
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_tuned.find(key);
+    if (it == g_tuned.end()) return 0;
+    h->march_rows = it->second;
+    h->P.rows = it->second;
+    return it->second;
+}
+
+int tune_march_rows(sfv_handle* h, std::string& why) {
+    if (h->fused != KV_MARCH || h->march_rows > 0) return SFV_OK;
+    const int fallback = march_rows_per_wave(h->P, 1);
+    if (!autotune_enabled()) {
+        h->march_rows = fallback;
+        h->P.rows = fallback;
+        return SFV_OK;
+    }
+    if (cached_march_rows(h) > 0) return SFV_OK;
+    const TuneKey key{h->device, h->cfg.precision, size_bucket(h->cfg.nx), size_bucket(h->cfg.ny)};
+    std::vector<int> cand;
+    for (int c : {1, 2, 3, 4, 6, 8, 12, 16, 24})
+        if (c <= h->cfg.ny) cand.push_back(c);
+    if (cand.empty() || cand.back() < std::min(h->cfg.ny, 24)) cand.push_back(std::min(h->cfg.ny, 24));
+    const int slot = (int)(h->step_parity & 1);
+    const bool f64 = h->cfg.precision == SFV_F64;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    auto bail = [&](hipError_t e) {
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+        h->P.rows = h->march_rows;
+        why = std::string("autotuning the march kernels: ") + hipGetErrorString(e);
+        return SFV_ERR_HIP;
+    };
+    hipError_t e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    if (e == hipSuccess)
+        e = launch_prepare(h->stream, h->P, h->ctl, slot, 0.0, NAN, 0, 0, 0.0, -1, 0, h->cfg.precision);
+    // time n steps with `rows`; the first call also sizes n so one
+    // measurement covers about a millisecond
+    auto timed = [&](int rows, int n, double& ms_out) -> hipError_t {
+        h->P.rows = rows;
+        hipError_t r = f64 ? tune_steps<double>(h, slot, 1) : tune_steps<float>(h, slot, 1);     // warm-up
+        if (r == hipSuccess) r = hipEventRecord(e0, h->stream);
+        if (r == hipSuccess) r = f64 ? tune_steps<double>(h, slot, n) : tune_steps<float>(h, slot, n);
+        if (r == hipSuccess) r = hipEventRecord(e1, h->stream);
+        if (r == hipSuccess) r = hipEventSynchronize(e1);
+        float ms = 0;
+        if (r == hipSuccess) r = hipEventElapsedTime(&ms, e0, e1);
+        ms_out = ms / n;
+        return r;
+    };
+    double probe = 0;
+    if (e == hipSuccess) e = timed(fallback, 2, probe);
+    const int n = (int)std::min(32.0, std::max(3.0, std::ceil(1.0 / std::max(probe, 1e-3))));
+    std::vector<double> best(cand.size(), 1e30);
+    for (int sweep = 0; sweep < 2 && e == hipSuccess; ++sweep)
+        for (size_t c = 0; c < cand.size() && e == hipSuccess; ++c) {
+            double ms = 0;
+            e = timed(cand[c], n, ms);
+            if (e == hipSuccess) best[c] = std::min(best[c], ms);
+        }
+    if (e != hipSuccess) return bail(e);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    // the fastest, unless the build default is within 3% of it: the timing
+    // starts from the current state (often a state at rest), where heights
+    // close in time can order differently than in the developed flow, and the
+    // default is the measured best for grids that fill the device
+    size_t fast = 0;
+    for (size_t c = 1; c < cand.size(); ++c)
+        if (best[c] < best[fast]) fast = c;
+    int pick = cand[fast];
+    for (size_t c = 0; c < cand.size(); ++c)
+        if (cand[c] == fallback && best[c] <= 1.03 * best[fast]) pick = fallback;
+    h->march_rows = pick;
+    h->P.rows = pick;
+    if (const char* dbg = std::getenv("SFV_TUNE_REPORT"); dbg && dbg[0] == '1') {
+        std::fprintf(stderr, "[sfv autotune] %dx%d %s:", h->cfg.nx, h->cfg.ny, f64 ? "f64" : "f32");
+        for (size_t c = 0; c < cand.size(); ++c) std::fprintf(stderr, " %d:%.4f", cand[c], best[c]);
+        std::fprintf(stderr, " -> %d (ms per step, %d steps per timing)\n", pick, n);
+    }
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tuned.emplace(key, pick);
+    return SFV_OK;
+}
+}  // namespace sfv
+
+namespace {
+
 // Run a prepared step loop of at most `nsteps` launched steps, in batches, and
 // finish when the device says the loop ended.
 int run_loop(sfv_handle* h, double t0, double endtime, int match_endtime, int has_max, double max_ts,
              int64_t max_steps, int until_endtime, int64_t* steps_out, double* t_out) {
+    {
+        std::string why;
+        const int rt = sfv::tune_march_rows(h, why);
+        if (rt) return fail(h, rt, why);
+    }
     const int slot0 = (int)(h->step_parity & 1);
     HIP_TRY(h, launch_prepare(h->stream, h->P, h->ctl, slot0, t0, endtime, match_endtime, has_max, max_ts,
                               (long long)max_steps, until_endtime, h->cfg.precision));
@@ -196,6 +341,23 @@ void sfv_handle::ghost_maps_into(int bc, int gc, int N, int* lo, int* hi, int* f
     ghost_maps(bc, gc, N, lo, hi, flo, fhi);
 }
 
+namespace sfv {
+int create_handle(const sfv_config* cfg, DevicePool* pool, sfv_handle** out, std::string& why) {
+    *out = nullptr;
+    const int vr = validate(cfg, why);
+    if (vr) return vr;
+    sfv_handle* h = new sfv_handle();
+    h->pool = pool;
+    const int rc = h->init(*cfg, why);
+    if (rc) {
+        delete h;
+        return rc;
+    }
+    *out = h;
+    return SFV_OK;
+}
+}  // namespace sfv
+
 extern "C" {
 
 void sfv_config_defaults(sfv_config* c, int32_t nx, int32_t ny, double dx, double dy) {
@@ -237,18 +399,9 @@ int64_t sfv_device_bytes_for(const sfv_config* c) {
 
 int sfv_create(const sfv_config* cfg, sfv_handle** out) {
     if (!out) return fail(nullptr, SFV_ERR_ARGUMENT, "out is NULL");
-    *out = nullptr;
     std::string why;
-    const int vr = validate(cfg, why);
-    if (vr) return fail(nullptr, vr, why);
-    sfv_handle* h = new sfv_handle();
-    const int rc = h->init(*cfg, why);
-    if (rc) {
-        g_create_error = why;
-        delete h;
-        return rc;
-    }
-    *out = h;
+    const int rc = sfv::create_handle(cfg, nullptr, out, why);
+    if (rc) return fail(nullptr, rc, why);
     return SFV_OK;
 }
 
@@ -295,12 +448,10 @@ int sfv_set_state(sfv_handle* h, const void* U, int32_t layout) {
     if (layout == SFV_LAYOUT_FULL) {
         HIP_TRY(h, hipMemcpyAsync(dst, U, 3 * h->ncell * es, hipMemcpyHostToDevice, h->stream));
     } else if (layout == SFV_LAYOUT_INTERIOR) {
-        for (int v = 0; v < 3; ++v) {
-            char* d = (char*)dst + ((size_t)v * h->ncell + (size_t)P.gc * P.nxT + P.gc) * es;
-            const char* s = (const char*)U + (size_t)v * P.nx * P.ny * es;
-            HIP_TRY(h, hipMemcpy2DAsync(d, (size_t)P.nxT * es, s, (size_t)P.nx * es, (size_t)P.nx * es, P.ny,
-                                        hipMemcpyHostToDevice, h->stream));
-        }
+        // one contiguous upload into the right-hand-side buffer (dead between
+        // steps), then a device scatter into the padded state
+        HIP_TRY(h, hipMemcpyAsync(h->RHS, U, 3 * (size_t)P.nx * P.ny * es, hipMemcpyHostToDevice, h->stream));
+        HIP_TRY(h, launch_interior_copy(h->stream, P, h->RHS, dst, es, 1));
         const int rc = h->cfg.precision == SFV_F64
                            ? (int)launch_bc<double>(h->stream, P, h->ctl, 0, (double*)dst, BC_STANDALONE)
                            : (int)launch_bc<float>(h->stream, P, h->ctl, 0, (float*)dst, BC_STANDALONE);
@@ -322,12 +473,9 @@ int sfv_get_state(sfv_handle* h, void* U, int32_t layout) {
     if (layout == SFV_LAYOUT_FULL) {
         HIP_TRY(h, hipMemcpyAsync(U, src, 3 * h->ncell * es, hipMemcpyDeviceToHost, h->stream));
     } else if (layout == SFV_LAYOUT_INTERIOR) {
-        for (int v = 0; v < 3; ++v) {
-            const char* s = (const char*)src + ((size_t)v * h->ncell + (size_t)P.gc * P.nxT + P.gc) * es;
-            char* d = (char*)U + (size_t)v * P.nx * P.ny * es;
-            HIP_TRY(h, hipMemcpy2DAsync(d, (size_t)P.nx * es, s, (size_t)P.nxT * es, (size_t)P.nx * es, P.ny,
-                                        hipMemcpyDeviceToHost, h->stream));
-        }
+        // device gather into the right-hand-side buffer, one contiguous download
+        HIP_TRY(h, launch_interior_copy(h->stream, P, h->RHS, const_cast<void*>(src), es, 0));
+        HIP_TRY(h, hipMemcpyAsync(U, h->RHS, 3 * (size_t)P.nx * P.ny * es, hipMemcpyDeviceToHost, h->stream));
     } else {
         return fail(h, SFV_ERR_ARGUMENT, "unknown layout");
     }
@@ -397,7 +545,8 @@ int sfv_time_kernel(sfv_handle* h, const char* which, int32_t n, double* ms_out)
     HIP_TRY(h, hipSetDevice(h->device));
     {
         std::string why;
-        const int rc2 = h->ensure_scratch(why);
+        int rc2 = sfv::tune_march_rows(h, why);
+        if (!rc2) rc2 = h->ensure_scratch(why);
         if (rc2) return fail(h, rc2, why);
     }
     const int slot = (int)(h->step_parity & 1), cur = (int)(h->cur & 1);
@@ -468,5 +617,16 @@ int sfv_set_fused(sfv_handle* h, int32_t fused) {
     h->fused = fused;
     return SFV_OK;
 }
+
+int sfv_set_march_rows(sfv_handle* h, int32_t rows) {
+    if (!h) return SFV_ERR_ARGUMENT;
+    if (rows < 0 || rows > 1024) return fail(h, SFV_ERR_ARGUMENT, "rows must be in 0..1024");
+    h->march_rows = rows;
+    h->rows_forced = rows > 0;
+    h->P.rows = rows;
+    return SFV_OK;
+}
+
+int32_t sfv_get_march_rows(const sfv_handle* h) { return h ? h->march_rows : -1; }
 
 }  // extern "C"

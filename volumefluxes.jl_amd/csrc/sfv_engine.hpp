@@ -17,6 +17,9 @@
 
 namespace sfv {
 
+// Segment height of batch members when the batch fills the device.
+constexpr int SFV_BATCH_DEFAULT_ROWS = 12;
+
 struct RunParams {
     double t0 = 0.0;
     double endtime = 0.0;         // NaN / unused when until_endtime = 0
@@ -44,6 +47,17 @@ public:
     // Replay the graph `k` times (k * graph_steps launched steps), then copy
     // every slot's step control to the host and wait.
     int replay(int k, std::string& why);
+    // The same in two halves, so the host can work while the device steps:
+    // replay_begin queues the replays and the step-control copy; replay_end
+    // waits for them.  Between the two, add / remove / finished / ctl must
+    // not be called (they use the table and the step-control mirror).
+    int replay_begin(int k, std::string& why);
+    int replay_end(std::string& why);
+
+    // Measure the march segment height of members whose grid size class has
+    // none yet (on by default).  Off, such members take the fill rule of
+    // upload_table alone, so adding them never stalls the device.
+    void set_tuning(bool on) { tune_ = on; }
 
     int members() const { return (int)rows_.size(); }
     // After replay: has the member in `slot` finished its run?
@@ -56,6 +70,7 @@ public:
 private:
     int grow(int capacity, std::string& why);
     int upload_table(std::string& why);
+    void choose_rows();
     int capture(std::string& why);
 
     int precision_ = 0;
@@ -72,6 +87,7 @@ private:
     StepCtl* h_ctl_ = nullptr;
     hipGraphExec_t graph_ = nullptr;
     bool table_dirty_ = true;
+    bool tune_ = true;
     std::vector<int> rows_;                  // slot of each table row
     std::vector<sfv_handle*> slot_handle_;   // nullptr: free slot
     std::vector<BatchMember> slot_row_;      // the member's row (host copy)

@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "../../include/sinfvm_hip.h"
+#include "sfv_pool.hpp"
 #include "swe_kernels.hpp"
 
 // Kernel variants of an RK2 step (all give identical results).
@@ -40,15 +41,48 @@ struct sfv_handle {
     bool have_dt = false;
     double last_dt = 0.0;
     int fused = KV_MARCH;               // kernel variant, KV_*
+    int march_rows = 0;                 // rows per march segment: 0 = not chosen yet (autotuned on first use)
+    bool rows_forced = false;           // set by sfv_set_march_rows
     std::string err;
+    // Owner-supplied pool (server handles): device blocks (class-rounded),
+    // the stream and the pinned mirror come from it and go back to it.
+    sfv::DevicePool* pool = nullptr;
+    std::vector<std::pair<void*, size_t>> pooled;
 
     int alloc(void** p, size_t bytes, std::string& why) {
+        if (pool) {
+            const size_t cls = sfv::DevicePool::size_class(bytes);
+            *p = pool->get(cls);
+            if (!*p) {
+                why = "hipMalloc of " + std::to_string(cls) + " bytes failed";
+                return SFV_ERR_OUT_OF_MEMORY;
+            }
+            pooled.emplace_back(*p, cls);
+            device_bytes += (int64_t)cls;
+            return SFV_OK;
+        }
         if (hipMalloc(p, bytes) != hipSuccess) {
             why = "hipMalloc of " + std::to_string(bytes) + " bytes failed";
             return SFV_ERR_OUT_OF_MEMORY;
         }
         device_bytes += (int64_t)bytes;
         return SFV_OK;
+    }
+
+    // Device bytes init() allocates for a configuration (ctl block included;
+    // the per-pass scratch state U[2] is not: it is allocated on first use),
+    // each buffer rounded to the pool's size class when `pooled_classes`.
+    static int64_t init_bytes(const sfv_config& c, bool pooled_classes) {
+        const size_t nxT = (size_t)c.nx + 2 * (size_t)c.gc, nyT = (size_t)c.ny + 2 * (size_t)c.gc;
+        const size_t es = c.precision == SFV_F32_STORAGE ? 4 : 8, ncell = nxT * nyT;
+        std::vector<size_t> sz = {3 * ncell * es, 3 * ncell * es, 3 * ncell * es, (nxT + 1) * (nyT + 1) * 8,
+                                  sizeof(sfv::StepCtl)};
+        if (c.rain == SFV_RAIN_TIME_TABLE && c.rain_table_len > 0) sz.push_back(2 * (size_t)c.rain_table_len * 8);
+        if (c.rain == SFV_RAIN_FIELD) sz.push_back(ncell * 8);
+        if (c.infiltration == SFV_INFILTRATION_HORTON) sz.push_back(ncell * 8);
+        int64_t b = 0;
+        for (size_t x : sz) b += (int64_t)(pooled_classes ? sfv::DevicePool::size_class(x) : x);
+        return b;
     }
 
     int init(const sfv_config& c, std::string& why) {
@@ -68,18 +102,19 @@ struct sfv_handle {
         elem = c.precision == SFV_F32_STORAGE ? 4 : 8;
 
         if (hipGetDevice(&device) != hipSuccess) { why = "no HIP device"; return SFV_ERR_HIP; }
-        if (hipStreamCreateWithFlags(&stream, hipStreamNonBlocking) != hipSuccess) {
-            why = "hipStreamCreate failed"; return SFV_ERR_HIP;
-        }
+        if (pool) stream = pool->get_stream();
+        else if (hipStreamCreateWithFlags(&stream, hipStreamNonBlocking) != hipSuccess) stream = nullptr;
+        if (!stream) { why = "hipStreamCreate failed"; return SFV_ERR_HIP; }
         int rc;
         for (int s = 0; s < 2; ++s)
             if ((rc = alloc(&U[s], 3 * ncell * elem, why))) return rc;
         if ((rc = alloc(&RHS, 3 * ncell * elem, why))) return rc;
         if ((rc = alloc((void**)&B, (size_t)(nxT + 1) * (nyT + 1) * sizeof(double), why))) return rc;
         if ((rc = alloc((void**)&ctl, sizeof(sfv::StepCtl), why))) return rc;
-        if (hipHostMalloc((void**)&ctl_host, sizeof(sfv::StepCtl), hipHostMallocDefault) != hipSuccess) {
-            why = "hipHostMalloc failed"; return SFV_ERR_OUT_OF_MEMORY;
-        }
+        if (pool) ctl_host = (sfv::StepCtl*)pool->get_pinned(sizeof(sfv::StepCtl));
+        else if (hipHostMalloc((void**)&ctl_host, sizeof(sfv::StepCtl), hipHostMallocDefault) != hipSuccess)
+            ctl_host = nullptr;
+        if (!ctl_host) { why = "hipHostMalloc failed"; return SFV_ERR_OUT_OF_MEMORY; }
         std::memset(ctl_host, 0, sizeof(sfv::StepCtl));
         for (int s = 0; s < 2; ++s) (void)hipMemsetAsync(U[s], 0, 3 * ncell * elem, stream);
         (void)hipMemsetAsync(RHS, 0, 3 * ncell * elem, stream);
@@ -144,6 +179,7 @@ struct sfv_handle {
         p.bc = c.bc;
         ghost_maps_into(c.bc, c.gc, p.nxT, p.src_lo_x, p.src_hi_x, p.flip_lo_x, p.flip_hi_x);
         ghost_maps_into(c.bc, c.gc, p.nyT, p.src_lo_y, p.src_hi_y, p.flip_lo_y, p.flip_hi_y);
+        p.rows = march_rows;
     }
 
     static void ghost_maps_into(int bc, int gc, int N, int* lo, int* hi, int* flo, int* fhi);
@@ -164,6 +200,12 @@ struct sfv_handle {
 
     ~sfv_handle() {
         if (stream) (void)hipStreamSynchronize(stream);
+        if (pool) {
+            for (auto& b : pooled) pool->put(b.first, b.second);
+            if (ctl_host) pool->put_pinned(ctl_host);
+            if (stream) pool->put_stream(stream);
+            return;
+        }
         for (void* p : U) if (p) (void)hipFree(p);
         if (RHS) (void)hipFree(RHS);
         if (B) (void)hipFree(B);
@@ -175,3 +217,13 @@ struct sfv_handle {
         if (stream) (void)hipStreamDestroy(stream);
     }
 };
+
+namespace sfv {
+// sfv_create with an optional owner pool (nullptr: plain allocations).
+int create_handle(const sfv_config* cfg, DevicePool* pool, sfv_handle** out, std::string& why);
+// Choose the march kernels' segment height for a handle (measured on the
+// handle's current state, cached per grid size class); a no-op once chosen.
+int tune_march_rows(sfv_handle* h, std::string& why);
+// The cached height for the handle's size class, without measuring (0: none yet).
+int cached_march_rows(sfv_handle* h);
+}  // namespace sfv

@@ -70,7 +70,8 @@ struct sfv_server {
     int64_t next_id = 1;
     int64_t submitted = 0, admitted = 0, completed = 0, failed = 0, replays = 0;
     int64_t cell_updates = 0;
-    double busy_s = 0;
+    double busy_s = 0, host_admit_s = 0, host_retire_s = 0;
+    sfv::DevicePool pool;              // worker-owned: device blocks, streams, pinned mirrors of finished requests
     std::chrono::steady_clock::time_point start = std::chrono::steady_clock::now();
 
     double now() const {
@@ -109,31 +110,20 @@ void fill_info(const Request& r, sfv_request_info* info) {
     info->done_s = r.done_s;
 }
 
-// Worker: create the request's handle and hand it to the engine.
-int start_request(sfv::BatchEngine& eng, Request* r, std::string& why) {
+// Worker: create the request's handle (buffers from the server's pool) and
+// upload its inputs.  The engine takes it later, between replays.
+int create_request(sfv::DevicePool* pool, Request* r, std::string& why) {
     sfv_config c = r->cfg;
     c.rain_table_rates = r->rain_rates.empty() ? nullptr : r->rain_rates.data();
     c.rain_table_times = r->rain_times.empty() ? nullptr : r->rain_times.data();
     c.infiltration_factor = r->factor.empty() ? nullptr : r->factor.data();
     sfv_handle* h = nullptr;
-    int rc = sfv_create(&c, &h);
-    if (rc) { why = sfv_create_error(); return rc; }
+    int rc = sfv::create_handle(&c, pool, &h, why);
+    if (rc) return rc;
     rc = sfv_set_topography(h, r->B.data(), r->nbx, r->nby);
     if (!rc) rc = sfv_set_state(h, r->U0.data(), SFV_LAYOUT_INTERIOR);
     if (rc) {
         why = sfv_last_error(h);
-        sfv_destroy(h);
-        return rc;
-    }
-    sfv::RunParams rp;
-    rp.t0 = r->t0;
-    rp.until_endtime = 1;
-    rp.endtime = r->t_end;
-    rp.match_endtime = r->match_endtime;
-    rp.maximum_timestep = r->maximum_timestep;
-    rp.max_steps = 0;
-    rc = eng.add(h, rp, &r->slot, why);
-    if (rc) {
         sfv_destroy(h);
         return rc;
     }
@@ -143,13 +133,29 @@ int start_request(sfv::BatchEngine& eng, Request* r, std::string& why) {
     return SFV_OK;
 }
 
+sfv::RunParams run_params(const Request* r) {
+    sfv::RunParams rp;
+    rp.t0 = r->t0;
+    rp.until_endtime = 1;
+    rp.endtime = r->t_end;
+    rp.match_endtime = r->match_endtime;
+    rp.maximum_timestep = r->maximum_timestep;
+    rp.max_steps = 0;
+    return rp;
+}
+
 }  // namespace
 
+// The worker overlaps host work with the device: while one replay of the
+// batched graph runs, it downloads the results of the requests that finished
+// in the previous replay and creates (and uploads) the newly admitted ones;
+// membership changes happen between replays.
 void sfv_server::work() {
     std::string why;
     sfv::BatchEngine eng;
     int rc = hipSetDevice(device) == hipSuccess ? SFV_OK : SFV_ERR_HIP;
     if (!rc) rc = eng.init(precision, 64, why);
+    eng.set_tuning(false);             // a new size class must not stall the other requests
     if (rc) {
         std::lock_guard<std::mutex> lk(mu);
         worker_failed = true;
@@ -169,13 +175,18 @@ void sfv_server::work() {
         if (status == REQ_DONE) { ++completed; cell_updates += r->steps * (int64_t)r->cfg.nx * r->cfg.ny; }
         else ++failed;
     };
+    std::vector<Request*> to_retire;   // finished, removed from the engine, result not yet downloaded
+    std::vector<Request*> created;     // handle made, not yet in the engine
+    bool in_flight = false;
+    double t_begin = 0;
     for (;;) {
+        // 1. admissions, in arrival order while the next request fits the budget
         std::vector<Request*> admit;
         {
             std::unique_lock<std::mutex> lk(mu);
-            cv_work.wait(lk, [&] { return stop || !queue.empty() || !running.empty(); });
+            if (!in_flight && to_retire.empty() && created.empty())
+                cv_work.wait(lk, [&] { return stop || !queue.empty() || !running.empty(); });
             if (stop) break;
-            // admission in arrival order while the next request fits the budget
             while (!queue.empty()) {
                 Request* r = queue.front();
                 if (r->bytes > budget) {
@@ -199,62 +210,116 @@ void sfv_server::work() {
                 admit.push_back(r);
             }
         }
-        for (Request* r : admit) {
-            const int rc2 = start_request(eng, r, why);
-            if (rc2) {
+        // 2. host work, overlapping the replay in flight: retire, then create
+        if (!to_retire.empty()) {
+            const double t0 = now();
+            for (Request* r : to_retire) {
+                r->result.resize((size_t)r->cfg.nx * r->cfg.ny * 3 * elem_of(r->cfg));
+                const int rc4 = sfv_get_state(r->h, r->result.data(), SFV_LAYOUT_INTERIOR);
+                const std::string msg4 = rc4 ? sfv_last_error(r->h) : "";
+                sfv_destroy(r->h);                   // buffers go back to the pool
+                r->h = nullptr;
                 std::lock_guard<std::mutex> lk(mu);
-                finish(r, REQ_FAILED, rc2, why);
+                if (rc4) finish(r, REQ_FAILED, rc4, msg4);
+                else if (r->error == SFV_ERR_NONFINITE_DT)
+                    finish(r, REQ_DONE, SFV_ERR_NONFINITE_DT, "a time step produced a non-finite dt");
+                else finish(r, REQ_DONE, SFV_OK, "");
+            }
+            to_retire.clear();
+            std::lock_guard<std::mutex> lk(mu);
+            host_retire_s += now() - t0;
+            cv_done.notify_all();
+        }
+        if (!admit.empty()) {
+            const double t0 = now();
+            for (Request* r : admit) {
+                int64_t held;
+                {
+                    std::lock_guard<std::mutex> lk(mu);
+                    held = in_use;
+                }
+                // cached blocks and the bytes of admitted requests stay within the budget
+                if ((int64_t)pool.cached() + held > budget) pool.trim((size_t)std::max<int64_t>(0, budget - held));
+                const int rc2 = create_request(&pool, r, why);
+                if (rc2) {
+                    std::lock_guard<std::mutex> lk(mu);
+                    finish(r, REQ_FAILED, rc2, why);
+                    cv_done.notify_all();
+                } else {
+                    created.push_back(r);
+                }
+            }
+            std::lock_guard<std::mutex> lk(mu);
+            host_admit_s += now() - t0;
+        }
+        // 3. end the replay in flight; finished requests leave the engine
+        if (in_flight) {
+            in_flight = false;
+            const int rc3 = eng.replay_end(why);
+            const double t_end = now();
+            if (rc3) {
+                std::lock_guard<std::mutex> lk(mu);
+                for (Request* r : running) {
+                    eng.remove(r->slot);
+                    sfv_destroy(r->h);
+                    r->h = nullptr;
+                    finish(r, REQ_FAILED, rc3, why);
+                }
+                running.clear();
+                cv_done.notify_all();
+            } else {
+                std::vector<Request*> still;
+                for (Request* r : running) {
+                    if (!eng.finished(r->slot)) { still.push_back(r); continue; }
+                    const sfv::StepCtl& c = eng.ctl(r->slot);
+                    r->t = c.t;
+                    r->steps = c.steps;
+                    r->error = c.nonfinite != 0 ? SFV_ERR_NONFINITE_DT : SFV_OK;
+                    eng.remove(r->slot);
+                    to_retire.push_back(r);
+                }
+                running.swap(still);
+                std::lock_guard<std::mutex> lk(mu);
+                ++replays;
+                busy_s += t_end - t_begin;
+            }
+        }
+        // 4. the created requests join
+        for (Request* r : created) {
+            const int rc5 = eng.add(r->h, run_params(r), &r->slot, why);
+            if (rc5) {
+                sfv_destroy(r->h);
+                r->h = nullptr;
+                std::lock_guard<std::mutex> lk(mu);
+                finish(r, REQ_FAILED, rc5, why);
                 cv_done.notify_all();
             } else {
                 running.push_back(r);
             }
         }
-        if (running.empty()) continue;
-        const double t_a = now();
-        const int rc3 = eng.replay(1, why);
-        const double t_b = now();
-        std::vector<Request*> still;
-        std::vector<Request*> done;
-        if (rc3) {
-            std::lock_guard<std::mutex> lk(mu);
-            for (Request* r : running) {
-                eng.remove(r->slot);
-                sfv_destroy(r->h);
-                r->h = nullptr;
-                finish(r, REQ_FAILED, rc3, why);
+        created.clear();
+        // 5. the next replay
+        if (!running.empty()) {
+            t_begin = now();
+            const int rc6 = eng.replay_begin(1, why);
+            if (rc6) {
+                (void)eng.replay_end(why);
+                std::lock_guard<std::mutex> lk(mu);
+                for (Request* r : running) {
+                    eng.remove(r->slot);
+                    sfv_destroy(r->h);
+                    r->h = nullptr;
+                    finish(r, REQ_FAILED, rc6, why);
+                }
+                running.clear();
+                cv_done.notify_all();
+            } else {
+                in_flight = true;
             }
-            running.clear();
-            cv_done.notify_all();
-            continue;
         }
-        for (Request* r : running) (eng.finished(r->slot) ? done : still).push_back(r);
-        for (Request* r : done) {
-            const sfv::StepCtl& c = eng.ctl(r->slot);
-            const double t_end = c.t;
-            const int64_t steps = c.steps;
-            const bool nonfinite = c.nonfinite != 0;
-            eng.remove(r->slot);
-            r->result.resize((size_t)r->cfg.nx * r->cfg.ny * 3 * elem_of(r->cfg));
-            const int rc4 = sfv_get_state(r->h, r->result.data(), SFV_LAYOUT_INTERIOR);
-            const std::string msg4 = rc4 ? sfv_last_error(r->h) : "";
-            sfv_destroy(r->h);
-            r->h = nullptr;
-            std::lock_guard<std::mutex> lk(mu);
-            r->t = t_end;
-            r->steps = steps;
-            if (rc4) finish(r, REQ_FAILED, rc4, msg4);
-            else if (nonfinite) finish(r, REQ_DONE, SFV_ERR_NONFINITE_DT, "a time step produced a non-finite dt");
-            else finish(r, REQ_DONE, SFV_OK, "");
-        }
-        {
-            std::lock_guard<std::mutex> lk(mu);
-            ++replays;
-            busy_s += t_b - t_a;
-            if (!done.empty()) cv_done.notify_all();
-        }
-        running.swap(still);
     }
     // stopped: release what is still on the device
+    if (in_flight) (void)eng.replay_end(why);
     std::lock_guard<std::mutex> lk(mu);
     for (Request* r : running) {
         eng.remove(r->slot);
@@ -263,6 +328,18 @@ void sfv_server::work() {
         finish(r, REQ_FAILED, SFV_ERR_STATE, "server stopped");
     }
     running.clear();
+    for (Request* r : to_retire) {
+        sfv_destroy(r->h);
+        r->h = nullptr;
+        finish(r, REQ_FAILED, SFV_ERR_STATE, "server stopped");
+    }
+    to_retire.clear();
+    for (Request* r : created) {
+        sfv_destroy(r->h);
+        r->h = nullptr;
+        finish(r, REQ_FAILED, SFV_ERR_STATE, "server stopped");
+    }
+    created.clear();
     for (Request* r : queue) {
         r->status = REQ_FAILED; r->error = SFV_ERR_STATE; r->message = "server stopped"; ++failed;
     }
@@ -343,7 +420,7 @@ int sfv_server_submit(sfv_server* s, const sfv_config* cfg, const double* B, int
     r->t_end = t_end;
     r->match_endtime = match_endtime ? 1 : 0;
     r->maximum_timestep = maximum_timestep;
-    r->bytes = sfv_device_bytes_for(cfg);
+    r->bytes = sfv_handle::init_bytes(*cfg, true);   // pooled blocks are class-rounded
     std::lock_guard<std::mutex> lk(s->mu);
     if (s->stop || s->worker_failed) {
         s->err = s->worker_failed ? s->worker_err : "server stopped";
@@ -436,6 +513,8 @@ int sfv_server_get_stats(sfv_server* s, sfv_server_stats* out) {
     out->replays = s->replays;
     out->busy_s = s->busy_s;
     out->uptime_s = s->now();
+    out->host_admit_s = s->host_admit_s;
+    out->host_retire_s = s->host_retire_s;
     return SFV_OK;
 }
 

@@ -25,6 +25,7 @@ struct KParams {
     int infil_kind; // 0 none, 1 constant, 2 horton
     int rain_first;
     int stepper;    // 1 FE, 2 RK2
+    int rows;       // rows per wave segment of the march kernels (0: the build default)
     int rain_n;
     const double* rain_rates;  // device (reference unit)
     const double* rain_times;  // device
@@ -32,14 +33,14 @@ struct KParams {
     double infil_rate, f0, fc, hk;
     const double* factor;      // device (nxT*nyT) Horton factor field
     const double* rain_field;  // device (nxT*nyT) FunctionalRain rates of the current step
+    int bcell_1d;   // B_cell = 0.5*(B[i,j] + B[i+1,j]) (a CartesianGrid{1} run embedded in (nx, 3))
+    int bc;
     // Ghost-cell source maps for the boundary condition, by side: low-side
     // ghost g is cell g, high-side ghost g is cell N-gc+g (N = nxT or nyT);
     // it takes the pre-BC value of cell src (same row/column), with the
     // normal momentum negated if flip is set.  Built on the host by
     // replaying bc.jl's per-ghost assignments in order, so order-dependent
     // maps (NeumannBC's high side, grids narrower than gc) are exact.
-    int bcell_1d;   // B_cell = 0.5*(B[i,j] + B[i+1,j]) (a CartesianGrid{1} run embedded in (nx, 3))
-    int bc;
     int src_lo_x[MAX_GC], src_hi_x[MAX_GC], src_lo_y[MAX_GC], src_hi_y[MAX_GC];
     int flip_lo_x[MAX_GC], flip_hi_x[MAX_GC], flip_lo_y[MAX_GC], flip_hi_y[MAX_GC];
 };

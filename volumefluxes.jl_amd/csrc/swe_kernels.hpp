@@ -60,6 +60,11 @@ constexpr int MARCH_WAVES = 4;   // waves (segments) per workgroup
 
 // Plain copy of `bytes` (a multiple of 4) from src to dst.
 hipError_t launch_copy(hipStream_t st, const void* src, void* dst, size_t bytes);
+// Interior cells between a contiguous (nx, ny, 3) array and the padded
+// (nxT, nyT, 3) state: to_state = 1 scatters packed -> state, 0 gathers
+// state -> packed.  es = element size (4 or 8).
+hipError_t launch_interior_copy(hipStream_t st, const KParams& P, const void* packed, void* state, size_t es,
+                                int to_state);
 template <class S>
 hipError_t launch_stage1(hipStream_t st, const KParams& P, StepCtl* ctl, int slot, const S* U0, S* RHS,
                          const double* B);

@@ -189,6 +189,35 @@ hipError_t launch_copy(hipStream_t st, const void* src, void* dst, size_t bytes)
     return hipPeekAtLastError();
 }
 
+template <class W>
+__global__ void __launch_bounds__(256) k_interior_copy(int nx, int ny, int nxT, int gc, size_t ncell, W* packed,
+                                                       W* state, int to_state) {
+    const int i = blockIdx.x * 256 + threadIdx.x, v = blockIdx.z;
+    if (i >= nx) return;
+    for (int j = blockIdx.y; j < ny; j += gridDim.y) {
+        const size_t a = (size_t)v * nx * ny + (size_t)j * nx + i;
+        const size_t b = (size_t)v * ncell + (size_t)(j + gc) * nxT + (i + gc);
+        if (to_state) state[b] = packed[a];
+        else packed[a] = state[b];
+    }
+}
+
+hipError_t launch_interior_copy(hipStream_t st, const KParams& P, const void* packed, void* state, size_t es,
+                                int to_state) {
+    if (P.nx <= 0 || P.ny <= 0) return hipSuccess;
+    const dim3 grid((unsigned)((P.nx + 255) / 256), (unsigned)(P.ny < 16384 ? P.ny : 16384), 3);
+    const size_t ncell = (size_t)P.nxT * P.nyT;
+    if (es == 8)
+        hipLaunchKernelGGL(k_interior_copy<unsigned long long>, grid, dim3(256), 0, st, P.nx, P.ny, P.nxT, P.gc,
+                           ncell, (unsigned long long*)packed, (unsigned long long*)state, to_state);
+    else if (es == 4)
+        hipLaunchKernelGGL(k_interior_copy<unsigned>, grid, dim3(256), 0, st, P.nx, P.ny, P.nxT, P.gc, ncell,
+                           (unsigned*)packed, (unsigned*)state, to_state);
+    else
+        return hipErrorInvalidValue;
+    return hipPeekAtLastError();
+}
+
 template hipError_t launch_bc<double>(hipStream_t, const KParams&, StepCtl*, int, double*, int);
 template hipError_t launch_bc<float>(hipStream_t, const KParams&, StepCtl*, int, float*, int);
 

@@ -514,7 +514,7 @@ int march_rows_per_wave(const KParams& P, int stage) {
 template <class S>
 hipError_t launch_march(hipStream_t st, const KParams& P, StepCtl* ctl, int slot, int stage, const S* U0,
                         const S* RHS1, S* OUT, const double* B) {
-    const int my = march_rows_per_wave(P, stage);
+    const int my = P.rows > 0 ? P.rows : march_rows_per_wave(P, stage);
     const int segs = (P.ny + my - 1) / my;
     const int strips = (P.nx + MX - 1) / MX, seggroups = (segs + MWAVES - 1) / MWAVES;
     const dim3 grid(strips * seggroups);

@@ -118,7 +118,7 @@ class ServerStats(ctypes.Structure):
 
     _fields_ = [(k, c_int64) for k in ("submitted", "admitted", "completed", "failed", "queued", "running",
                                        "bytes_in_use", "bytes_peak", "budget_bytes", "cell_updates", "replays")] + \
-               [("busy_s", c_double), ("uptime_s", c_double)]
+               [("busy_s", c_double), ("uptime_s", c_double), ("host_admit_s", c_double), ("host_retire_s", c_double)]
 
     def as_dict(self) -> dict:
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -190,6 +190,10 @@ def _declare(L) -> None:
     L.sfv_device_count.restype = c_int32
     L.sfv_set_fused.argtypes = [h, c_int32]
     L.sfv_set_fused.restype = c_int32
+    L.sfv_set_march_rows.argtypes = [h, c_int32]
+    L.sfv_set_march_rows.restype = c_int32
+    L.sfv_get_march_rows.argtypes = [h]
+    L.sfv_get_march_rows.restype = c_int32
     b = c_void_p
     L.sfv_batch_create.argtypes = [POINTER(c_void_p), c_int32, POINTER(c_void_p)]
     L.sfv_batch_create.restype = c_int32

@@ -142,6 +142,17 @@ class Simulator:
     def set_fused(self, fused: bool) -> None:
         self.set_kernels("march" if fused else "per-pass")
 
+    def set_march_rows(self, rows: int) -> None:
+        """Rows per wave segment of the march kernels: 0 (default) = measured
+        on the first step and cached per grid size class; > 0 forces it.
+        Results are bitwise the same for every value."""
+        C.check(self._h, C.lib().sfv_set_march_rows(self._h, int(rows)))
+
+    @property
+    def march_rows(self) -> int:
+        """The segment height in use (0 until the first step chooses it)."""
+        return int(C.lib().sfv_get_march_rows(self._h))
+
     def _step_status(self, rc: int) -> None:
         if rc == C.SFV_ERR_NONFINITE_DT:
             # the reference carries on with the NaN/Inf dt (simulator.jl:190-192)
