@@ -25,6 +25,7 @@
 // (ghost cells through the boundary-condition source maps), so the stage-1
 // state never touches HBM.
 #include <type_traits>
+#include <utility>
 
 #include "swe_kernels.hpp"
 
@@ -36,9 +37,11 @@ namespace sfv {
 //   SFV_MARCH_WETFAST     straight-line face evaluation for all-wet waves
 //   SFV_MARCH_ROWS        rows per wave segment (0: chosen per grid)
 //   SFV_MARCH_MIN_WAVES   minimum waves per SIMD the register allocation must allow
+//                         (SFV_MARCH_MIN_WAVES1: the same for stage 1 only)
 //   SFV_MARCH_XCD         block order: 1 = each XCD gets a contiguous run of tiles,
 //                         segments fastest (vertical halos stay in one L2);
 //                         2 = same, strips fastest; 0 = hardware order
+//   SFV_MARCH_UNROLL      unroll factor of the row loop
 #ifndef SFV_MARCH_PREFETCH
 #define SFV_MARCH_PREFETCH 0
 #endif
@@ -54,9 +57,21 @@ namespace sfv {
 #ifndef SFV_MARCH_MIN_WAVES
 #define SFV_MARCH_MIN_WAVES 1
 #endif
+#ifndef SFV_MARCH_MIN_WAVES1
+#define SFV_MARCH_MIN_WAVES1 SFV_MARCH_MIN_WAVES
+#endif
 #ifndef SFV_MARCH_XCD
 #define SFV_MARCH_XCD 2
 #endif
+#ifndef SFV_MARCH_UNROLL
+#define SFV_MARCH_UNROLL 1
+#endif
+
+
+template <class F, size_t... I>
+__device__ __forceinline__ void unrolled_rows(F& body, int r, std::index_sequence<I...>) {
+    (body(r + (int)I, std::true_type{}), ...);
+}
 
 constexpr int MW = 64;                // lanes
 constexpr int MX = MARCH_COLS;        // output columns per wave
@@ -365,8 +380,13 @@ __device__ __forceinline__ void march_tile(const KParams& P, StepCtl* ctl, int s
 
     const int rmain = min(4, th + 4);
     for (int r = 0; r < rmain; ++r) body(r, std::false_type{});
-#pragma unroll 2
-    for (int r = rmain; r < th + 4; ++r) body(r, std::true_type{});
+    // The row loop is unrolled by hand (its cross-lane shifts are convergent
+    // operations, which the compiler will not unroll with a runtime trip count).
+    int r = rmain;
+    if (SFV_MARCH_UNROLL > 1)
+        for (; r + SFV_MARCH_UNROLL <= th + 4; r += SFV_MARCH_UNROLL)
+            unrolled_rows(body, r, std::make_index_sequence<SFV_MARCH_UNROLL>{});
+    for (; r < th + 4; ++r) body(r, std::true_type{});
 
     if (STAGE == 1) {
         smx = wave_max_u64m(smx);
@@ -439,7 +459,8 @@ __device__ __forceinline__ int xcd_tile(int L, int N) {
 // 1D grid of strips x seggroups blocks; a block is 4 waves on 4 consecutive
 // segments of one strip.
 template <class S, int STAGE>
-__global__ void __launch_bounds__(MW* MWAVES) __attribute__((amdgpu_waves_per_eu(SFV_MARCH_MIN_WAVES)))
+__global__ void __launch_bounds__(MW* MWAVES)
+__attribute__((amdgpu_waves_per_eu(STAGE == 1 ? SFV_MARCH_MIN_WAVES1 : SFV_MARCH_MIN_WAVES)))
 k_march(KParams P, StepCtl* ctl, int slot, int my, int strips, int seggroups, const S* __restrict__ U0,
         const S* __restrict__ RHS1, S* __restrict__ OUT, const double* __restrict__ B) {
     const int t = xcd_tile(blockIdx.x, strips * seggroups);
