@@ -118,3 +118,23 @@ def test_recycled_device_buffers_give_the_same_result():
     assert st["host_admit_s"] >= 0 and st["host_retire_s"] >= 0 and st["busy_s"] > 0
     for (sc, te), U in zip([(a, 1.5), (b, 1.0), (a, 0.5)], outs):
         assert np.array_equal(U, _interior(_alone(sc, te))), sc.name
+
+
+def test_large_and_small_grids_served_together():
+    """Grids above 2^18 cells step in the large-grid engine, smaller ones in
+    the small-grid engine (replayed several times per large replay); both
+    give the results of the requests run alone."""
+    reqs = [(terrain(600, 500, seed=51), 0.6), (terrain(48, 40, seed=52), 3.0),
+            (dam_break(150, 20, hr=0.0), 2.0), (terrain(520, 640, seed=53, bc=VF.PeriodicBC()), 0.3)]
+    with VF.Server(8 << 30) as sv:
+        ids = []
+        for sc, te in reqs:
+            ids.append(sv.submit(sc.system, sc.grid, sc.U0, te, cfl=sc.cfl))
+            time.sleep(0.005)
+        outs = [sv.fetch(rid) if sv.wait(rid, timeout=120) else None for rid in ids]
+        st = sv.stats()
+    assert st["completed"] == len(reqs) and st["failed"] == 0
+    for (sc, te), (U, info) in zip(reqs, outs):
+        alone = _alone(sc, te)
+        assert info["t"] == alone.t == te, sc.name
+        assert np.array_equal(U, _interior(alone)), sc.name

@@ -54,6 +54,7 @@ int BatchEngine::init(int precision, int capacity, std::string& why) {
     int cus = 256;
     if (hipGetDeviceProperties(&prop, device_) == hipSuccess && prop.multiProcessorCount > 0)
         cus = prop.multiProcessorCount;
+    cus_ = cus;
     const int per_cu = std::min(march_batch_blocks_per_cu(precision, 1), march_batch_blocks_per_cu(precision, 2));
     blocks_stage_ = std::max(8, (cus * per_cu + 7) / 8 * 8);
     blocks_bc_ = std::max(1, cus);
@@ -120,12 +121,6 @@ int BatchEngine::add(sfv_handle* h, const RunParams& rp, int* slot_out, std::str
         if (rc) return rc;
     }
     ETRY(hipStreamSynchronize(h->stream));   // pending set_state / topography uploads
-    if (tune_) {
-        const int rt = tune_march_rows(h, why);
-        if (rt) return rt;
-    } else {
-        (void)cached_march_rows(h);
-    }
     BatchMember M;
     std::memset(&M, 0, sizeof(M));
     M.P = h->P;
@@ -184,13 +179,14 @@ bool BatchEngine::finished(int slot) const {
 // Segment heights of the members for the current membership.  When the
 // batch fills the device's resident wave slots at the build default height,
 // every member takes it (fewest halo instructions).  Otherwise the device is
-// underfilled and the step time is a wave's serial path: each member takes
-// its own measured height (short segments for small grids), raised to the
-// tallest height at which the batch still fills the slots.  Heights set with
-// sfv_set_march_rows are kept.  No height changes any result.
+// underfilled and a step lasts about one wave's serial path, which shorter
+// segments shorten as long as no SIMD has to run two of the batch's waves:
+// every member takes the shortest height at which the batch's waves fit one
+// per SIMD (the default if none does).  Heights set with sfv_set_march_rows
+// are kept.  No height changes any result.
 void BatchEngine::choose_rows() {
     const int R0 = SFV_BATCH_DEFAULT_ROWS;
-    const long slots = (long)blocks_stage_ * MARCH_WAVES;
+    const long slots = (long)blocks_stage_ * MARCH_WAVES, simds = 4L * cus_;
     auto waves = [&](int r) {
         long w = 0;
         for (int s : rows_) {
@@ -200,17 +196,14 @@ void BatchEngine::choose_rows() {
         }
         return w;
     };
-    const bool filled = waves(R0) >= slots;
-    int r_fill = 1;
-    if (!filled)
-        for (int r : {8, 6, 4, 3, 2})
-            if (waves(r) >= slots) { r_fill = r; break; }
+    int r = R0;
+    if (waves(R0) < slots)
+        for (int c : {1, 2, 3, 4, 6, 8})
+            if (waves(c) <= simds) { r = c; break; }
     for (int s : rows_) {
         const sfv_handle* h = slot_handle_[s];
         BatchMember& M = slot_row_[s];
-        int my = R0;
-        if (h->rows_forced) my = h->march_rows;
-        else if (!filled) my = std::max(h->march_rows > 0 ? h->march_rows : 1, r_fill);
+        const int my = h->rows_forced ? h->march_rows : r;
         M.my = my;
         M.P.rows = my;
         const int segs = (h->P.ny + my - 1) / my;

@@ -54,11 +54,6 @@ public:
     int replay_begin(int k, std::string& why);
     int replay_end(std::string& why);
 
-    // Measure the march segment height of members whose grid size class has
-    // none yet (on by default).  Off, such members take the fill rule of
-    // upload_table alone, so adding them never stalls the device.
-    void set_tuning(bool on) { tune_ = on; }
-
     int members() const { return (int)rows_.size(); }
     // After replay: has the member in `slot` finished its run?
     bool finished(int slot) const;
@@ -87,7 +82,7 @@ private:
     StepCtl* h_ctl_ = nullptr;
     hipGraphExec_t graph_ = nullptr;
     bool table_dirty_ = true;
-    bool tune_ = true;
+    int cus_ = 256;
     std::vector<int> rows_;                  // slot of each table row
     std::vector<sfv_handle*> slot_handle_;   // nullptr: free slot
     std::vector<BatchMember> slot_row_;      // the member's row (host copy)

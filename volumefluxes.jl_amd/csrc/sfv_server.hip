@@ -1,10 +1,11 @@
 // sfv_server.hip — serving: flood scenarios submitted at any time are
 // admitted under an HBM byte budget and advanced together with the batch
-// engine (continuous batching).  A worker thread owns every HIP call: it
+// engines (continuous batching).  A worker thread owns every HIP call: it
 // admits queued requests in arrival order while their device bytes fit the
-// budget, replays the batched step graph over everything admitted, and
-// retires each request when it reaches its end time: the result is copied to
-// host memory and its device memory freed, which admits the next ones.
+// budget, replays the batched step graphs over everything admitted (one
+// engine for large grids, one for small ones), and retires each request when
+// it reaches its end time: the result is copied to host memory and its
+// device buffers go back to the server's pool, which admits the next ones.
 //
 // Each request's result is bitwise that of running it alone
 // (simulate_to_time with the same arguments): it keeps its own dt and time,
@@ -49,6 +50,7 @@ struct Request {
     std::vector<unsigned char> result;  // interior state at the end
     sfv_handle* h = nullptr;
     int slot = -1;
+    int eng = 0;                        // engine: 0 large grids, 1 small grids
 };
 
 }  // namespace
@@ -62,7 +64,6 @@ struct sfv_server {
     std::condition_variable cv_work, cv_done;
     std::deque<Request*> queue;
     std::unordered_map<int64_t, std::unique_ptr<Request>> requests;
-    std::vector<Request*> running;     // worker-owned
     std::thread worker;
     bool stop = false;
     bool worker_failed = false;
@@ -146,16 +147,24 @@ sfv::RunParams run_params(const Request* r) {
 
 }  // namespace
 
-// The worker overlaps host work with the device: while one replay of the
-// batched graph runs, it downloads the results of the requests that finished
-// in the previous replay and creates (and uploads) the newly admitted ones;
-// membership changes happen between replays.
+// Grids of at most this many cells run in the small-grid engine.
+constexpr int64_t SMALL_GRID_CELLS = 1 << 18;
+
+// The worker keeps two batch engines.  Large grids step in one, replayed once
+// per loop (8 steps of every member), and while that replay runs the worker
+// downloads finished results and creates the newly admitted requests.  Small
+// grids, whose steps are short and latency-bound, step in the other, replayed
+// k times per loop between the large-grid replays, with k chosen so that the
+// small grids take about half of the device time: a small request that
+// needs many steps then is not held to the step rate of the large ones.
+// Membership changes happen between replays.
 void sfv_server::work() {
     std::string why;
-    sfv::BatchEngine eng;
+    sfv::BatchEngine engs[2];
     int rc = hipSetDevice(device) == hipSuccess ? SFV_OK : SFV_ERR_HIP;
-    if (!rc) rc = eng.init(precision, 64, why);
-    eng.set_tuning(false);             // a new size class must not stall the other requests
+    for (int e = 0; e < 2 && !rc; ++e) {
+        rc = engs[e].init(precision, 64, why);
+    }
     if (rc) {
         std::lock_guard<std::mutex> lk(mu);
         worker_failed = true;
@@ -175,17 +184,43 @@ void sfv_server::work() {
         if (status == REQ_DONE) { ++completed; cell_updates += r->steps * (int64_t)r->cfg.nx * r->cfg.ny; }
         else ++failed;
     };
-    std::vector<Request*> to_retire;   // finished, removed from the engine, result not yet downloaded
-    std::vector<Request*> created;     // handle made, not yet in the engine
-    bool in_flight = false;
-    double t_begin = 0;
+    std::vector<Request*> run[2];      // members of each engine
+    std::vector<Request*> to_retire;   // finished, removed from their engine, result not yet downloaded
+    std::vector<Request*> created;     // handle made, not yet in an engine
+    bool large_in_flight = false;
+    double t_large = 0, large_s = 0, small_s = 0;   // replay durations (moving averages)
+    auto fail_all = [&](int e, int code) {
+        std::lock_guard<std::mutex> lk(mu);
+        for (Request* r : run[e]) {
+            engs[e].remove(r->slot);
+            sfv_destroy(r->h);
+            r->h = nullptr;
+            finish(r, REQ_FAILED, code, why);
+        }
+        run[e].clear();
+        cv_done.notify_all();
+    };
+    // after a replay of engine e: finished requests leave it
+    auto collect = [&](int e) {
+        std::vector<Request*> still;
+        for (Request* r : run[e]) {
+            if (!engs[e].finished(r->slot)) { still.push_back(r); continue; }
+            const sfv::StepCtl& c = engs[e].ctl(r->slot);
+            r->t = c.t;
+            r->steps = c.steps;
+            r->error = c.nonfinite != 0 ? SFV_ERR_NONFINITE_DT : SFV_OK;
+            engs[e].remove(r->slot);
+            to_retire.push_back(r);
+        }
+        run[e].swap(still);
+    };
     for (;;) {
         // 1. admissions, in arrival order while the next request fits the budget
         std::vector<Request*> admit;
         {
             std::unique_lock<std::mutex> lk(mu);
-            if (!in_flight && to_retire.empty() && created.empty())
-                cv_work.wait(lk, [&] { return stop || !queue.empty() || !running.empty(); });
+            if (!large_in_flight && to_retire.empty() && created.empty() && run[1].empty())
+                cv_work.wait(lk, [&] { return stop || !queue.empty() || !run[0].empty() || !run[1].empty(); });
             if (stop) break;
             while (!queue.empty()) {
                 Request* r = queue.front();
@@ -210,7 +245,7 @@ void sfv_server::work() {
                 admit.push_back(r);
             }
         }
-        // 2. host work, overlapping the replay in flight: retire, then create
+        // 2. host work, overlapping the large-grid replay in flight
         if (!to_retire.empty()) {
             const double t0 = now();
             for (Request* r : to_retire) {
@@ -246,47 +281,30 @@ void sfv_server::work() {
                     finish(r, REQ_FAILED, rc2, why);
                     cv_done.notify_all();
                 } else {
+                    r->eng = (int64_t)r->cfg.nx * r->cfg.ny <= SMALL_GRID_CELLS ? 1 : 0;
                     created.push_back(r);
                 }
             }
             std::lock_guard<std::mutex> lk(mu);
             host_admit_s += now() - t0;
         }
-        // 3. end the replay in flight; finished requests leave the engine
-        if (in_flight) {
-            in_flight = false;
-            const int rc3 = eng.replay_end(why);
-            const double t_end = now();
-            if (rc3) {
-                std::lock_guard<std::mutex> lk(mu);
-                for (Request* r : running) {
-                    eng.remove(r->slot);
-                    sfv_destroy(r->h);
-                    r->h = nullptr;
-                    finish(r, REQ_FAILED, rc3, why);
-                }
-                running.clear();
-                cv_done.notify_all();
-            } else {
-                std::vector<Request*> still;
-                for (Request* r : running) {
-                    if (!eng.finished(r->slot)) { still.push_back(r); continue; }
-                    const sfv::StepCtl& c = eng.ctl(r->slot);
-                    r->t = c.t;
-                    r->steps = c.steps;
-                    r->error = c.nonfinite != 0 ? SFV_ERR_NONFINITE_DT : SFV_OK;
-                    eng.remove(r->slot);
-                    to_retire.push_back(r);
-                }
-                running.swap(still);
+        // 3. end the large-grid replay
+        if (large_in_flight) {
+            large_in_flight = false;
+            const int rc3 = engs[0].replay_end(why);
+            const double dt = now() - t_large;
+            large_s = large_s > 0 ? 0.7 * large_s + 0.3 * dt : dt;
+            {
                 std::lock_guard<std::mutex> lk(mu);
                 ++replays;
-                busy_s += t_end - t_begin;
+                busy_s += dt;
             }
+            if (rc3) fail_all(0, rc3);
+            else collect(0);
         }
-        // 4. the created requests join
+        // 4. the created requests join their engine (both are idle here)
         for (Request* r : created) {
-            const int rc5 = eng.add(r->h, run_params(r), &r->slot, why);
+            const int rc5 = engs[r->eng].add(r->h, run_params(r), &r->slot, why);
             if (rc5) {
                 sfv_destroy(r->h);
                 r->h = nullptr;
@@ -294,40 +312,55 @@ void sfv_server::work() {
                 finish(r, REQ_FAILED, rc5, why);
                 cv_done.notify_all();
             } else {
-                running.push_back(r);
+                run[r->eng].push_back(r);
             }
         }
         created.clear();
-        // 5. the next replay
-        if (!running.empty()) {
-            t_begin = now();
-            const int rc6 = eng.replay_begin(1, why);
-            if (rc6) {
-                (void)eng.replay_end(why);
+        // 5. small grids: k replays, then their finished requests leave
+        if (!run[1].empty()) {
+            // with large grids running, the small ones get about half the
+            // device time; alone, 4 replays (32 steps) per host round trip
+            int k = 4;
+            if (!run[0].empty())
+                k = large_s > 0 && small_s > 0 ? (int)std::min(64.0, std::max(1.0, std::round(large_s / small_s))) : 1;
+            const double t0 = now();
+            int rc6 = engs[1].replay_begin(k, why);
+            const int rc7 = engs[1].replay_end(why);
+            if (!rc6) rc6 = rc7;
+            const double dt = now() - t0;
+            small_s = small_s > 0 ? 0.7 * small_s + 0.3 * dt / k : dt / k;
+            {
                 std::lock_guard<std::mutex> lk(mu);
-                for (Request* r : running) {
-                    eng.remove(r->slot);
-                    sfv_destroy(r->h);
-                    r->h = nullptr;
-                    finish(r, REQ_FAILED, rc6, why);
-                }
-                running.clear();
-                cv_done.notify_all();
+                replays += k;
+                busy_s += dt;
+            }
+            if (rc6) fail_all(1, rc6);
+            else collect(1);
+        }
+        // 6. the next large-grid replay, left in flight
+        if (!run[0].empty()) {
+            t_large = now();
+            const int rc8 = engs[0].replay_begin(1, why);
+            if (rc8) {
+                (void)engs[0].replay_end(why);
+                fail_all(0, rc8);
             } else {
-                in_flight = true;
+                large_in_flight = true;
             }
         }
     }
     // stopped: release what is still on the device
-    if (in_flight) (void)eng.replay_end(why);
+    if (large_in_flight) (void)engs[0].replay_end(why);
     std::lock_guard<std::mutex> lk(mu);
-    for (Request* r : running) {
-        eng.remove(r->slot);
-        sfv_destroy(r->h);
-        r->h = nullptr;
-        finish(r, REQ_FAILED, SFV_ERR_STATE, "server stopped");
+    for (int e = 0; e < 2; ++e) {
+        for (Request* r : run[e]) {
+            engs[e].remove(r->slot);
+            sfv_destroy(r->h);
+            r->h = nullptr;
+            finish(r, REQ_FAILED, SFV_ERR_STATE, "server stopped");
+        }
+        run[e].clear();
     }
-    running.clear();
     for (Request* r : to_retire) {
         sfv_destroy(r->h);
         r->h = nullptr;
