@@ -77,6 +77,19 @@ def test_adjoint_single_corners_and_friction():
     assert VF.manning_gradient(res["dJ_dCz"], 0.03) == 2 * 0.03 * res["dJ_dCz"]
 
 
+def test_adjoint_recompute_path_equals_kept_path(monkeypatch):
+    """Segments either keep their stage-1 states and right-hand sides from the
+    recompute or recompute them per reverse step: the same kernels on the same
+    inputs, so the gradients are identical."""
+    sc = terrain(44, 36, seed=25)
+    tgt = _target(sc)
+    kept = VF.adjoint_gradient(make_hip(sc), 14, tgt, checkpoint_every=4)
+    monkeypatch.setenv("SFV_ADJOINT_KEEP", "0")
+    again = VF.adjoint_gradient(make_hip(sc), 14, tgt, checkpoint_every=4)
+    assert kept["J"] == again["J"] and kept["dJ_dCz"] == again["dJ_dCz"]
+    assert np.array_equal(kept["dJ_dB"], again["dJ_dB"]) and np.array_equal(kept["dJ_dU0"], again["dJ_dU0"])
+
+
 def test_adjoint_rejects_unsupported_runs():
     sc = terrain(20, 20, seed=1, bc=VF.NeumannBC())
     with pytest.raises(VF.ArgumentError):

@@ -22,6 +22,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <type_traits>
@@ -432,10 +433,19 @@ __device__ inline void face_speeds(const KParams& P, const double* U, const doub
     sl = ad::central_upwind<double>(d, Rl, Lm, P.phys, F);
 }
 
-__global__ void __launch_bounds__(256) k_adj_argmax(KParams P, const StepCtl* ctl, int dsel, const double* __restrict__ U,
+// The direction whose wave speed set dt: min(Δx/a_x, Δy/a_y) picks y on a
+// tie (simulator.jl:147-150 with Julia's minimum).
+__device__ __forceinline__ int dt_direction(const KParams& P, const StepCtl* ctl) {
+    const double ax = __longlong_as_double((long long)ctl->slot[0].wsmax[0]);
+    const double ay = __longlong_as_double((long long)ctl->slot[0].wsmax[1]);
+    return (P.dx / ax < P.dy / ay) ? 0 : 1;
+}
+
+__global__ void __launch_bounds__(256) k_adj_argmax(KParams P, const StepCtl* ctl, const double* __restrict__ U,
                                                     const double* __restrict__ B, unsigned long long* __restrict__ best) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= P.nx * P.ny) return;
+    const int dsel = dt_direction(P, ctl);
     const int i = P.gc + t % P.nx, j = P.gc + t / P.nx;
     double sr, sl;
     face_speeds(P, U, B, dsel, i, j, sr, sl);
@@ -444,10 +454,20 @@ __global__ void __launch_bounds__(256) k_adj_argmax(KParams P, const StepCtl* ct
         atomicMin(best, (unsigned long long)cid(P, i, j));
 }
 
-__global__ void k_adj_dtface(KParams P, const StepCtl* ctl, int d, int dsel, const double* __restrict__ U,
+// A step whose dt came from a finite positive wave speed must have a face
+// that reproduces that speed; otherwise the sticky flag is raised.
+__global__ void k_adj_check(const StepCtl* ctl, const unsigned long long* __restrict__ best, int* __restrict__ flag) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const double ax = __longlong_as_double((long long)ctl->slot[0].wsmax[0]);
+    const double ay = __longlong_as_double((long long)ctl->slot[0].wsmax[1]);
+    const bool armed = __builtin_isfinite(ax) && __builtin_isfinite(ay) && (ax > 0.0 || ay > 0.0);
+    if (armed && *best == ~0ull) *flag = 1;
+}
+
+__global__ void k_adj_dtface(KParams P, const StepCtl* ctl, int d, const double* __restrict__ U,
                              const double* __restrict__ B, const unsigned long long* __restrict__ best,
                              const double* __restrict__ sums, double* __restrict__ LfL, double* __restrict__ LfR) {
-    if (threadIdx.x != 0 || blockIdx.x != 0 || d != dsel) return;
+    if (threadIdx.x != 0 || blockIdx.x != 0 || d != dt_direction(P, ctl)) return;
     const unsigned long long kbest = *best;
     if (kbest >= (unsigned long long)P.nxT * P.nyT) return;
     const int i = (int)(kbest % P.nxT), j = (int)(kbest / P.nxT);
@@ -478,7 +498,9 @@ __global__ void k_adj_dtface(KParams P, const StepCtl* ctl, int d, int dsel, con
 
 // Sum the two stages' block partials (dt) and the Cz partials, in order.
 __global__ void k_adj_sum(const double* __restrict__ part_dt, const double* __restrict__ part_cz, int count,
-                          double* __restrict__ sums, double* __restrict__ cz_total) {
+                          double* __restrict__ sums, double* __restrict__ cz_total,
+                          unsigned long long* __restrict__ best) {
+    if (threadIdx.x == 0) *best = ~0ull;     // argmax search of this step starts afresh
     __shared__ double red[2][256];
     double a = 0.0, b = 0.0;
     for (int k = threadIdx.x; k < count; k += blockDim.x) { a += part_dt[k]; b += part_cz[k]; }
@@ -575,6 +597,18 @@ struct AdjBuffers {
         }                                                                                            \
     } while (0)
 
+// One step U0 -> U2 of the forward pass with the march kernels, in step
+// slot `slot` of a loop prepared once (its boundary condition advances t and
+// arms the next slot).
+int march_step(sfv_handle* h, int slot, const double* U0, double* U2) {
+    const KParams& P = h->P;
+    hipStream_t st = h->stream;
+    ATRY(launch_march<double>(st, P, h->ctl, slot, 1, U0, nullptr, (double*)h->RHS, h->B));
+    ATRY(launch_march<double>(st, P, h->ctl, slot, 2, U0, (const double*)h->RHS, U2, h->B));
+    ATRY(launch_bc<double>(st, P, h->ctl, slot, U2, BC_LAST_STAGE));
+    return SFV_OK;
+}
+
 // One forward step U0 -> U2 with the per-pass kernels (bitwise the fused
 // path), keeping the stage-1 state and both right-hand sides.
 int forward_step(sfv_handle* h, double t, const double* U0, double* U1, double* R1, double* R2, double* U2) {
@@ -636,25 +670,42 @@ extern "C" int sfv_adjoint(sfv_handle* h, int64_t nsteps, int32_t checkpoint_eve
     double* sums = buf.alloc(4, why, rc);                      // [0] dt adjoint of the step, [1] Cz total, [2] J
     double* part_j = buf.alloc((size_t)nblk, why, rc);
     unsigned long long* best = (unsigned long long*)buf.alloc(1, why, rc);
+    double* d_times = buf.alloc((size_t)nsteps + 1, why, rc);
+    int* flag = (int*)buf.alloc(1, why, rc);
+    // The stage-1 states, right-hand sides and step controls of a segment are
+    // kept from its recompute when they fit in half the free device memory;
+    // otherwise each reverse step recomputes its own.
+    size_t free_b = 0, total_b = 0;
+    const size_t keep_bytes = (size_t)ce * (3 * n3 * sizeof(double) + sizeof(StepCtl));
+    const char* keep_env = std::getenv("SFV_ADJOINT_KEEP");      // "0": always recompute
+    const bool keep = !rc && !(keep_env && keep_env[0] == '0') && hipMemGetInfo(&free_b, &total_b) == hipSuccess &&
+                      keep_bytes < free_b / 2;
+    double* kU1 = keep ? buf.alloc(n3 * (size_t)ce, why, rc) : nullptr;
+    double* kR1 = keep ? buf.alloc(n3 * (size_t)ce, why, rc) : nullptr;
+    double* kR2 = keep ? buf.alloc(n3 * (size_t)ce, why, rc) : nullptr;
+    StepCtl* kctl = keep ? (StepCtl*)buf.alloc((size_t)ce * (sizeof(StepCtl) / sizeof(double) + 1), why, rc) : nullptr;
     if (rc) { h->err = why; return rc; }
     ATRY(hipMemcpyAsync(tgt, h_target, sizeof(double) * P.nx * P.ny, hipMemcpyHostToDevice, st));
 
-    // forward pass with checkpoints; step start times kept on the host
+    // forward pass with checkpoints, device-resident: one prepared loop of
+    // march steps; each step's end time is copied on the device and the
+    // times come back once
     std::vector<double> times((size_t)nsteps + 1);
-    double t = *t_io;
     double* cur = (double*)h->U[h->cur & 1];
     ATRY(hipMemcpyAsync(seg, cur, n3 * sizeof(double), hipMemcpyDeviceToDevice, st));
+    ATRY(launch_prepare(st, P, h->ctl, 0, *t_io, NAN, 0, 0, 0.0, -1, 0, SFV_F64));
     for (int64_t s = 0; s < nsteps; ++s) {
         double* a = seg + (size_t)(s & 1) * n3;
         double* b = seg + (size_t)((s + 1) & 1) * n3;
         if (s % ce == 0) ATRY(hipMemcpyAsync(ck + (size_t)(s / ce) * n3, a, n3 * sizeof(double), hipMemcpyDeviceToDevice, st));
-        times[(size_t)s] = t;
-        if ((rc = forward_step(h, t, a, U1, R1, R2, b))) return rc;
-        ATRY(hipMemcpyAsync(h->ctl_host, h->ctl, sizeof(StepCtl), hipMemcpyDeviceToHost, st));
-        ATRY(hipStreamSynchronize(st));
-        t = h->ctl_host->t;
+        if ((rc = march_step(h, (int)(s & 1), a, b))) return rc;
+        ATRY(hipMemcpyAsync(d_times + s + 1, &h->ctl->t, sizeof(double), hipMemcpyDeviceToDevice, st));
     }
-    times[(size_t)nsteps] = t;
+    ATRY(hipMemcpyAsync(times.data() + 1, d_times + 1, (size_t)nsteps * sizeof(double), hipMemcpyDeviceToHost, st));
+    ATRY(hipMemcpyAsync(h->ctl_host, h->ctl, sizeof(StepCtl), hipMemcpyDeviceToHost, st));
+    ATRY(hipStreamSynchronize(st));
+    times[0] = *t_io;
+    const double t = times[(size_t)nsteps];
     const double last_dt = h->ctl_host->dt;
     double* UT = seg + (size_t)(nsteps & 1) * n3;
     // the final state becomes the handle's state
@@ -677,13 +728,27 @@ extern "C" int sfv_adjoint(sfv_handle* h, int64_t nsteps, int32_t checkpoint_eve
     for (int64_t sg = nseg - 1; sg >= 0; --sg) {
         const int64_t s0 = sg * ce, s1 = std::min<int64_t>(nsteps, s0 + ce);
         ATRY(hipMemcpyAsync(seg, ck + (size_t)sg * n3, n3 * sizeof(double), hipMemcpyDeviceToDevice, st));
-        for (int64_t s = s0; s < s1 - 1; ++s) {
+        for (int64_t s = s0; s < (keep ? s1 : s1 - 1); ++s) {
             double* a = seg + (size_t)(s - s0) * n3;
-            if ((rc = forward_step(h, times[(size_t)s], a, U1, R1, R2, a + n3))) return rc;
+            const size_t o = (size_t)(s - s0) * n3;
+            if (keep) {
+                if ((rc = forward_step(h, times[(size_t)s], a, kU1 + o, kR1 + o, kR2 + o, a + n3))) return rc;
+                ATRY(hipMemcpyAsync(kctl + (s - s0), h->ctl, sizeof(StepCtl), hipMemcpyDeviceToDevice, st));
+            } else {
+                if ((rc = forward_step(h, times[(size_t)s], a, U1, R1, R2, a + n3))) return rc;
+            }
         }
         for (int64_t s = s1 - 1; s >= s0; --s) {
             const double* U0 = seg + (size_t)(s - s0) * n3;
-            if ((rc = forward_step(h, times[(size_t)s], U0, U1, R1, R2, Ut))) return rc;
+            if (keep) {
+                const size_t o = (size_t)(s - s0) * n3;
+                U1 = kU1 + o;
+                R1 = kR1 + o;
+                R2 = kR2 + o;
+                ATRY(hipMemcpyAsync(h->ctl, kctl + (s - s0), sizeof(StepCtl), hipMemcpyDeviceToDevice, st));
+            } else if ((rc = forward_step(h, times[(size_t)s], U0, U1, R1, R2, Ut))) {
+                return rc;
+            }
             // fold the output's ghosts (its boundary condition)
             hipLaunchKernelGGL(ad::k_adj_bc, dim3(bcx), dim3(256), 0, st, P, L, 0);
             hipLaunchKernelGGL(ad::k_adj_bc, dim3(bcy), dim3(256), 0, st, P, L, 1);
@@ -701,29 +766,15 @@ extern "C" int sfv_adjoint(sfv_handle* h, int64_t nsteps, int32_t checkpoint_eve
             // stage 1: U1 = clamp(friction(U0 + dt R1))
             hipLaunchKernelGGL(ad::k_adj_tail, cg, cb, 0, st, P, h->ctl, 1, U0, U0, R1, h->B, L1, Lnew, nullptr, LR,
                                LBc, part_dt, part_cz);
-            hipLaunchKernelGGL(ad::k_adj_sum, dim3(1), dim3(256), 0, st, part_dt, part_cz, 2 * nblk, sums, sums + 1);
-            // the direction that set dt (jl_min picks y on a tie)
-            ATRY(hipMemcpyAsync(h->ctl_host, h->ctl, sizeof(StepCtl), hipMemcpyDeviceToHost, st));
-            ATRY(hipStreamSynchronize(st));
-            const double ax = __builtin_bit_cast(double, (long long)h->ctl_host->slot[0].wsmax[0]);
-            const double ay = __builtin_bit_cast(double, (long long)h->ctl_host->slot[0].wsmax[1]);
-            const int dsel = (P.dx / ax < P.dy / ay) ? 0 : 1;
-            const unsigned long long none = ~0ull;
-            ATRY(hipMemcpyAsync(best, &none, sizeof(none), hipMemcpyHostToDevice, st));
-            hipLaunchKernelGGL(ad::k_adj_argmax, dim3((P.nx * P.ny + 255) / 256), dim3(256), 0, st, P, h->ctl, dsel, U0,
+            hipLaunchKernelGGL(ad::k_adj_sum, dim3(1), dim3(256), 0, st, part_dt, part_cz, 2 * nblk, sums, sums + 1,
+                               best);
+            // the face that set dt, found as the reference's maximum(ws) would
+            hipLaunchKernelGGL(ad::k_adj_argmax, dim3((P.nx * P.ny + 255) / 256), dim3(256), 0, st, P, h->ctl, U0,
                                h->B, best);
-            if (std::isfinite(ax) && std::isfinite(ay) && (ax > 0.0 || ay > 0.0)) {
-                unsigned long long found = none;
-                ATRY(hipMemcpyAsync(&found, best, sizeof(found), hipMemcpyDeviceToHost, st));
-                ATRY(hipStreamSynchronize(st));
-                if (found == none) {
-                    h->err = "adjoint: no face reproduces the step's maximum wave speed";
-                    return SFV_ERR_STATE;
-                }
-            }
+            hipLaunchKernelGGL(ad::k_adj_check, dim3(1), dim3(64), 0, st, h->ctl, best, flag);
             for (int d = 0; d < 2; ++d) {
                 hipLaunchKernelGGL(ad::k_adj_faces, faces_grid(d), dim3(256), 0, st, P, d, U0, h->B, LR, LfL, LfR);
-                hipLaunchKernelGGL(ad::k_adj_dtface, dim3(1), dim3(64), 0, st, P, h->ctl, d, dsel, U0, h->B, best, sums,
+                hipLaunchKernelGGL(ad::k_adj_dtface, dim3(1), dim3(64), 0, st, P, h->ctl, d, U0, h->B, best, sums,
                                    LfL, LfR);
                 hipLaunchKernelGGL(ad::k_adj_recon, cg, cb, 0, st, P, d, U0, h->B, LR, LfL, LfR, Lnew, LBf);
             }
@@ -737,6 +788,8 @@ extern "C" int sfv_adjoint(sfv_handle* h, int64_t nsteps, int32_t checkpoint_eve
     hipLaunchKernelGGL(ad::k_adj_corners, dim3((P.nxT + 1 + 63) / 64, (P.nyT + 1 + 3) / 4), cb, 0, st, P, LBc, LBf, LB);
     ATRY(hipPeekAtLastError());
     double czsum = 0.0;
+    int missing = 0;
+    ATRY(hipMemcpyAsync(&missing, flag, sizeof(int), hipMemcpyDeviceToHost, st));
     ATRY(hipMemcpyAsync(&czsum, sums + 1, sizeof(double), hipMemcpyDeviceToHost, st));
     if (dJ_dB) ATRY(hipMemcpyAsync(dJ_dB, LB, nB * sizeof(double), hipMemcpyDeviceToHost, st));
     std::vector<double> Lh;
@@ -745,6 +798,10 @@ extern "C" int sfv_adjoint(sfv_handle* h, int64_t nsteps, int32_t checkpoint_eve
         ATRY(hipMemcpyAsync(Lh.data(), L, n3 * sizeof(double), hipMemcpyDeviceToHost, st));
     }
     ATRY(hipStreamSynchronize(st));
+    if (missing) {
+        h->err = "adjoint: no face reproduces a step's maximum wave speed";
+        return SFV_ERR_STATE;
+    }
     if (dJ_dU0)
         for (int v = 0; v < 3; ++v)
             for (int j = 0; j < P.ny; ++j)
