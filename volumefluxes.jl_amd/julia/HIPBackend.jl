@@ -280,4 +280,55 @@ function upload_interior!(sim, box)
     box.host_valid = false          # ghosts now differ from the mirror until the next read
 end
 
+# ---- segment height of the march kernels (autotuned on the first step) -----------
+set_march_rows!(sim::SinFVM.Simulator{<:HIPBackend}, rows::Integer) =
+    check(handle!(sim).h, ccall((:sfv_set_march_rows, libsinfvm), Cint, (Ptr{Cvoid}, Int32), handle!(sim).h, rows))
+march_rows(sim::SinFVM.Simulator{<:HIPBackend}) =
+    Int(ccall((:sfv_get_march_rows, libsinfvm), Int32, (Ptr{Cvoid},), handle!(sim).h))
+
+# ---- reverse-mode gradient (sfv_adjoint) ------------------------------------------
+# J = sum((h_T - h_target).^2) over the interior after nsteps perform_step!(sim, Inf)
+# from the simulator's current state; returns (J, dJ/dB over the corners, dJ/dCz,
+# dJ/dU0 over the interior).  The simulator ends at the final state, as after
+# nsteps steps of its own.
+function adjoint_gradient(sim::SinFVM.Simulator{<:HIPBackend}, nsteps::Integer, h_target::AbstractMatrix;
+                          checkpoint_every::Integer=20)
+    box = handle!(sim)
+    nx, ny = sim.grid.totalcells[1] - 2sim.grid.ghostcells[1], sim.grid.totalcells[2] - 2sim.grid.ghostcells[2]
+    tgt = collect(Float64, h_target)
+    size(tgt) == (nx, ny) || throw(DimensionMismatch("h_target must be ($nx, $ny)"))
+    dB = zeros(Float64, sim.grid.totalcells[1] + 1, sim.grid.totalcells[2] + 1)
+    dU0 = zeros(Float64, nx, ny, 3)
+    J, dCz, t = Ref{Float64}(0.0), Ref{Float64}(0.0), Ref{Float64}(sim.t[1])
+    check(box.h, ccall((:sfv_adjoint, libsinfvm), Cint,
+                       (Ptr{Cvoid}, Int64, Int32, Ref{Float64}, Ptr{Float64}, Ref{Float64}, Ptr{Float64},
+                        Ref{Float64}, Ptr{Float64}),
+                       box.h, nsteps, checkpoint_every, t, tgt, J, dB, dCz, dU0))
+    sim.t[1] = t[]
+    box.host_valid = false
+    return J[], dB, dCz[], dU0
+end
+
+# ChainRules rule for the loss of a run, so Zygote / ChainRules users get the
+# device adjoint instead of differentiating the stepping loop.  Parameters are
+# the corner topography B and Manning's n (Cz = n^2, ImplicitFriction's
+# default pattern, src/friction.jl:26); `make_sim(B, n)` builds a fresh
+# HIP simulator with its initial state set.  The pullback returns the
+# cotangents of B and n (no cotangent for make_sim, nsteps or h_target).
+function run_loss(make_sim, B, n, nsteps, h_target)
+    J, _, _, _ = adjoint_gradient(make_sim(B, n), nsteps, h_target)
+    return J
+end
+
+if isdefined(@__MODULE__, :ChainRulesCore) || Base.find_package("ChainRulesCore") !== nothing
+    @eval import ChainRulesCore
+    @eval function ChainRulesCore.rrule(::typeof(run_loss), make_sim, B, n, nsteps, h_target)
+        J, dB, dCz, _ = adjoint_gradient(make_sim(B, n), nsteps, h_target)
+        dn = 2n * dCz
+        pullback(ȳ) = (ChainRulesCore.NoTangent(), ChainRulesCore.NoTangent(), ȳ .* dB, ȳ * dn,
+                       ChainRulesCore.NoTangent(), ChainRulesCore.NoTangent())
+        return J, pullback
+    end
+end
+
 end # module
