@@ -95,6 +95,103 @@ __device__ T central_upwind(int d, const T m[3], const T q[3], const PhysParams&
     return djl_max(dabs(aplus), dabs(aminus));
 }
 
+// Which operand Julia's max / min returns (0: x, 1: y), as djl_max / djl_min.
+__device__ __forceinline__ int jmax_pick(double x, double y) {
+    if (x != x) return 0;
+    if (y != y) return 1;
+    if (x > y) return 0;
+    if (y > x) return 1;
+    return __builtin_signbit(x) ? 1 : 0;
+}
+__device__ __forceinline__ int jmin_pick(double x, double y) {
+    if (x != x) return 0;
+    if (y != y) return 1;
+    if (x < y) return 0;
+    if (y < x) return 1;
+    return __builtin_signbit(x) ? 0 : 1;
+}
+
+// One side of a face: its flux f and eigenvalues e (swe_2D.jl:43-81) and
+// their Jacobians with respect to the side state (h, hu, hv); all zero for
+// a dry side (centralupwind.jl:51-61 leaves them at zero).
+__device__ void side_jacobian(int d, const double s[3], const PhysParams& p, double f[3], double e[2],
+                              double Jf[3][3], double Je[2][3]) {
+    for (int v = 0; v < 3; ++v) {
+        f[v] = 0.0;
+        for (int a = 0; a < 3; ++a) Jf[v][a] = 0.0;
+    }
+    for (int k = 0; k < 2; ++k) {
+        e[k] = 0.0;
+        for (int a = 0; a < 3; ++a) Je[k][a] = 0.0;
+    }
+    if (!(s[0] > p.cutoff)) return;
+    using D3 = Dual<3>;
+    const D3 h = D3::var(s[0], 0), hu = D3::var(s[1], 1), hv = D3::var(s[2], 2);
+    D3 F[3], E[2];
+    ad::phys(d, h, hu, hv, p, F, E);
+    for (int v = 0; v < 3; ++v) {
+        f[v] = F[v].v;
+        for (int a = 0; a < 3; ++a) Jf[v][a] = F[v].d[a];
+    }
+    for (int k = 0; k < 2; ++k) {
+        e[k] = E[k].v;
+        for (int a = 0; a < 3; ++a) Je[k][a] = E[k].d[a];
+    }
+}
+
+// Reverse sweep of the central-upwind flux (centralupwind.jl:48-77) at one
+// face: given the flux adjoint lF, the adjoints of the minus side m and the
+// plus side q.  The values and every branch are those of central_upwind on
+// doubles; each side's flux and eigenvalue Jacobians come from a Dual<3>
+// evaluation, and the combination of the two sides is differentiated by hand.
+__device__ void face_vjp(int d, const double m[3], const double q[3], const PhysParams& p, const double lF[3],
+                         double lm[3], double lq[3]) {
+    double fm[3], em[2], Jfm[3][3], Jem[2][3];
+    double fp[3], ep[2], Jfp[3][3], Jep[2][3];
+    side_jacobian(d, m, p, fm, em, Jfm, Jem);
+    side_jacobian(d, q, p, fp, ep, Jfp, Jep);
+    for (int v = 0; v < 3; ++v) lm[v] = lq[v] = 0.0;
+    const int pk1 = jmax_pick(ep[0], em[0]);
+    const double ap1 = pk1 ? em[0] : ep[0];
+    const int pk2 = jmax_pick(ap1, 0.0);
+    const double aplus = pk2 ? 0.0 : ap1;
+    const int mk1 = jmin_pick(ep[1], em[1]);
+    const double am1 = mk1 ? em[1] : ep[1];
+    const int mk2 = jmin_pick(am1, 0.0);
+    const double aminus = mk2 ? 0.0 : am1;
+    const double D = aplus - aminus;
+    if (__builtin_fabs(D) < p.kappa) return;       // the flux is the constant zero
+    const double pm = (aplus * aminus) / D;
+    double l_ap = 0.0, l_am = 0.0, l_D = 0.0, l_pm = 0.0, l_fm[3], l_fp[3];
+    for (int v = 0; v < 3; ++v) {
+        const double N = aplus * fm[v] - aminus * fp[v];
+        const double lN = lF[v] / D;
+        l_D -= lN * (N / D);
+        l_pm += lF[v] * (q[v] - m[v]);
+        lq[v] += lF[v] * pm;
+        lm[v] -= lF[v] * pm;
+        l_ap += lN * fm[v];
+        l_am -= lN * fp[v];
+        l_fm[v] = lN * aplus;
+        l_fp[v] = -lN * aminus;
+    }
+    l_ap += l_pm * aminus / D;
+    l_am += l_pm * aplus / D;
+    l_D -= l_pm * pm / D;
+    l_ap += l_D;
+    l_am -= l_D;
+    double l_ep[2] = {0.0, 0.0}, l_em[2] = {0.0, 0.0};
+    if (!pk2) (pk1 ? l_em[0] : l_ep[0]) += l_ap;
+    if (!mk2) (mk1 ? l_em[1] : l_ep[1]) += l_am;
+    for (int a = 0; a < 3; ++a) {
+        double gm = 0.0, gq = 0.0;
+        for (int v = 0; v < 3; ++v) { gm += l_fm[v] * Jfm[v][a]; gq += l_fp[v] * Jfp[v][a]; }
+        for (int k = 0; k < 2; ++k) { gm += l_em[k] * Jem[k][a]; gq += l_ep[k] * Jep[k][a]; }
+        lm[a] += gm;
+        lq[a] += gq;
+    }
+}
+
 // linearreconstruction.jl:28-35: the argument minmod selects (0 if none), ties
 // resolved as Julia's min/max resolve them.
 template <class T>
@@ -335,15 +432,11 @@ __global__ void __launch_bounds__(256) k_adj_faces(KParams P, int d, const doubl
     double Lc[3], Rc[3], L1[3], R1[3];
     recon_at<double>(P, U, B, d, i, j, -2, false, Lc, Rc);
     recon_at<double>(P, U, B, d, i1, j1, -2, false, L1, R1);
-    using D6 = Dual<6>;
-    D6 m[3], q[3], F[3];
-    for (int v = 0; v < 3; ++v) { m[v] = D6::var(Rc[v], v); q[v] = D6::var(L1[v], 3 + v); }
-    (void)ad::central_upwind<D6>(d, m, q, P.phys, F);
+    double gm[3], gq[3];
+    ad::face_vjp(d, Rc, L1, P.phys, lF, gm, gq);
     for (int a = 0; a < 3; ++a) {
-        double gm = 0.0, gq = 0.0;
-        for (int v = 0; v < 3; ++v) { gm += lF[v] * F[v].d[a]; gq += lF[v] * F[v].d[3 + a]; }
-        LfR[a * n + kc] = gm;
-        LfL[a * n + k1] = gq;
+        LfR[a * n + kc] = gm[a];
+        LfL[a * n + k1] = gq[a];
     }
 }
 
