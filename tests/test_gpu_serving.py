@@ -138,3 +138,44 @@ def test_large_and_small_grids_served_together():
         alone = _alone(sc, te)
         assert info["t"] == alone.t == te, sc.name
         assert np.array_equal(U, _interior(alone)), sc.name
+
+
+def test_nonfinite_request_is_delivered_with_a_warning():
+    """A NaN in the initial state makes dt NaN, as in the reference, which
+    steps on with it (simulator.jl:190-192): the request finishes, its state
+    is delivered with a RuntimeWarning, and it equals the solo run's."""
+    sc = terrain(40, 30, seed=61)
+    from volumefluxes.scenarios import cell_bottom
+    depth = sc.U0[:, :, 0] - cell_bottom(sc.B)[2:-2, 2:-2]
+    i, j = np.argwhere(depth > 0.5)[0]           # a wet cell: its NaN momentum reaches dt
+    U0 = sc.U0.copy()
+    U0[i, j, 1] = np.nan
+    good = terrain(50, 20, seed=62)
+    with VF.Server(1 << 30) as sv:
+        rb = sv.submit(sc.system, sc.grid, U0, 2.0, cfl=sc.cfl)
+        rg = sv.submit(good.system, good.grid, good.U0, 1.0, cfl=good.cfl)
+        assert sv.wait(rb, timeout=60)["status"] == 2
+        with pytest.warns(RuntimeWarning):
+            Ub, ib = sv.fetch(rb)
+        Ug, ig = sv.fetch(rg) if sv.wait(rg, timeout=60) else (None, None)
+    solo = make_hip(sc)
+    VF.set_current_state(solo, U0)
+    with pytest.warns(RuntimeWarning):
+        VF.simulate_to_time(solo, 2.0)
+    a, b = Ub, _interior(solo)
+    assert np.array_equal(np.isnan(a), np.isnan(b)) and np.array_equal(a[~np.isnan(a)], b[~np.isnan(b)])
+    assert np.array_equal(Ug, _interior(_alone(good, 1.0)))     # the other request is unaffected
+
+
+def test_destroying_a_busy_server_fails_its_requests_cleanly():
+    sc = terrain(300, 300, seed=63)
+    sv = VF.Server(1 << 30)
+    ids = [sv.submit(sc.system, sc.grid, sc.U0, 50.0, cfl=sc.cfl) for _ in range(3)]
+    time.sleep(0.05)
+    sv.close()                         # stops the worker: queued and running requests fail
+    with VF.Server(1 << 30) as sv2:    # and the device is usable afterwards
+        rid = sv2.submit(sc.system, sc.grid, sc.U0, 0.2, cfl=sc.cfl)
+        assert sv2.wait(rid, timeout=60)["status"] == 2
+        U, _ = sv2.fetch(rid)
+    assert np.array_equal(U, _interior(_alone(sc, 0.2)))
+    assert len(ids) == 3
