@@ -59,3 +59,41 @@ def test_1d_members_in_an_ensemble():
     VF.Ensemble(sims).advance_steps(16)
     for sim, ref, c in zip(sims, refs, cases):
         assert np.array_equal(sim.get_full_state(), ref), c[0]
+
+
+def _bump_run(system, grid, stepper, T=0.05, elevate=0.0):
+    """test_swe1D_sim.jl:26-41 / test_friction_1d.jl:26-41: a Gaussian bump of
+    water, θ = 1.05, CFL 0.2, to T = 0.05 with match_endtime."""
+    xc = grid.cell_centers()
+    U0 = np.zeros((grid.nx, 2))
+    U0[:, 0] = np.exp(-(xc - 0.5) ** 2 / 0.001) + 1.5 + elevate
+    sim = VF.Simulator(VF.make_hip_backend(), system, stepper, grid, cfl=0.2)
+    VF.set_current_state(sim, U0)
+    VF.simulate_to_time(sim, T)
+    assert sim.t == T
+    return VF.current_interior_state(sim)
+
+
+@pytest.mark.parametrize("law", ["none", "friction"])
+@pytest.mark.parametrize("bottom", ["constant", "field"])
+def test_raising_bottom_and_water_by_one_changes_nothing(law, bottom):
+    """The practical-equation part of test_swe1D_sim.jl:99-124 and
+    test_friction_1d.jl:94-119: B = 1 (constant or a field on the faces) with
+    the water raised by 1 gives the B = 0 depths and momenta, Σ|Δ| within the
+    reference's 1e-6 (1e-2 with friction and RK2)."""
+    grid = VF.CartesianGrid(1024, gc=2)
+    stepper = VF.ForwardEulerStepper() if law == "none" else VF.RungeKutta2()
+    fr = VF.ImplicitFriction() if law == "friction" else None
+    rec = VF.LinearReconstruction(1.05)
+
+    def system(B, sources):
+        eq = VF.ShallowWaterEquations1D(B)
+        return VF.ConservedSystem(None, rec, VF.CentralUpwind(eq), eq, grid, sources, fr)
+
+    flat = _bump_run(system(VF.ConstantBottomTopography(0.0), []), grid, stepper)
+    B = (VF.ConstantBottomTopography(1.0) if bottom == "constant"
+         else VF.BottomTopography1D(np.ones(grid.nx + 2 * grid.gc + 1), None, grid))
+    raised = _bump_run(system(B, [VF.SourceTermBottom()]), grid, stepper, elevate=1.0)
+    tol = 1e-6 if law == "none" else 1e-2
+    assert np.sum(np.abs(flat.h - (raised.h - 1.0))) <= tol
+    assert np.sum(np.abs(flat.hu - raised.hu)) <= tol
