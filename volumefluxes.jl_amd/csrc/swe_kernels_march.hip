@@ -79,9 +79,22 @@ constexpr int MWAVES = MARCH_WAVES;   // waves per block
 
 // Lane shifts by one across the whole wave with DPP (wave_shr:1 / wave_shl:1,
 // VALU latency instead of an LDS round trip): lane i receives lane i-1
-// (shfl_up1) or lane i+1 (shfl_dn1); lane 0 / lane 63 keep their own value.
+// (shfl_up1) or lane i+1 (shfl_dn1).  Lane 0 (shfl_up1) / lane 63
+// (shfl_dn1) have no source lane and read zero; both are halo lanes whose
+// shifted values feed no output (the one exception, the corner column lane
+// 63 needs, is supplied separately by next_col).  With no old value to keep,
+// each shift is a single DPP move into a fresh register.
+#ifndef SFV_MARCH_DPP_ZERO
+#define SFV_MARCH_DPP_ZERO 1
+#endif
 template <int CTRL>
-__device__ __forceinline__ int dpp_shift(int v) { return __builtin_amdgcn_update_dpp(v, v, CTRL, 0xf, 0xf, false); }
+__device__ __forceinline__ int dpp_shift(int v) {
+#if SFV_MARCH_DPP_ZERO
+    return __builtin_amdgcn_mov_dpp(v, CTRL, 0xf, 0xf, true);
+#else
+    return __builtin_amdgcn_update_dpp(v, v, CTRL, 0xf, 0xf, false);
+#endif
+}
 template <int CTRL>
 __device__ __forceinline__ double dpp_shift(double v) {
     const long long b = __double_as_longlong(v);
