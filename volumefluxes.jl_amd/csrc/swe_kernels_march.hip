@@ -24,6 +24,7 @@
 // Stage 2 rebuilds U1 = clamp(friction(U0 + dt*RHS1)) for every loaded cell
 // (ghost cells through the boundary-condition source maps), so the stage-1
 // state never touches HBM.
+#include <cmath>
 #include <type_traits>
 #include <utility>
 
@@ -42,6 +43,7 @@ namespace sfv {
 //                         segments fastest (vertical halos stay in one L2);
 //                         2 = same, strips fastest; 0 = hardware order
 //   SFV_MARCH_UNROLL      unroll factor of the row loop
+//   SFV_MARCH_FAST        specialised kernels for the common configuration (march_fast)
 #ifndef SFV_MARCH_PREFETCH
 #define SFV_MARCH_PREFETCH 0
 #endif
@@ -62,6 +64,9 @@ namespace sfv {
 #endif
 #ifndef SFV_MARCH_XCD
 #define SFV_MARCH_XCD 2
+#endif
+#ifndef SFV_MARCH_FAST
+#define SFV_MARCH_FAST 1
 #endif
 #ifndef SFV_MARCH_UNROLL
 #define SFV_MARCH_UNROLL 1
@@ -146,7 +151,10 @@ __device__ __forceinline__ void stage1_cell(const KParams& P, const S x[3], cons
 // gc + seg*my .. +my-1.  Shared by the single-grid kernel (strip, seg from
 // the block index) and the batched kernel (from the batch's tile table).
 // Has no barriers: every wave is independent.
-template <class S, int STAGE>
+// FAST: the configuration is known to have linear reconstruction with a
+// finite nonzero θ, the bottom source, the 2D cell-bottom lookup and
+// power-of-two spacings (march_fast), so those run-time checks fold away.
+template <class S, int STAGE, bool FAST = false>
 __device__ __forceinline__ void march_tile(const KParams& P, StepCtl* ctl, int slot, int my, int strip, int seg,
                                            const S* __restrict__ U0, const S* __restrict__ RHS1,
                                            S* __restrict__ OUT, const double* __restrict__ B) {
@@ -161,6 +169,11 @@ __device__ __forceinline__ void march_tile(const KParams& P, StepCtl* ctl, int s
     const bool in_dom = i < nxT;
     const bool out_col = lane >= 2 && lane < MX + 2 && i < nxT - gc;
     const size_t n = (size_t)nxT * nyT;
+    const bool rec_linear = FAST || P.rec == 1, bottom = FAST || P.bottom;
+    auto bc4 = [&](double a, double b, double c, double d) {
+        return FAST ? 0.25 * (((a + b) + c) + d) : bcell4(P, a, b, c, d);
+    };
+    auto sdiv = [&](double x, double dd, double inv, int pow2) { return FAST ? x * inv : slope_div(x, dd, inv, pow2); };
 
     double dt_d = 0.0;
     S dt_s = S(0);
@@ -208,7 +221,7 @@ __device__ __forceinline__ void march_tile(const KParams& P, StepCtl* ctl, int s
             o.r[0] = RHS1[k]; o.r[1] = RHS1[n + k]; o.r[2] = RHS1[2 * n + k];
             o.remap = sci != i || sj != jr;
             if (o.remap)   // the source cell's bottom
-                o.bsrc = bcell4(P, Bload(P, B, sci, sj), Bload(P, B, sci + 1, sj), Bload(P, B, sci, sj + 1),
+                o.bsrc = bc4(Bload(P, B, sci, sj), Bload(P, B, sci + 1, sj), Bload(P, B, sci, sj + 1),
                                 Bload(P, B, sci + 1, sj + 1));
         }
     };
@@ -258,7 +271,7 @@ __device__ __forceinline__ void march_tile(const KParams& P, StepCtl* ctl, int s
         if (STAGE == 1) {
             cC[0] = cur.x[0]; cC[1] = cur.x[1]; cC[2] = cur.x[2];
         } else if (in_dom) {
-            const double Bm = cur.remap ? cur.bsrc : bcell4(P, b0, n0, bp1, np1);
+            const double Bm = cur.remap ? cur.bsrc : bc4(b0, n0, bp1, np1);
             stage1_cell<S>(P, cur.x, cur.r, Bm, dt_d, dt_s, cC);
             if (fxc) cC[1] = (S)(-cC[1]);
             if (cur.fy) cC[2] = (S)(-cC[2]);
@@ -271,20 +284,20 @@ __device__ __forceinline__ void march_tile(const KParams& P, StepCtl* ctl, int s
         //    neighbours; bottom corner rows jr-1, jr)
         S Ly[3] = {S(0), S(0), S(0)}, Ry[3] = {S(0), S(0), S(0)};
         const double Bly = 0.5 * (bm1 + nm1), Bry = 0.5 * (b0 + n0);
-        const double bcq = bcell4(P, bm1, nm1, b0, n0);           // B_cell of row jr-1
+        const double bcq = bc4(bm1, nm1, b0, n0);                 // B_cell of row jr-1
         S Lx[3] = {S(0), S(0), S(0)}, Rx[3] = {S(0), S(0), S(0)}, Rxl[3] = {S(0), S(0), S(0)};
         const double Blx = 0.5 * (bm1 + b0), Brx = 0.5 * (nm1 + n0);
         if (FULL || r >= 1) {
             S cl[3], cr[3];
 #pragma unroll
             for (int v = 0; v < 3; ++v) { cl[v] = shfl_up1(cB[v]); cr[v] = shfl_dn1(cB[v]); }
-            if (P.rec == 1) {
+            if (rec_linear) {
 #if SFV_MARCH_BRANCHFREE
                 if (FULL || r >= 2) reconstruct_cell_nb(cA, cB, cC, P.theta, Bly, Bry, Ly, Ry);
                 reconstruct_cell_nb(cl, cB, cr, P.theta, Blx, Brx, Lx, Rx);
 #else
-                if (FULL || r >= 2) reconstruct_cell(cA, cB, cC, P.theta, Bly, Bry, Ly, Ry);
-                reconstruct_cell(cl, cB, cr, P.theta, Blx, Brx, Lx, Rx);
+                if (FULL || r >= 2) reconstruct_cell(cA, cB, cC, P.theta, Bly, Bry, Ly, Ry, FAST);
+                reconstruct_cell(cl, cB, cr, P.theta, Blx, Brx, Lx, Rx, FAST);
 #endif
             } else {
 #pragma unroll
@@ -331,9 +344,9 @@ __device__ __forceinline__ void march_tile(const KParams& P, StepCtl* ctl, int s
             for (int v = 0; v < 3; ++v) Fxp[v] = shfl_dn1(Fxm[v]);
 #pragma unroll
             for (int v = 0; v < 3; ++v) accx_n[v] = (S)(S(0) - P.inv_dx * (Fxp[v] - Fxm[v]));
-            if (P.bottom) {
+            if (bottom) {
                 const S hsum = Rx[0] + Lx[0];
-                accx_n[1] = (S)(accx_n[1] + -g * slope_div(Brx - Blx, P.dx, P.inv_dx, P.dx_pow2) * (hsum / 2.0));
+                accx_n[1] = (S)(accx_n[1] + -g * sdiv(Brx - Blx, P.dx, P.inv_dx, P.dx_pow2) * (hsum / 2.0));
             }
         }
         // 4. finish row p = jr-2: y part on top of its x part, then sources
@@ -343,9 +356,9 @@ __device__ __forceinline__ void march_tile(const KParams& P, StepCtl* ctl, int s
             S acc[3] = {accx[0], accx[1], accx[2]};
 #pragma unroll
             for (int v = 0; v < 3; ++v) acc[v] = (S)(acc[v] - P.inv_dy * (Fy[v] - Fyp[v]));
-            if (P.bottom) {
+            if (bottom) {
                 const S hsum = Ryp[0] + Lyp0;
-                acc[2] = (S)(acc[2] + -g * slope_div(Bryp - Blyp, P.dy, P.inv_dy, P.dy_pow2) * (hsum / 2.0));
+                acc[2] = (S)(acc[2] + -g * sdiv(Bryp - Blyp, P.dy, P.inv_dy, P.dy_pow2) * (hsum / 2.0));
             }
             acc[0] = apply_sources(P, ctl, acc[0], k);
             if (STAGE == 1) {
@@ -425,7 +438,7 @@ __device__ __forceinline__ void march_tile(const KParams& P, StepCtl* ctl, int s
             const int ci = (corner & 1) ? nxT - gc + rr % gc : rr % gc;
             const int cj = (corner >> 1) ? nyT - gc + rr / gc : rr / gc;
             const size_t k = (size_t)ci + (size_t)nxT * cj;
-            const double Bm = bcell4(P, Bload(P, B, ci, cj), Bload(P, B, ci + 1, cj), Bload(P, B, ci, cj + 1),
+            const double Bm = bc4(Bload(P, B, ci, cj), Bload(P, B, ci + 1, cj), Bload(P, B, ci, cj + 1),
                                      Bload(P, B, ci + 1, cj + 1));
             const S src = apply_sources(P, ctl, S(0), k);
             const S x[3] = {U0[k], U0[n + k], U0[2 * n + k]};
@@ -471,7 +484,7 @@ __device__ __forceinline__ int xcd_tile(int L, int N) {
 
 // 1D grid of strips x seggroups blocks; a block is 4 waves on 4 consecutive
 // segments of one strip.
-template <class S, int STAGE>
+template <class S, int STAGE, bool FAST>
 __global__ void __launch_bounds__(MW* MWAVES)
 __attribute__((amdgpu_waves_per_eu(STAGE == 1 ? SFV_MARCH_MIN_WAVES1 : SFV_MARCH_MIN_WAVES)))
 k_march(KParams P, StepCtl* ctl, int slot, int my, int strips, int seggroups, const S* __restrict__ U0,
@@ -482,7 +495,7 @@ k_march(KParams P, StepCtl* ctl, int slot, int my, int strips, int seggroups, co
 #else
     const int strip = t / seggroups, sg = t % seggroups;
 #endif
-    march_tile<S, STAGE>(P, ctl, slot, my, strip, sg * MWAVES + threadIdx.x / MW, U0, RHS1, OUT, B);
+    march_tile<S, STAGE, FAST>(P, ctl, slot, my, strip, sg * MWAVES + threadIdx.x / MW, U0, RHS1, OUT, B);
 }
 
 // Batched stage kernel: a persistent grid (a multiple of 8 workgroups) walks
@@ -526,9 +539,9 @@ int march_rows_per_wave(const KParams& P, int stage) {
         for (int s = 0; s < 2; ++s) {
             int blocks = 0;
             const hipError_t e = s == 0 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                                              &blocks, k_march<double, 1>, MW * MWAVES, 0)
+                                              &blocks, k_march<double, 1, false>, MW * MWAVES, 0)
                                         : hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                                              &blocks, k_march<double, 2>, MW * MWAVES, 0);
+                                              &blocks, k_march<double, 2, false>, MW * MWAVES, 0);
             wpc[s] = (e == hipSuccess && blocks > 0) ? blocks * MWAVES : 2 * MWAVES;
         }
     }
@@ -545,6 +558,12 @@ int march_rows_per_wave(const KParams& P, int stage) {
     return best;
 }
 
+// The configurations the FAST kernels cover.
+bool march_fast(const KParams& P) {
+    return SFV_MARCH_FAST && P.rec == 1 && std::isfinite(P.theta) && P.theta != 0.0 && P.bottom && !P.bcell_1d &&
+           P.dx_pow2 && P.dy_pow2;
+}
+
 template <class S>
 hipError_t launch_march(hipStream_t st, const KParams& P, StepCtl* ctl, int slot, int stage, const S* U0,
                         const S* RHS1, S* OUT, const double* B) {
@@ -552,12 +571,22 @@ hipError_t launch_march(hipStream_t st, const KParams& P, StepCtl* ctl, int slot
     const int segs = (P.ny + my - 1) / my;
     const int strips = (P.nx + MX - 1) / MX, seggroups = (segs + MWAVES - 1) / MWAVES;
     const dim3 grid(strips * seggroups);
-    if (stage == 1)
-        hipLaunchKernelGGL((k_march<S, 1>), grid, dim3(MW * MWAVES), 0, st, P, ctl, slot, my, strips, seggroups, U0,
-                           RHS1, OUT, B);
-    else
-        hipLaunchKernelGGL((k_march<S, 2>), grid, dim3(MW * MWAVES), 0, st, P, ctl, slot, my, strips, seggroups, U0,
-                           RHS1, OUT, B);
+    const dim3 block(MW * MWAVES);
+    if (march_fast(P)) {
+        if (stage == 1)
+            hipLaunchKernelGGL((k_march<S, 1, true>), grid, block, 0, st, P, ctl, slot, my, strips, seggroups, U0, RHS1,
+                               OUT, B);
+        else
+            hipLaunchKernelGGL((k_march<S, 2, true>), grid, block, 0, st, P, ctl, slot, my, strips, seggroups, U0, RHS1,
+                               OUT, B);
+    } else {
+        if (stage == 1)
+            hipLaunchKernelGGL((k_march<S, 1, false>), grid, block, 0, st, P, ctl, slot, my, strips, seggroups, U0,
+                               RHS1, OUT, B);
+        else
+            hipLaunchKernelGGL((k_march<S, 2, false>), grid, block, 0, st, P, ctl, slot, my, strips, seggroups, U0,
+                               RHS1, OUT, B);
+    }
     return hipPeekAtLastError();
 }
 template <class S>

@@ -129,8 +129,8 @@ __device__ __forceinline__ double central_upwind(int d, const S m[3], const S q[
 // left and right face states (h, hu, hv) — variable 1 becomes a depth.
 template <class S>
 __device__ __forceinline__ void reconstruct_cell(const S l[3], const S c[3], const S r[3], double theta,
-                                                 double Bl, double Br, S L[3], S R[3]) {
-    const bool regular = __builtin_isfinite(theta) && theta != 0.0;
+                                                 double Bl, double Br, S L[3], S R[3], bool known_regular = false) {
+    const bool regular = known_regular || (__builtin_isfinite(theta) && theta != 0.0);
     double slope[3];
 #pragma unroll
     for (int v = 0; v < 3; ++v) {
