@@ -130,14 +130,15 @@ __device__ __forceinline__ double Bload(const KParams& P, const double* __restri
 
 // Stage-1 result of one cell (rungekutta.jl:38-41 with the Float64 dt,
 // friction.jl:80-84 from U0 with the stored dt, timestepper.jl:53-63).
-template <class S>
+// FAST: the friction law is none or bh2021 (march_fast).
+template <class S, bool FAST = false>
 __device__ __forceinline__ void stage1_cell(const KParams& P, const S x[3], const S r[3], double Bm, double dt_d,
                                             S dt_s, S o[3]) {
     o[0] = (S)(x[0] + dt_d * r[0]);
     o[1] = (S)(x[1] + dt_d * r[1]);
     o[2] = (S)(x[2] + dt_d * r[2]);
     if (P.friction) {
-        const double fs = friction_scalar(P.friction, P.Cz, P.phys.kappa, x[0], x[1], x[2], Bm);
+        const double fs = friction_scalar(FAST ? 1 : P.friction, P.Cz, P.phys.kappa, x[0], x[1], x[2], Bm);
         apply_friction(o, dt_s, fs);
     }
     if (o[0] - Bm < P.phys.cutoff) {
@@ -152,8 +153,9 @@ __device__ __forceinline__ void stage1_cell(const KParams& P, const S x[3], cons
 // the block index) and the batched kernel (from the batch's tile table).
 // Has no barriers: every wave is independent.
 // FAST: the configuration is known to have linear reconstruction with a
-// finite nonzero θ, the bottom source, the 2D cell-bottom lookup and
-// power-of-two spacings (march_fast), so those run-time checks fold away.
+// finite nonzero θ, the bottom source, the 2D cell-bottom lookup,
+// power-of-two spacings, no friction or bh2021, no infiltration and a rain
+// rate uniform in space (march_fast), so those run-time checks fold away.
 template <class S, int STAGE, bool FAST = false>
 __device__ __forceinline__ void march_tile(const KParams& P, StepCtl* ctl, int slot, int my, int strip, int seg,
                                            const S* __restrict__ U0, const S* __restrict__ RHS1,
@@ -174,6 +176,11 @@ __device__ __forceinline__ void march_tile(const KParams& P, StepCtl* ctl, int s
         return FAST ? 0.25 * (((a + b) + c) + d) : bcell4(P, a, b, c, d);
     };
     auto sdiv = [&](double x, double dd, double inv, int pow2) { return FAST ? x * inv : slope_div(x, dd, inv, pow2); };
+    // FAST: no infiltration and a rain rate that is the same for every cell
+    auto sources = [&](S w, size_t k) -> S {
+        if (FAST) return P.rain_kind ? (S)(w + ctl->rain_now) : w;
+        return apply_sources(P, ctl, w, k);
+    };
 
     double dt_d = 0.0;
     S dt_s = S(0);
@@ -272,7 +279,7 @@ __device__ __forceinline__ void march_tile(const KParams& P, StepCtl* ctl, int s
             cC[0] = cur.x[0]; cC[1] = cur.x[1]; cC[2] = cur.x[2];
         } else if (in_dom) {
             const double Bm = cur.remap ? cur.bsrc : bc4(b0, n0, bp1, np1);
-            stage1_cell<S>(P, cur.x, cur.r, Bm, dt_d, dt_s, cC);
+            stage1_cell<S, FAST>(P, cur.x, cur.r, Bm, dt_d, dt_s, cC);
             if (fxc) cC[1] = (S)(-cC[1]);
             if (cur.fy) cC[2] = (S)(-cC[2]);
         } else {
@@ -360,7 +367,7 @@ __device__ __forceinline__ void march_tile(const KParams& P, StepCtl* ctl, int s
                 const S hsum = Ryp[0] + Lyp0;
                 acc[2] = (S)(acc[2] + -g * sdiv(Bryp - Blyp, P.dy, P.inv_dy, P.dy_pow2) * (hsum / 2.0));
             }
-            acc[0] = apply_sources(P, ctl, acc[0], k);
+            acc[0] = sources(acc[0], k);
             if (STAGE == 1) {
                 OUT[k] = acc[0];
                 OUT[n + k] = acc[1];
@@ -374,7 +381,7 @@ __device__ __forceinline__ void march_tile(const KParams& P, StepCtl* ctl, int s
                 o[1] = (S)(0.5 * (U0[n + k] + o[1]));
                 o[2] = (S)(0.5 * (U0[2 * n + k] + o[2]));
                 if (P.friction) {
-                    const double fs = friction_scalar(P.friction, P.Cz, P.phys.kappa, cA[0], cA[1], cA[2], bcp);
+                    const double fs = friction_scalar(FAST ? 1 : P.friction, P.Cz, P.phys.kappa, cA[0], cA[1], cA[2], bcp);
                     apply_friction(o, dt_s, fs);
                 }
                 const bool dry = o[0] - bcp < P.phys.cutoff;
@@ -440,18 +447,18 @@ __device__ __forceinline__ void march_tile(const KParams& P, StepCtl* ctl, int s
             const size_t k = (size_t)ci + (size_t)nxT * cj;
             const double Bm = bc4(Bload(P, B, ci, cj), Bload(P, B, ci + 1, cj), Bload(P, B, ci, cj + 1),
                                      Bload(P, B, ci + 1, cj + 1));
-            const S src = apply_sources(P, ctl, S(0), k);
+            const S src = sources(S(0), k);
             const S x[3] = {U0[k], U0[n + k], U0[2 * n + k]};
             const S rs[3] = {src, S(0), S(0)};
             S u1[3];
-            stage1_cell<S>(P, x, rs, Bm, dt_d, dt_s, u1);
+            stage1_cell<S, FAST>(P, x, rs, Bm, dt_d, dt_s, u1);
             S o[3];
 #pragma unroll
             for (int v = 0; v < 3; ++v) o[v] = (S)(u1[v] + dt_s * rs[v]);
 #pragma unroll
             for (int v = 0; v < 3; ++v) o[v] = (S)(0.5 * (x[v] + o[v]));
             if (P.friction) {
-                const double fs = friction_scalar(P.friction, P.Cz, P.phys.kappa, u1[0], u1[1], u1[2], Bm);
+                const double fs = friction_scalar(FAST ? 1 : P.friction, P.Cz, P.phys.kappa, u1[0], u1[1], u1[2], Bm);
                 apply_friction(o, dt_s, fs);
             }
             if (o[0] - Bm < P.phys.cutoff) {
@@ -561,7 +568,8 @@ int march_rows_per_wave(const KParams& P, int stage) {
 // The configurations the FAST kernels cover.
 bool march_fast(const KParams& P) {
     return SFV_MARCH_FAST && P.rec == 1 && std::isfinite(P.theta) && P.theta != 0.0 && P.bottom && !P.bcell_1d &&
-           P.dx_pow2 && P.dy_pow2;
+           P.dx_pow2 && P.dy_pow2 && (P.friction == 0 || P.friction == 1) && P.infil_kind == 0 &&
+           (P.rain_kind == 0 || P.rain_kind == 1 || P.rain_kind == 2);
 }
 
 template <class S>
