@@ -241,12 +241,12 @@ int BatchEngine::capture(std::string& why) {
     for (int s = 0; s < graph_steps_ && e == hipSuccess; ++s) {
         const int lp = s & 1;
         if (precision_ == SFV_F64) {
-            e = launch_march_batch<double>(stream_, 1, d_mem_, d_hdr_, lp, blocks_stage_);
-            if (e == hipSuccess) e = launch_march_batch<double>(stream_, 2, d_mem_, d_hdr_, lp, blocks_stage_);
+            e = launch_march_batch<double>(stream_, 1, d_mem_, d_hdr_, lp, blocks_stage_, fast_);
+            if (e == hipSuccess) e = launch_march_batch<double>(stream_, 2, d_mem_, d_hdr_, lp, blocks_stage_, fast_);
             if (e == hipSuccess) e = launch_bc_batch<double>(stream_, d_mem_, d_hdr_, lp, blocks_bc_);
         } else {
-            e = launch_march_batch<float>(stream_, 1, d_mem_, d_hdr_, lp, blocks_stage_);
-            if (e == hipSuccess) e = launch_march_batch<float>(stream_, 2, d_mem_, d_hdr_, lp, blocks_stage_);
+            e = launch_march_batch<float>(stream_, 1, d_mem_, d_hdr_, lp, blocks_stage_, fast_);
+            if (e == hipSuccess) e = launch_march_batch<float>(stream_, 2, d_mem_, d_hdr_, lp, blocks_stage_, fast_);
             if (e == hipSuccess) e = launch_bc_batch<float>(stream_, d_mem_, d_hdr_, lp, blocks_bc_);
         }
     }
@@ -270,7 +270,17 @@ int BatchEngine::capture(std::string& why) {
 
 int BatchEngine::replay_begin(int k, std::string& why) {
     int rc;
-    if (table_dirty_ && (rc = upload_table(why))) return rc;
+    if (table_dirty_) {
+        // the graph holds the specialised kernels only while every member fits them
+        bool fast = !rows_.empty();
+        for (int s : rows_) fast = fast && march_fast(slot_handle_[s]->P);
+        if (graph_ && fast != fast_) {
+            (void)hipGraphExecDestroy(graph_);
+            graph_ = nullptr;
+        }
+        fast_ = fast;
+        if ((rc = upload_table(why))) return rc;
+    }
     if (!graph_ && (rc = capture(why))) return rc;
     for (int r = 0; r < k; ++r) ETRY(hipGraphLaunch(graph_, stream_));
     ETRY(hipMemcpyAsync(h_ctl_, d_ctl_, sizeof(StepCtl) * capacity_, hipMemcpyDeviceToHost, stream_));

@@ -82,6 +82,7 @@ private:
     StepCtl* h_ctl_ = nullptr;
     hipGraphExec_t graph_ = nullptr;
     bool table_dirty_ = true;
+    bool fast_ = false;           // the captured graph uses the specialised march kernels
     int cus_ = 256;
     std::vector<int> rows_;                  // slot of each table row
     std::vector<sfv_handle*> slot_handle_;   // nullptr: free slot

@@ -46,9 +46,12 @@ hipError_t launch_prepare(hipStream_t st, const KParams& P, StepCtl* ctl, int sl
                           int until_endtime, int precision);
 // Batched kernels over a device member table (BatchMember[hdr->members]).
 // Stage kernels are persistent: `blocks` workgroups walk the batch's tiles.
+// fast: every member satisfies march_fast (the specialised kernels).
 template <class S>
 hipError_t launch_march_batch(hipStream_t st, int stage, const BatchMember* mem, const BatchHeader* hdr, int lp,
-                              int blocks);
+                              int blocks, bool fast);
+// The configurations the specialised march kernels cover.
+bool march_fast(const KParams& P);
 template <class S>
 hipError_t launch_bc_batch(hipStream_t st, const BatchMember* mem, const BatchHeader* hdr, int lp, int blocks);
 hipError_t launch_prepare_batch(hipStream_t st, const BatchMember* mem, const BatchHeader* hdr, int members);

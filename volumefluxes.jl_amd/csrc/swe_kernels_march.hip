@@ -509,7 +509,7 @@ k_march(KParams P, StepCtl* ctl, int slot, int my, int strips, int seggroups, co
 // the tiles of every member.  The tile order is split into 8 contiguous
 // chunks, one per XCD, and the workgroups of XCD x (blockIdx mod 8 == x) walk
 // chunk x, so a member's neighbouring tiles share that XCD's L2.
-template <class S, int STAGE>
+template <class S, int STAGE, bool FAST>
 __global__ void __launch_bounds__(MW* MWAVES) __attribute__((amdgpu_waves_per_eu(SFV_MARCH_MIN_WAVES)))
 k_march_batch(const BatchMember* __restrict__ mem, const BatchHeader* __restrict__ hdr, int lp) {
     const int members = hdr->members, tiles = hdr->tiles;
@@ -525,9 +525,9 @@ k_march_batch(const BatchMember* __restrict__ mem, const BatchHeader* __restrict
         const int seg = sg * MWAVES + threadIdx.x / MW;
         const S* U0 = (const S*)M.U[cur];
         if (STAGE == 1)
-            march_tile<S, 1>(M.P, M.ctl, slot, M.my, strip, seg, U0, nullptr, (S*)M.RHS, M.B);
+            march_tile<S, 1, FAST>(M.P, M.ctl, slot, M.my, strip, seg, U0, nullptr, (S*)M.RHS, M.B);
         else
-            march_tile<S, 2>(M.P, M.ctl, slot, M.my, strip, seg, U0, (const S*)M.RHS, (S*)M.U[cur ^ 1], M.B);
+            march_tile<S, 2, FAST>(M.P, M.ctl, slot, M.my, strip, seg, U0, (const S*)M.RHS, (S*)M.U[cur ^ 1], M.B);
     }
 }
 
@@ -599,12 +599,16 @@ hipError_t launch_march(hipStream_t st, const KParams& P, StepCtl* ctl, int slot
 }
 template <class S>
 hipError_t launch_march_batch(hipStream_t st, int stage, const BatchMember* mem, const BatchHeader* hdr, int lp,
-                              int blocks) {
+                              int blocks, bool fast) {
     if (blocks < N_XCD || blocks % N_XCD) return hipErrorInvalidValue;
-    if (stage == 1)
-        hipLaunchKernelGGL((k_march_batch<S, 1>), dim3(blocks), dim3(MW * MWAVES), 0, st, mem, hdr, lp);
-    else
-        hipLaunchKernelGGL((k_march_batch<S, 2>), dim3(blocks), dim3(MW * MWAVES), 0, st, mem, hdr, lp);
+    const dim3 grid(blocks), block(MW * MWAVES);
+    if (fast) {
+        if (stage == 1) hipLaunchKernelGGL((k_march_batch<S, 1, true>), grid, block, 0, st, mem, hdr, lp);
+        else hipLaunchKernelGGL((k_march_batch<S, 2, true>), grid, block, 0, st, mem, hdr, lp);
+    } else {
+        if (stage == 1) hipLaunchKernelGGL((k_march_batch<S, 1, false>), grid, block, 0, st, mem, hdr, lp);
+        else hipLaunchKernelGGL((k_march_batch<S, 2, false>), grid, block, 0, st, mem, hdr, lp);
+    }
     return hipPeekAtLastError();
 }
 
@@ -612,16 +616,20 @@ int march_batch_blocks_per_cu(int precision, int stage) {
     int blocks = 0;
     hipError_t e;
     if (precision == 0)
-        e = stage == 1 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_march_batch<double, 1>, MW * MWAVES, 0)
-                       : hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_march_batch<double, 2>, MW * MWAVES, 0);
+        e = stage == 1
+                ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_march_batch<double, 1, false>, MW * MWAVES, 0)
+                : hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_march_batch<double, 2, false>, MW * MWAVES, 0);
     else
-        e = stage == 1 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_march_batch<float, 1>, MW * MWAVES, 0)
-                       : hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_march_batch<float, 2>, MW * MWAVES, 0);
+        e = stage == 1
+                ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_march_batch<float, 1, false>, MW * MWAVES, 0)
+                : hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_march_batch<float, 2, false>, MW * MWAVES, 0);
     return (e == hipSuccess && blocks > 0) ? blocks : 2;
 }
 
-template hipError_t launch_march_batch<double>(hipStream_t, int, const BatchMember*, const BatchHeader*, int, int);
-template hipError_t launch_march_batch<float>(hipStream_t, int, const BatchMember*, const BatchHeader*, int, int);
+template hipError_t launch_march_batch<double>(hipStream_t, int, const BatchMember*, const BatchHeader*, int, int,
+                                               bool);
+template hipError_t launch_march_batch<float>(hipStream_t, int, const BatchMember*, const BatchHeader*, int, int,
+                                              bool);
 template hipError_t launch_march<double>(hipStream_t, const KParams&, StepCtl*, int, int, const double*,
                                          const double*, double*, const double*);
 template hipError_t launch_march<float>(hipStream_t, const KParams&, StepCtl*, int, int, const float*, const float*,
