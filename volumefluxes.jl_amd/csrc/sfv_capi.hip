@@ -264,16 +264,23 @@ int tune_march_rows(sfv_handle* h, std::string& why) {
     if (e != hipSuccess) return bail(e);
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    // the fastest, unless the build default is within 3% of it: the timing
-    // starts from the current state (often a state at rest), where heights
-    // close in time can order differently than in the developed flow, and the
-    // default is the measured best for grids that fill the device
+    // the fastest, unless the build default is close to it: the timing starts
+    // from the current state (often a state at rest), where heights close in
+    // time can order differently than in the developed flow.  The default is
+    // the measured best for grids whose waves fill the device's resident wave
+    // slots (3 per SIMD), so there it must lose by 10% to be replaced; for
+    // smaller grids, where the serial path of a wave decides, by 3%.
+    int cus = 256;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, h->device) != hipSuccess || cus <= 0)
+        cus = 256;
+    const long waves = (long)((h->cfg.nx + MARCH_COLS - 1) / MARCH_COLS) * ((h->cfg.ny + fallback - 1) / fallback);
+    const double margin = waves >= 12L * cus ? 1.10 : 1.03;
     size_t fast = 0;
     for (size_t c = 1; c < cand.size(); ++c)
         if (best[c] < best[fast]) fast = c;
     int pick = cand[fast];
     for (size_t c = 0; c < cand.size(); ++c)
-        if (cand[c] == fallback && best[c] <= 1.03 * best[fast]) pick = fallback;
+        if (cand[c] == fallback && best[c] <= margin * best[fast]) pick = fallback;
     h->march_rows = pick;
     h->P.rows = pick;
     if (const char* dbg = std::getenv("SFV_TUNE_REPORT"); dbg && dbg[0] == '1') {

@@ -43,6 +43,7 @@ namespace sfv {
 //                         segments fastest (vertical halos stay in one L2);
 //                         2 = same, strips fastest; 0 = hardware order
 //   SFV_MARCH_UNROLL      unroll factor of the row loop
+//   SFV_MARCH_KEEPU0      stage 2 keeps U0 of the last two rows in registers instead of reloading it
 //   SFV_MARCH_FAST        specialised kernels for the common configuration (march_fast)
 #ifndef SFV_MARCH_PREFETCH
 #define SFV_MARCH_PREFETCH 0
@@ -57,7 +58,7 @@ namespace sfv {
 #define SFV_MARCH_ROWS 12
 #endif
 #ifndef SFV_MARCH_MIN_WAVES
-#define SFV_MARCH_MIN_WAVES 1
+#define SFV_MARCH_MIN_WAVES 3
 #endif
 #ifndef SFV_MARCH_MIN_WAVES1
 #define SFV_MARCH_MIN_WAVES1 SFV_MARCH_MIN_WAVES
@@ -67,6 +68,9 @@ namespace sfv {
 #endif
 #ifndef SFV_MARCH_FAST
 #define SFV_MARCH_FAST 1
+#endif
+#ifndef SFV_MARCH_KEEPU0
+#define SFV_MARCH_KEEPU0 1
 #endif
 #ifndef SFV_MARCH_UNROLL
 #define SFV_MARCH_UNROLL 1
@@ -252,6 +256,9 @@ __device__ __forceinline__ void march_tile(const KParams& P, StepCtl* ctl, int s
     if (SFV_MARCH_PREFETCH) fetch(j0 - 2, cur);
 
     S cA[3] = {S(0), S(0), S(0)}, cB[3] = {S(0), S(0), S(0)};
+    // stage 2 with SFV_MARCH_KEEPU0: U0 of rows jr-2, jr-1 as loaded, for the
+    // RK combination of row jr-2 (instead of loading it again)
+    S xA[3] = {S(0), S(0), S(0)}, xB[3] = {S(0), S(0), S(0)};
     S Lyp0 = S(0), Ryp[3] = {S(0), S(0), S(0)};   // previous row's y faces (left: depth only)
     double Blyp = 0.0, Bryp = 0.0;                 // its y face bottoms
     double Fyp[3] = {0.0, 0.0, 0.0};               // y flux below that row
@@ -377,9 +384,16 @@ __device__ __forceinline__ void march_tile(const KParams& P, StepCtl* ctl, int s
                 S o[3];
 #pragma unroll
                 for (int v = 0; v < 3; ++v) o[v] = (S)(cA[v] + dt_s * acc[v]);
-                o[0] = (S)(0.5 * (U0[k] + o[0]));
-                o[1] = (S)(0.5 * (U0[n + k] + o[1]));
-                o[2] = (S)(0.5 * (U0[2 * n + k] + o[2]));
+                if (SFV_MARCH_KEEPU0) {
+                    // output rows are interior, so xA holds U0 at (i, p) itself
+                    o[0] = (S)(0.5 * (xA[0] + o[0]));
+                    o[1] = (S)(0.5 * (xA[1] + o[1]));
+                    o[2] = (S)(0.5 * (xA[2] + o[2]));
+                } else {
+                    o[0] = (S)(0.5 * (U0[k] + o[0]));
+                    o[1] = (S)(0.5 * (U0[n + k] + o[1]));
+                    o[2] = (S)(0.5 * (U0[2 * n + k] + o[2]));
+                }
                 if (P.friction) {
                     const double fs = friction_scalar(FAST ? 1 : P.friction, P.Cz, P.phys.kappa, cA[0], cA[1], cA[2], bcp);
                     apply_friction(o, dt_s, fs);
@@ -398,6 +412,7 @@ __device__ __forceinline__ void march_tile(const KParams& P, StepCtl* ctl, int s
 #pragma unroll
         for (int v = 0; v < 3; ++v) {
             cA[v] = cB[v]; cB[v] = cC[v];
+            if (STAGE == 2 && SFV_MARCH_KEEPU0) { xA[v] = xB[v]; xB[v] = cur.x[v]; }
             Ryp[v] = Ry[v];
             Fyp[v] = Fy[v];
             accx[v] = accx_n[v];
