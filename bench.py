@@ -73,7 +73,7 @@ def cpu_baseline(grid, system, realtype, U0, B, steps_per_run: int, runs: int, t
     on this host: 1 warm-up step, then the median of `runs` timed runs of
     `steps_per_run` steps, OpenMP over grid rows on `threads` threads."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import oracle as O  # the checker / baseline, never the measured product
+    import oracle as O  # the reference restatement: CPU baseline only, never the measured product
     O.set_threads(threads)
     cfg = VF.build_config(system, VF.RungeKutta2(), grid, 0.25, C.SFV_F64 if realtype == np.float64 else 1)
     o = O.Oracle2D(cfg, B)
@@ -94,7 +94,7 @@ def cpu_baseline(grid, system, realtype, U0, B, steps_per_run: int, runs: int, t
 def oracle_first_step(grid, system, realtype, U0, B):
     """The oracle's state after one step from U0 (the parity probe's reference)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import oracle as O  # the checker
+    import oracle as O  # the reference restatement, used only to check results
     O.set_threads(os.cpu_count() or 1)
     cfg = VF.build_config(system, VF.RungeKutta2(), grid, 0.25, C.SFV_F64 if realtype == np.float64 else 1)
     o = O.Oracle2D(cfg, B)
@@ -405,7 +405,7 @@ def adjoint_bench(seed: int, n: int = 512, steps: int = 200, checkpoint_every: i
     }
     try:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
-        import oracle as O  # the checker
+        import oracle as O  # the reference restatement, used only to check results
         dB = np.asfortranarray(rng.normal(size=B.shape) * bump)
         cfg = VF.build_config(system_for(B, manning ** 2), VF.RungeKutta2(), grid, 0.25, C.SFV_F64)
         t1 = time.perf_counter()

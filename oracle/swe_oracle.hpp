@@ -8,7 +8,7 @@
 // face buffers, wavespeed scalar field, n_substeps+1 state volumes) and its
 // evaluation order, so that it is bit-reproducible against the reference in
 // Float64.  Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline
-// leg use it, as the checker and the timed CPU baseline.
+// leg use it, to check results and as the timed CPU baseline.
 //
 // Storage type S is double (Float64 backend) or float (make_cpu_backend(Float32)).
 // Values read from S-typed buffers stay S and parameters stay double, so C++'s
