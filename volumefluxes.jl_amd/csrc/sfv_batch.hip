@@ -55,11 +55,29 @@ int BatchEngine::init(int precision, int capacity, std::string& why) {
     if (hipGetDeviceProperties(&prop, device_) == hipSuccess && prop.multiProcessorCount > 0)
         cus = prop.multiProcessorCount;
     cus_ = cus;
-    const int per_cu = std::min(march_batch_blocks_per_cu(precision, 1), march_batch_blocks_per_cu(precision, 2));
-    blocks_stage_ = std::max(8, (cus * per_cu + 7) / 8 * 8);
+    per_cu_ = std::min(march_batch_blocks_per_cu(precision, 1), march_batch_blocks_per_cu(precision, 2));
+    size_grid();
     blocks_bc_ = std::max(1, cus);
     return grow(std::max(1, capacity), why);
 }
+
+void BatchEngine::size_grid() {
+    const int per = share_ == Share::ALONE ? per_cu_ : share_ == Share::HOST ? std::max(1, per_cu_ - 1) : 1;
+    blocks_stage_ = std::max(8, (cus_ * per + 7) / 8 * 8);
+}
+
+void BatchEngine::set_share(Share s) {
+    if (s == share_) return;
+    share_ = s;
+    size_grid();
+    if (graph_) {
+        (void)hipGraphExecDestroy(graph_);
+        graph_ = nullptr;
+    }
+    table_dirty_ = true;   // segment heights and the priority flag follow the grid
+}
+
+bool BatchEngine::replay_ready() const { return hipStreamQuery(stream_) != hipErrorNotReady; }
 
 int BatchEngine::grow(int capacity, std::string& why) {
     BatchMember* dm = nullptr;
@@ -225,7 +243,7 @@ int BatchEngine::upload_table(std::string& why) {
     h_hdr_->members = (int)rows_.size();
     h_hdr_->tiles = tiles;
     h_hdr_->bc_lines = lines;
-    h_hdr_->pad = 0;
+    h_hdr_->prio = share_ == Share::GUEST ? 1 : 0;
     if (!rows_.empty())
         ETRY(hipMemcpyAsync(d_mem_, h_mem_, sizeof(BatchMember) * rows_.size(), hipMemcpyHostToDevice, stream_));
     ETRY(hipMemcpyAsync(d_hdr_, h_hdr_, sizeof(BatchHeader), hipMemcpyHostToDevice, stream_));

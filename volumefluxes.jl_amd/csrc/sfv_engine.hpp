@@ -20,6 +20,13 @@ namespace sfv {
 // Segment height of batch members when the batch fills the device.
 constexpr int SFV_BATCH_DEFAULT_ROWS = 12;
 
+// How an engine's persistent grid shares the device with another engine's.
+//   ALONE: the grid fills every resident block slot.
+//   HOST:  one block slot per CU is left free for a GUEST engine.
+//   GUEST: one block per CU, and the waves raise their issue priority; for
+//          small, latency-bound grids stepping beside a HOST engine.
+enum class Share { ALONE, HOST, GUEST };
+
 struct RunParams {
     double t0 = 0.0;
     double endtime = 0.0;         // NaN / unused when until_endtime = 0
@@ -53,6 +60,13 @@ public:
     // not be called (they use the table and the step-control mirror).
     int replay_begin(int k, std::string& why);
     int replay_end(std::string& why);
+    // Without waiting: has the work queued by replay_begin finished?  (Also
+    // true after a device error, which replay_end then reports.)
+    bool replay_ready() const;
+    // The grid's share of the device; takes effect at the next replay_begin
+    // (the graph is recaptured when it changes).  Results do not depend on it.
+    // Not to be called between replay_begin and replay_end.
+    void set_share(Share s);
 
     int members() const { return (int)rows_.size(); }
     // After replay: has the member in `slot` finished its run?
@@ -67,12 +81,15 @@ private:
     int upload_table(std::string& why);
     void choose_rows();
     int capture(std::string& why);
+    void size_grid();
 
     int precision_ = 0;
     int device_ = 0;
     int capacity_ = 0;
     int graph_steps_ = 8;         // even: launch parity is 0 at every replay start
     int blocks_stage_ = 0, blocks_bc_ = 0;
+    int per_cu_ = 1;              // resident stage-kernel blocks per CU
+    Share share_ = Share::ALONE;
     hipStream_t stream_ = nullptr;
     BatchMember* d_mem_ = nullptr;
     BatchHeader* d_hdr_ = nullptr;

@@ -153,11 +153,13 @@ constexpr int64_t SMALL_GRID_CELLS = 1 << 18;
 // The worker keeps two batch engines.  Large grids step in one, replayed once
 // per loop (8 steps of every member), and while that replay runs the worker
 // downloads finished results and creates the newly admitted requests.  Small
-// grids, whose steps are short and latency-bound, step in the other, replayed
-// k times per loop between the large-grid replays, with k chosen so that the
-// small grids take about half of the device time: a small request that
-// needs many steps then is not held to the step rate of the large ones.
-// Membership changes happen between replays.
+// grids, whose steps are short and latency-bound, step in the other, which
+// replays concurrently with the large-grid replay in flight: the large
+// engine then leaves one block slot per CU free (Share::HOST) and the small
+// one runs one block per CU at raised wave priority (Share::GUEST), so a
+// small request that needs many steps advances at close to its solo step
+// rate while the large grids fill the rest of the device.  Alone, either
+// engine fills the device.  Membership changes happen between replays.
 void sfv_server::work() {
     std::string why;
     sfv::BatchEngine engs[2];
@@ -188,7 +190,7 @@ void sfv_server::work() {
     std::vector<Request*> to_retire;   // finished, removed from their engine, result not yet downloaded
     std::vector<Request*> created;     // handle made, not yet in an engine
     bool large_in_flight = false;
-    double t_large = 0, large_s = 0, small_s = 0;   // replay durations (moving averages)
+    double t_large = 0;                // start of the large-grid replay in flight
     auto fail_all = [&](int e, int code) {
         std::lock_guard<std::mutex> lk(mu);
         for (Request* r : run[e]) {
@@ -245,7 +247,7 @@ void sfv_server::work() {
                 admit.push_back(r);
             }
         }
-        // 2. host work, overlapping the large-grid replay in flight
+        // 2. host work, overlapping the replays in flight
         if (!to_retire.empty()) {
             const double t0 = now();
             for (Request* r : to_retire) {
@@ -288,12 +290,12 @@ void sfv_server::work() {
             std::lock_guard<std::mutex> lk(mu);
             host_admit_s += now() - t0;
         }
-        // 3. end the large-grid replay
-        if (large_in_flight) {
+        // 3. end the large-grid replay: once it has finished while small grids
+        //    step beside it, else by waiting for it
+        if (large_in_flight && (run[1].empty() || engs[0].replay_ready())) {
             large_in_flight = false;
             const int rc3 = engs[0].replay_end(why);
             const double dt = now() - t_large;
-            large_s = large_s > 0 ? 0.7 * large_s + 0.3 * dt : dt;
             {
                 std::lock_guard<std::mutex> lk(mu);
                 ++replays;
@@ -302,43 +304,27 @@ void sfv_server::work() {
             if (rc3) fail_all(0, rc3);
             else collect(0);
         }
-        // 4. the created requests join their engine (both are idle here)
-        for (Request* r : created) {
-            const int rc5 = engs[r->eng].add(r->h, run_params(r), &r->slot, why);
-            if (rc5) {
-                sfv_destroy(r->h);
-                r->h = nullptr;
-                std::lock_guard<std::mutex> lk(mu);
-                finish(r, REQ_FAILED, rc5, why);
-                cv_done.notify_all();
-            } else {
-                run[r->eng].push_back(r);
+        // 4. the created requests join their engine while it is idle
+        {
+            std::vector<Request*> later;
+            for (Request* r : created) {
+                if (r->eng == 0 && large_in_flight) { later.push_back(r); continue; }
+                const int rc5 = engs[r->eng].add(r->h, run_params(r), &r->slot, why);
+                if (rc5) {
+                    sfv_destroy(r->h);
+                    r->h = nullptr;
+                    std::lock_guard<std::mutex> lk(mu);
+                    finish(r, REQ_FAILED, rc5, why);
+                    cv_done.notify_all();
+                } else {
+                    run[r->eng].push_back(r);
+                }
             }
+            created.swap(later);
         }
-        created.clear();
-        // 5. small grids: k replays, then their finished requests leave
-        if (!run[1].empty()) {
-            // with large grids running, the small ones get about half the
-            // device time; alone, 4 replays (32 steps) per host round trip
-            int k = 4;
-            if (!run[0].empty())
-                k = large_s > 0 && small_s > 0 ? (int)std::min(64.0, std::max(1.0, std::round(large_s / small_s))) : 1;
-            const double t0 = now();
-            int rc6 = engs[1].replay_begin(k, why);
-            const int rc7 = engs[1].replay_end(why);
-            if (!rc6) rc6 = rc7;
-            const double dt = now() - t0;
-            small_s = small_s > 0 ? 0.7 * small_s + 0.3 * dt / k : dt / k;
-            {
-                std::lock_guard<std::mutex> lk(mu);
-                replays += k;
-                busy_s += dt;
-            }
-            if (rc6) fail_all(1, rc6);
-            else collect(1);
-        }
-        // 6. the next large-grid replay, left in flight
-        if (!run[0].empty()) {
+        // 5. the next large-grid replay, left in flight
+        if (!large_in_flight && !run[0].empty()) {
+            engs[0].set_share(run[1].empty() ? sfv::Share::ALONE : sfv::Share::HOST);
             t_large = now();
             const int rc8 = engs[0].replay_begin(1, why);
             if (rc8) {
@@ -347,6 +333,24 @@ void sfv_server::work() {
             } else {
                 large_in_flight = true;
             }
+        }
+        // 6. small grids: 2 replays (16 steps) beside the large-grid replay,
+        //    or 4 alone, then their finished requests leave
+        if (!run[1].empty()) {
+            const bool beside = large_in_flight;
+            engs[1].set_share(beside ? sfv::Share::GUEST : sfv::Share::ALONE);
+            const int k = beside ? 2 : 4;
+            const double t0 = now();
+            int rc6 = engs[1].replay_begin(k, why);
+            const int rc7 = engs[1].replay_end(why);
+            if (!rc6) rc6 = rc7;
+            {
+                std::lock_guard<std::mutex> lk(mu);
+                replays += k;
+                if (!beside) busy_s += now() - t0;
+            }
+            if (rc6) fail_all(1, rc6);
+            else collect(1);
         }
     }
     // stopped: release what is still on the device

@@ -97,7 +97,7 @@ struct BatchHeader {
     int members;
     int tiles;            // sum of strips * seggroups
     int bc_lines;         // sum of nx + ny
-    int pad;
+    int prio;             // nonzero: the batch kernels raise their waves' issue priority
 };
 
 // The member owning flattened index t of the batch order: the last member

@@ -116,11 +116,12 @@ __global__ void k_prepare(KParams P, StepCtl* ctl, int slot, double t, double en
 }
 
 // Batched boundary condition (last stage of a step): a grid-stride walk over
-// the boundary lines of every member.
+// the boundary lines of every member (issue priority as in k_march_batch).
 template <class S>
 __global__ void __launch_bounds__(256) k_bc_batch(const BatchMember* __restrict__ mem,
                                                   const BatchHeader* __restrict__ hdr, int lp) {
     const int members = hdr->members, lines = hdr->bc_lines;
+    if (hdr->prio) __builtin_amdgcn_s_setprio(3);
     for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < lines; t += gridDim.x * blockDim.x) {
         const int m = batch_find<1>(mem, members, t);
         const BatchMember& M = mem[m];

@@ -523,11 +523,15 @@ k_march(KParams P, StepCtl* ctl, int slot, int my, int strips, int seggroups, co
 // Batched stage kernel: a persistent grid (a multiple of 8 workgroups) walks
 // the tiles of every member.  The tile order is split into 8 contiguous
 // chunks, one per XCD, and the workgroups of XCD x (blockIdx mod 8 == x) walk
-// chunk x, so a member's neighbouring tiles share that XCD's L2.
+// chunk x, so a member's neighbouring tiles share that XCD's L2.  With
+// hdr->prio set (a batch of small grids stepping beside another batch) the
+// waves take the highest issue priority: their short serial chains issue
+// first and the other batch's waves fill the gaps.
 template <class S, int STAGE, bool FAST>
 __global__ void __launch_bounds__(MW* MWAVES) __attribute__((amdgpu_waves_per_eu(SFV_MARCH_MIN_WAVES)))
 k_march_batch(const BatchMember* __restrict__ mem, const BatchHeader* __restrict__ hdr, int lp) {
     const int members = hdr->members, tiles = hdr->tiles;
+    if (hdr->prio) __builtin_amdgcn_s_setprio(3);
     const int x = blockIdx.x % N_XCD, k = blockIdx.x / N_XCD, per = gridDim.x / N_XCD;
     const int q = tiles / N_XCD, r = tiles % N_XCD;
     const int c0 = x * q + min(x, r), c1 = c0 + q + (x < r ? 1 : 0);
