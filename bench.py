@@ -176,6 +176,7 @@ def ensemble_bench(members: int, steps: int, seed: int):
     picks = sorted(set([0, members // 3, members // 2, members - 1] + list(range(0, members, max(1, members // 8)))))
     same = all(np.array_equal(plain[k].get_full_state(), batched[k].get_full_state()) and plain[k].t == batched[k].t
                for k in picks)
+    batch_rows = ens.march_rows
     ens.close()
 
     def pct(v, q):
@@ -185,7 +186,8 @@ def ensemble_bench(members: int, steps: int, seed: int):
                      f"rain U[0,5e-5] m/s, Manning n U[0.01,0.05], f64 RK2, {steps} steps each, own dt"),
         "cells_total": cells,
         "batched": {"cell_updates_per_s": cells * steps / batched_s, "seconds": batched_s,
-                    "p50_s": pct(done_batched, 50), "p99_s": pct(done_batched, 99)},
+                    "p50_s": pct(done_batched, 50), "p99_s": pct(done_batched, 99),
+                    "march_rows": batch_rows},
         "plain": {"cell_updates_per_s": cells * steps / plain_s, "seconds": plain_s,
                   "p50_s": pct(done_plain, 50), "p99_s": pct(done_plain, 99)},
         "throughput_ratio": plain_s / batched_s,

@@ -248,6 +248,13 @@ int32_t sfv_batch_size(const sfv_batch* b);
  * names the first member whose dt was NaN or Inf. */
 int sfv_batch_run(sfv_batch* b, const double* t_end, int32_t match_endtime, double maximum_timestep,
                   int64_t max_steps, double* t_io, int64_t* steps_out);
+/* Segment height of the march kernels that the last sfv_batch_run used for
+ * members without a height set by sfv_set_march_rows (0 before any run).
+ * A run times the height rule's choice against its neighbours on its first
+ * replays and keeps the fastest (SFV_AUTOTUNE=0: the rule's choice); the
+ * choice is kept for later runs with the same member sizes.  No height
+ * changes any result. */
+int sfv_batch_get_march_rows(const sfv_batch* b, int32_t* rows);
 
 /* ------------------------------------------------------------------ serving
  * No counterpart in the reference.  A server runs flood scenarios submitted

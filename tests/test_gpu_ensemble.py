@@ -123,3 +123,31 @@ def test_members_with_different_segment_heights():
         assert sim.t == solo.t, sc.name
         assert np.array_equal(sim.get_full_state(), solo.get_full_state()), sc.name
     ens.close()
+
+
+def test_batch_height_tuning_keeps_results(monkeypatch):
+    """A batch run times the height rule's choice against its neighbours on
+    its first replays and keeps the fastest.  Heights never change results:
+    the tuned batch ends bitwise where an untuned batch and the solo runs
+    end, and a later run with the same member sizes keeps the choice."""
+    scs = [terrain(70, 45, seed=51), terrain(200, 90, seed=52), terrain(33, 130, seed=53),
+           dam_break(90, 12, hl=0.005, hr=0.0)]
+    sims = [make_hip(sc) for sc in scs]
+    ens = VF.Ensemble(sims)
+    assert ens.march_rows == 0
+    ens.advance_steps(48)                 # 6 replays: at most 5 timed trials, then the kept height
+    rows = ens.march_rows
+    assert rows in (1, 2, 3, 4, 6, 8, 12, 16, 24)
+    ens.advance_steps(16)
+    assert ens.march_rows == rows
+    monkeypatch.setenv("SFV_AUTOTUNE", "0")
+    sims0 = [make_hip(sc) for sc in scs]
+    ens0 = VF.Ensemble(sims0)
+    ens0.advance_steps(64)
+    for sc, sim, sim0 in zip(scs, sims, sims0):
+        assert sim.t == sim0.t, sc.name
+        assert np.array_equal(sim.get_full_state(), sim0.get_full_state()), sc.name
+        solo = _solo(sc, 64)
+        assert np.array_equal(sim.get_full_state(), solo.get_full_state()), sc.name
+    ens.close()
+    ens0.close()

@@ -12,6 +12,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -26,6 +28,8 @@ namespace sfv {
 BatchEngine::~BatchEngine() {
     if (stream_) (void)hipStreamSynchronize(stream_);
     if (graph_) (void)hipGraphExecDestroy(graph_);
+    for (hipEvent_t e : ev_)
+        if (e) (void)hipEventDestroy(e);
     if (d_mem_) (void)hipFree(d_mem_);
     if (d_hdr_) (void)hipFree(d_hdr_);
     if (d_ctl_) (void)hipFree(d_ctl_);
@@ -48,6 +52,8 @@ int BatchEngine::init(int precision, int capacity, std::string& why) {
     precision_ = precision;
     ETRY(hipGetDevice(&device_));
     ETRY(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    ETRY(hipEventCreate(&ev_[0]));
+    ETRY(hipEventCreate(&ev_[1]));
     ETRY(hipMalloc((void**)&d_hdr_, sizeof(BatchHeader)));
     ETRY(hipHostMalloc((void**)&h_hdr_, sizeof(BatchHeader), hipHostMallocDefault));
     hipDeviceProp_t prop;
@@ -168,6 +174,7 @@ int BatchEngine::add(sfv_handle* h, const RunParams& rp, int* slot_out, std::str
     ran_[s] = 0;
     rows_.push_back(s);
     table_dirty_ = true;
+    members_changed_ = true;
     *slot_out = s;
     return SFV_OK;
 }
@@ -186,6 +193,7 @@ void BatchEngine::remove(int slot) {
     for (size_t r = 0; r < rows_.size(); ++r)
         if (rows_[r] == slot) { rows_.erase(rows_.begin() + (long)r); break; }
     table_dirty_ = true;
+    members_changed_ = true;
 }
 
 bool BatchEngine::finished(int slot) const {
@@ -201,8 +209,9 @@ bool BatchEngine::finished(int slot) const {
 // segments shorten as long as no SIMD has to run two of the batch's waves:
 // every member takes the shortest height at which the batch's waves fit one
 // per SIMD (the default if none does).  Heights set with sfv_set_march_rows
-// are kept.  No height changes any result.
-void BatchEngine::choose_rows() {
+// are kept.  No height changes any result.  The online tuner (set_autotune)
+// times this choice against its neighbours.
+int BatchEngine::rule_rows() const {
     const int R0 = SFV_BATCH_DEFAULT_ROWS;
     const long slots = (long)blocks_stage_ * MARCH_WAVES, simds = 4L * cus_;
     auto waves = [&](int r) {
@@ -218,6 +227,12 @@ void BatchEngine::choose_rows() {
     if (waves(R0) < slots)
         for (int c : {1, 2, 3, 4, 6, 8})
             if (waves(c) <= simds) { r = c; break; }
+    return r;
+}
+
+void BatchEngine::choose_rows() {
+    const int r = rows_override_ > 0 ? rows_override_ : rule_rows();
+    rows_in_use_ = rows_.empty() ? 0 : r;
     for (int s : rows_) {
         const sfv_handle* h = slot_handle_[s];
         BatchMember& M = slot_row_[s];
@@ -227,6 +242,48 @@ void BatchEngine::choose_rows() {
         const int segs = (h->P.ny + my - 1) / my;
         M.seggroups = (segs + MARCH_WAVES - 1) / MARCH_WAVES;
     }
+}
+
+// What a kept height choice depends on: the share of the device and every
+// member's size and forced height.
+std::vector<long long> BatchEngine::signature() const {
+    std::vector<long long> sig;
+    sig.push_back(blocks_stage_);
+    for (int s : rows_) {
+        const sfv_handle* h = slot_handle_[s];
+        sig.push_back(((long long)h->P.nx << 32) | (unsigned)h->P.ny);
+        sig.push_back(h->rows_forced ? h->march_rows : 0);
+    }
+    std::sort(sig.begin() + 1, sig.end());
+    return sig;
+}
+
+// After a membership change: reuse the kept choice for the same member sizes,
+// else line up the timed trials: the rule's height, the next lower and the
+// two next higher candidates, the rule's height again.
+void BatchEngine::plan_tuning() {
+    trials_.clear();
+    trial_ms_.clear();
+    trial_ = 0;
+    rows_override_ = 0;
+    if (!autotune_ || rows_.empty()) return;
+    bool any_free = false;
+    for (int s : rows_) any_free = any_free || !slot_handle_[s]->rows_forced;
+    if (!any_free) return;
+    const std::vector<long long> sig = signature();
+    if (sig == tuned_sig_) { rows_override_ = tuned_rows_; return; }
+    static const int C[] = {1, 2, 3, 4, 6, 8, 12, 16, 24};
+    const int n = (int)(sizeof(C) / sizeof(C[0]));
+    const int r0 = rule_rows();
+    int i0 = 0;
+    while (i0 < n - 1 && C[i0] < r0) ++i0;
+    trials_.push_back(r0);
+    if (i0 > 0) trials_.push_back(C[i0 - 1]);
+    if (i0 < n - 1) trials_.push_back(C[i0 + 1]);
+    if (i0 < n - 2) trials_.push_back(C[i0 + 2]);
+    trials_.push_back(r0);
+    trial_ms_.assign(trials_.size(), 0.f);
+    rows_override_ = trials_[0];
 }
 
 int BatchEngine::upload_table(std::string& why) {
@@ -297,10 +354,17 @@ int BatchEngine::replay_begin(int k, std::string& why) {
             graph_ = nullptr;
         }
         fast_ = fast;
+        if (members_changed_) {
+            members_changed_ = false;
+            plan_tuning();
+        }
         if ((rc = upload_table(why))) return rc;
     }
     if (!graph_ && (rc = capture(why))) return rc;
+    timed_k_ = tuning() ? k : 0;
+    if (timed_k_) ETRY(hipEventRecord(ev_[0], stream_));
     for (int r = 0; r < k; ++r) ETRY(hipGraphLaunch(graph_, stream_));
+    if (timed_k_) ETRY(hipEventRecord(ev_[1], stream_));
     ETRY(hipMemcpyAsync(h_ctl_, d_ctl_, sizeof(StepCtl) * capacity_, hipMemcpyDeviceToHost, stream_));
     return SFV_OK;
 }
@@ -308,6 +372,39 @@ int BatchEngine::replay_begin(int k, std::string& why) {
 int BatchEngine::replay_end(std::string& why) {
     ETRY(hipStreamSynchronize(stream_));
     for (int s : rows_) ran_[s] = 1;      // membership cannot change between begin and end
+    if (timed_k_) {
+        float ms = 0.f;
+        ETRY(hipEventElapsedTime(&ms, ev_[0], ev_[1]));
+        trial_ms_[trial_] = ms / (float)timed_k_;
+        timed_k_ = 0;
+        if (++trial_ < (int)trials_.size()) {
+            rows_override_ = trials_[trial_];
+        } else {
+            // A step's cost drifts as the flow develops, so each trial is set
+            // against the rule's height timed first and last, interpolated to
+            // the trial's place; another height must be 3% faster than that.
+            const int last = trial_ - 1;
+            int best = trials_[last];
+            float best_ratio = 0.97f;
+            for (int i = 1; i < last; ++i) {
+                const float ref = trial_ms_[0] + (trial_ms_[last] - trial_ms_[0]) * (float)i / (float)last;
+                const float ratio = trial_ms_[i] / ref;
+                if (ref > 0.f && ratio < best_ratio) {
+                    best = trials_[i];
+                    best_ratio = ratio;
+                }
+            }
+            rows_override_ = best;
+            tuned_rows_ = best;
+            tuned_sig_ = signature();
+            if (const char* e = std::getenv("SFV_TUNE_REPORT"); e && e[0] == '1') {
+                std::fprintf(stderr, "[sfv batch autotune] %d members:", members());
+                for (size_t i = 0; i < trials_.size(); ++i) std::fprintf(stderr, " %d:%.3fms", trials_[i], trial_ms_[i]);
+                std::fprintf(stderr, " -> %d\n", best);
+            }
+        }
+        if (rows_override_ != rows_in_use_) table_dirty_ = true;
+    }
     return SFV_OK;
 }
 
@@ -372,6 +469,7 @@ int sfv_batch_create(sfv_handle* const* handles, int32_t n, sfv_batch** out) {
         delete b;
         return bfail(nullptr, rc, why);
     }
+    b->engine.set_autotune(sfv::autotune_enabled());
     *out = b;
     return SFV_OK;
 }
@@ -381,6 +479,12 @@ void sfv_batch_destroy(sfv_batch* b) { delete b; }
 const char* sfv_batch_last_error(const sfv_batch* b) { return b ? b->err.c_str() : g_batch_error.c_str(); }
 
 int32_t sfv_batch_size(const sfv_batch* b) { return b ? (int32_t)b->members.size() : 0; }
+
+int sfv_batch_get_march_rows(const sfv_batch* b, int32_t* rows) {
+    if (!b || !rows) return SFV_ERR_ARGUMENT;
+    *rows = b->engine.rows_in_use();
+    return SFV_OK;
+}
 
 int sfv_batch_run(sfv_batch* b, const double* t_end, int32_t match_endtime, double maximum_timestep,
                   int64_t max_steps, double* t_io, int64_t* steps_out) {
@@ -418,6 +522,7 @@ int sfv_batch_run(sfv_batch* b, const double* t_end, int32_t match_endtime, doub
         if (max_steps > 0) k = std::min<int64_t>(k, (max_steps - launched + gs - 1) / gs);
         if (!t_end) k = (max_steps - launched + gs - 1) / gs;
         if (k <= 0) break;
+        if (b->engine.tuning()) k = 1;     // one timed replay per height trial
         const int rc = b->engine.replay((int)k, why);
         if (rc) { drop_all(); return bfail(b, rc, why); }
         launched += k * gs;

@@ -174,11 +174,6 @@ std::map<TuneKey, int> g_tuned;
 
 int size_bucket(int n) { return (int)std::lround(4.0 * std::log2((double)std::max(n, 1))); }
 
-bool autotune_enabled() {
-    const char* e = std::getenv("SFV_AUTOTUNE");
-    return !(e && e[0] == '0');
-}
-
 template <class S>
 hipError_t tune_steps(sfv_handle* h, int slot, int n) {
     const KParams& P = h->P;
@@ -198,6 +193,11 @@ hipError_t tune_steps(sfv_handle* h, int slot, int n) {
 }  // namespace
 
 namespace sfv {
+bool autotune_enabled() {
+    const char* e = std::getenv("SFV_AUTOTUNE");
+    return !(e && e[0] == '0');
+}
+
 int cached_march_rows(sfv_handle* h) {
     if (h->march_rows > 0) return h->march_rows;
     const TuneKey key{h->device, h->cfg.precision, size_bucket(h->cfg.nx), size_bucket(h->cfg.ny)};

@@ -63,6 +63,22 @@ public:
     // Without waiting: has the work queued by replay_begin finished?  (Also
     // true after a device error, which replay_end then reports.)
     bool replay_ready() const;
+    // Online segment-height tuning (off by default).  While on, the replays
+    // after a membership change time the height rule's choice and its
+    // neighbours in the list {1, 2, 3, 4, 6, 8, 12, 16, 24} (one below, two
+    // above), one replay each, with the rule's choice first and last, and
+    // then keep the fastest: the rule's choice unless another is more than
+    // 3% faster than the rule's time interpolated to the trial's place.
+    // The choice is kept for later runs with the same member sizes.  Every
+    // height gives bitwise the same results, so the timed replays are
+    // ordinary steps of the run.
+    void set_autotune(bool on) { autotune_ = on; }
+    // A timed trial is pending, or a membership change not yet replayed may
+    // start one: replay one graph at a time so that each trial is timed alone.
+    bool tuning() const { return (autotune_ && members_changed_) || trial_ < (int)trials_.size(); }
+    // Segment height of the members without a forced one (0: no members yet).
+    int rows_in_use() const { return rows_in_use_; }
+
     // The grid's share of the device; takes effect at the next replay_begin
     // (the graph is recaptured when it changes).  Results do not depend on it.
     // Not to be called between replay_begin and replay_end.
@@ -80,6 +96,9 @@ private:
     int grow(int capacity, std::string& why);
     int upload_table(std::string& why);
     void choose_rows();
+    int rule_rows() const;
+    void plan_tuning();
+    std::vector<long long> signature() const;
     int capture(std::string& why);
     void size_grid();
 
@@ -99,6 +118,17 @@ private:
     StepCtl* h_ctl_ = nullptr;
     hipGraphExec_t graph_ = nullptr;
     bool table_dirty_ = true;
+    bool members_changed_ = false;
+    bool autotune_ = false;
+    int rows_override_ = 0;                  // > 0: every member without a forced height takes it
+    int rows_in_use_ = 0;
+    std::vector<int> trials_;                // heights to time, in order
+    std::vector<float> trial_ms_;            // ms per replay of each trial
+    int trial_ = 0;                          // next trial; == trials_.size(): none pending
+    int timed_k_ = 0;                        // replays between the events (0: none timed)
+    std::vector<long long> tuned_sig_;       // member sizes the kept choice was made for
+    int tuned_rows_ = 0;
+    hipEvent_t ev_[2] = {nullptr, nullptr};
     bool fast_ = false;           // the captured graph uses the specialised march kernels
     int cus_ = 256;
     std::vector<int> rows_;                  // slot of each table row

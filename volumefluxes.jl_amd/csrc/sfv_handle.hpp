@@ -226,4 +226,6 @@ int create_handle(const sfv_config* cfg, DevicePool* pool, sfv_handle** out, std
 int tune_march_rows(sfv_handle* h, std::string& why);
 // The cached height for the handle's size class, without measuring (0: none yet).
 int cached_march_rows(sfv_handle* h);
+// False when SFV_AUTOTUNE=0 is set in the environment: no height is measured.
+bool autotune_enabled();
 }  // namespace sfv

@@ -206,6 +206,8 @@ def _declare(L) -> None:
     L.sfv_batch_run.argtypes = [b, POINTER(c_double), c_int32, c_double, c_int64, POINTER(c_double),
                                 POINTER(c_int64)]
     L.sfv_batch_run.restype = c_int32
+    L.sfv_batch_get_march_rows.argtypes = [b, POINTER(c_int32)]
+    L.sfv_batch_get_march_rows.restype = c_int32
     sv = c_void_p
     L.sfv_server_create.argtypes = [c_int64, c_int32, POINTER(c_void_p)]
     L.sfv_server_create.restype = c_int32

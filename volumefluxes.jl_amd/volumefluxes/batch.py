@@ -37,6 +37,15 @@ class Ensemble:
     def __len__(self) -> int:
         return len(self.sims)
 
+    @property
+    def march_rows(self) -> int:
+        """Segment height the last run used for members without a forced one
+        (timed against its neighbours on the run's first replays; 0 before
+        any run).  It never changes results."""
+        r = ctypes.c_int32()
+        C.check_batch(self._b, C.lib().sfv_batch_get_march_rows(self._b, ctypes.byref(r)))
+        return int(r.value)
+
     def close(self) -> None:
         b = getattr(self, "_b", None)
         if b:
