@@ -280,6 +280,54 @@ function upload_interior!(sim, box)
     box.host_valid = false          # ghosts now differ from the mirror until the next read
 end
 
+# ---- ensembles (sfv_batch_*) ---------------------------------------------------------
+function check_batch(b, rc)
+    rc == SFV_OK && return nothing
+    msg = unsafe_string(ccall((:sfv_batch_last_error, libsinfvm), Cstring, (Ptr{Cvoid},), b))
+    rc == SFV_ERR_ARGUMENT && throw(ArgumentError(msg))
+    rc == SFV_ERR_NONFINITE_DT && (@warn msg; return nothing)
+    error("libsinfvm_hip: $msg (status $rc)")
+end
+
+# simulate_to_time (src/simulator.jl:179-212, without callbacks) for several HIP
+# simulators at once: one persistent kernel launch per RK stage covers all of
+# them (a replayed HIP graph), each with its own dt until its own endtime.  Each
+# ends bitwise where it would alone.  `endtimes` is one value or one per
+# simulator.  Members use RungeKutta2 with WallBC or PeriodicBC.  Returns the
+# steps of each member.
+function simulate_ensemble!(sims::AbstractVector, endtimes; match_endtime=true, maximum_timestep=nothing)
+    n = length(sims)
+    n > 0 || throw(ArgumentError("an ensemble needs at least one simulator"))
+    te = endtimes isa Number ? fill(Float64(endtimes), n) : collect(Float64, endtimes)
+    length(te) == n || throw(DimensionMismatch("give one endtime per simulator"))
+    boxes = [handle!(s) for s in sims]
+    hs = Ptr{Cvoid}[b.h for b in boxes]
+    bref = Ref{Ptr{Cvoid}}(C_NULL)
+    check_batch(C_NULL, ccall((:sfv_batch_create, libsinfvm), Cint, (Ptr{Ptr{Cvoid}}, Int32, Ref{Ptr{Cvoid}}),
+                              hs, n, bref))
+    try
+        t = Float64[s.t[1] for s in sims]
+        steps = zeros(Int64, n)
+        mts = maximum_timestep === nothing ? 0.0 : Float64(maximum_timestep)
+        rc = ccall((:sfv_batch_run, libsinfvm), Cint,
+                   (Ptr{Cvoid}, Ptr{Float64}, Int32, Float64, Int64, Ptr{Float64}, Ptr{Int64}),
+                   bref[], te, match_endtime ? 1 : 0, mts, 0, t, steps)
+        for (i, (s, box)) in enumerate(zip(sims, boxes))
+            s.t[1] = t[i]
+            if steps[i] > 0
+                dt = Ref{Float64}(0.0)
+                check(box.h, ccall((:sfv_current_timestep, libsinfvm), Cint, (Ptr{Cvoid}, Ref{Float64}), box.h, dt))
+                s.current_timestep[1] = dt[]
+            end
+            box.host_valid = false
+        end
+        check_batch(bref[], rc)
+        return steps
+    finally
+        ccall((:sfv_batch_destroy, libsinfvm), Cvoid, (Ptr{Cvoid},), bref[])
+    end
+end
+
 # ---- segment height of the march kernels (autotuned on the first step) -----------
 set_march_rows!(sim::SinFVM.Simulator{<:HIPBackend}, rows::Integer) =
     check(handle!(sim).h, ccall((:sfv_set_march_rows, libsinfvm), Cint, (Ptr{Cvoid}, Int32), handle!(sim).h, rows))
