@@ -423,6 +423,102 @@ def adjoint_bench(seed: int, n: int = 512, steps: int = 200, checkpoint_every: i
     return out
 
 
+def variant_rain(t, x, y):
+    """A smooth per-cell rain field for the FunctionalRain variant (m/s)."""
+    return 1e-5 * (1.0 + 0.5 * np.sin(x / 97.0) * np.cos(y / 61.0)) * (1.0 + 1e-3 * t)
+
+
+def variants_bench(size: int, seed: int, steps: int, check: bool) -> dict:
+    """The other configurations of the step on the terrain workload (same
+    grid, bottom, initial state and friction), each timed device-resident on
+    the GPU: ForwardEuler and NeumannBC (the per-pass kernels), PeriodicBC and
+    NoReconstruction (the march kernels), FunctionalRain (the host evaluates
+    the rain function on every cell and uploads the field before each step,
+    so its rate includes that work), plus the 1D Stoker dam break (1024
+    cells to t = 20 s).  With `check`, each first step (1D: the whole run) is
+    compared with the oracle's."""
+    grid0, _, _, U0, B = build_scenario(size, "f64", seed)
+    backend = VF.make_hip_backend(np.float64)
+    L = C.lib()
+    if check:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O  # the reference restatement, used only to check results
+        O.set_threads(os.cpu_count() or 1)
+
+    def system_for(bc, rec, rain_fn):
+        g = VF.CartesianGrid(size, size, gc=2, boundary=bc, extent=[[0.0, float(size)], [0.0, float(size)]])
+        eq = VF.ShallowWaterEquations(VF.BottomTopography2D(B, None, g), desingularizing_kappa=1e-4)
+        rain = VF.FunctionalRain(rain_fn, g) if rain_fn else VF.ConstantRain(1e-5 * 3600)
+        system = VF.ConservedSystem(None, rec, VF.CentralUpwind(eq), eq, g, [VF.SourceTermBottom(), rain],
+                                    VF.ImplicitFriction(Cz=0.03 * 0.03))
+        return g, system, rain
+
+    cases = [
+        ("forward_euler", VF.WallBC(), VF.LinearReconstruction(), VF.ForwardEulerStepper(), None),
+        ("neumann_bc", VF.NeumannBC(), VF.LinearReconstruction(), VF.RungeKutta2(), None),
+        ("periodic_bc", VF.PeriodicBC(), VF.LinearReconstruction(), VF.RungeKutta2(), None),
+        ("no_reconstruction", VF.WallBC(), VF.NoReconstruction(), VF.RungeKutta2(), None),
+        ("functional_rain", VF.WallBC(), VF.LinearReconstruction(), VF.RungeKutta2(), variant_rain),
+    ]
+    out = {}
+    for name, bc, rec, stepper, rain_fn in cases:
+        g, system, rain = system_for(bc, rec, rain_fn)
+        sim = VF.Simulator(backend, system, stepper, g, cfl=0.25)
+        VF.set_current_state(sim, U0)
+        VF.perform_step(sim, float("inf"))
+        first_gpu = sim.get_full_state()
+        VF.set_current_state(sim, U0)
+        sim.t = 0.0
+        VF.advance_steps(sim, 5)
+        C.check(sim.handle, L.sfv_synchronize(sim.handle))
+        t0 = time.perf_counter()
+        VF.advance_steps(sim, steps)
+        C.check(sim.handle, L.sfv_synchronize(sim.handle))
+        el = time.perf_counter() - t0
+        row = {"value": g.nx * g.ny * steps / el, "unit": "cell-updates/s", "ms_per_step": el / steps * 1e3,
+               "steps": steps, "kernels": sim.kernels,
+               "state_finite": bool(np.isfinite(sim.get_full_state()).all())}
+        if check:
+            cfg = VF.build_config(system, stepper, g, 0.25, C.SFV_F64)
+            o = O.Oracle2D(cfg, B)
+            o.set_interior(U0)
+            if rain_fn:
+                o.set_rain_field(VF.rain_field_values(rain, g, 0.0))
+            o.step(float("inf"), 0.0)
+            row["parity_first_step"] = "bitwise" if np.array_equal(first_gpu, o.state()) else "MISMATCH"
+        out[name] = row
+
+    # the 1D workload: Stoker dam break, 1024 cells on [0, 1000] m, RK2, CFL 0.25, to t = 20 s
+    g1 = VF.CartesianGrid(1024, gc=2, boundary=VF.WallBC(), extent=[[0.0, 1000.0]])
+    eq1 = VF.ShallowWaterEquations1D(VF.ConstantBottomTopography())
+    s1 = VF.ConservedSystem(None, VF.LinearReconstruction(), VF.CentralUpwind(eq1), eq1, g1, [], None)
+    U1 = np.zeros((1024, 2))
+    U1[:, 0] = np.where(g1.cell_centers() < 500.0, 2.0, 1.0)
+    sim = VF.Simulator(backend, s1, VF.RungeKutta2(), g1, cfl=0.25)
+    VF.set_current_state(sim, U1)
+    VF.simulate_to_time(sim, 20.0)            # warm-up run (tuning, first launches)
+    VF.set_current_state(sim, U1)
+    t1, n1 = ctypes.c_double(0.0), ctypes.c_int64()
+    C.check(sim.handle, L.sfv_synchronize(sim.handle))
+    t0 = time.perf_counter()
+    C.check(sim.handle, L.sfv_run(sim.handle, 20.0, 1, 0.0, 0, ctypes.byref(t1), ctypes.byref(n1)))
+    el = time.perf_counter() - t0
+    n1 = n1.value
+    row = {"value": 1024 * n1 / el, "unit": "cell-updates/s", "steps": int(n1), "seconds": el,
+           "ms_per_step": el / max(n1, 1) * 1e3, "t_end": t1.value}
+    if check:
+        g2, s2 = VF.embed_1d(s1, g1)
+        cfg = VF.build_config(s2, VF.RungeKutta2(), g2, 0.25, C.SFV_F64)
+        cfg.ny = 1
+        cfg.dy = cfg.dx
+        o = O.Oracle1D(cfg, np.ascontiguousarray(s1.equation.B.corners(g1), dtype=np.float64))
+        o.set_interior(U1)
+        o.simulate_to_time(20.0)
+        row["parity_full_run"] = "bitwise" if np.array_equal(sim.get_full_state(), o.state()) else "MISMATCH"
+    out["stoker_1d"] = row
+    return out
+
+
 def grid_leg(size: int, precision: str, seed: int, steps: int, warmup: int, kernels: str = "march") -> dict:
     """One terrain run on the GPU: the first step kept for the parity probe,
     then `warmup` untimed and `steps` timed device-resident steps, and each
@@ -510,6 +606,8 @@ def main():
     ap.add_argument("--no-f32", dest="f32_leg", action="store_false",
                     help="skip the Float32-storage run of the same workload")
     ap.add_argument("--adjoint-size", type=int, default=512, help="adjoint leg grid side (0: skip)")
+    ap.add_argument("--variant-steps", type=int, default=50,
+                    help="timed steps of each configuration variant (0: skip the variants leg)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -595,6 +693,8 @@ def main():
                                           args.serving_budget_gb, args.seed)
     if rank == 0 and args.adjoint_size > 0:
         result["adjoint"] = adjoint_bench(args.seed, args.adjoint_size)
+    if rank == 0 and args.variant_steps > 0:
+        result["variants"] = variants_bench(args.size, args.seed, args.variant_steps, not args.no_cpu)
     print(json.dumps(result))
     return 0
 

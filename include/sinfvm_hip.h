@@ -182,13 +182,19 @@ int sfv_time_kernel(sfv_handle* h, const char* which, int32_t n, double* ms_out)
 /* Number of visible HIP devices (0 when none).  Needs no handle. */
 int sfv_device_count(void);
 
-/* Select the kernels of an RK2 step for this handle: 2 = fused stage kernels,
+/* Select the kernels of a step for this handle: 2 = fused RK2 stage kernels,
  * one wavefront per 60-column strip marching in y (the default whenever the
  * boundary-condition sources of the ghost cells are interior cells); 1 =
- * fused stage kernels tiled through LDS; 0 = one kernel per reference pass
- * (the device-side reference, and the path for NeumannBC and ForwardEuler).
- * All give bitwise identical results.  No counterpart in the reference. */
+ * fused RK2 stage kernels tiled through LDS; 3 = every right-hand side from
+ * the march stage-1 kernel, the updates and boundary conditions from the
+ * per-pass kernels (any stepper and boundary condition with gc >= 2: the
+ * default for NeumannBC and ForwardEuler); 0 = one kernel per reference pass
+ * (the device-side reference).  All give bitwise identical results.  No
+ * counterpart in the reference. */
 int sfv_set_fused(sfv_handle* h, int32_t fused);
+/* The kernel variant a step of this handle runs (2, 1 or 0 as above; -1 for
+ * a NULL handle).  No counterpart in the reference. */
+int32_t sfv_get_fused(const sfv_handle* h);
 
 /* Rows per wave segment of the march kernels (sfv_set_fused(h, 2)).  0 (the
  * default) lets the library choose on the first step: candidate heights are

@@ -12,7 +12,7 @@ from helpers import dam_break, make_hip, make_oracle, max_rel_diff, run_both_com
 pytestmark = pytest.mark.gpu
 
 
-KERNELS = ["march", "tiled", "per-pass"]
+KERNELS = ["march", "tiled", "hybrid", "per-pass"]
 
 
 def _assert_bitwise(sc, nsteps, kernels=None, **kw):
@@ -35,12 +35,45 @@ def test_terrain_rk2_periodic(kernels):
     _assert_bitwise(terrain(64, 16, seed=5, bc=VF.PeriodicBC()), 25, kernels=kernels)
 
 
-def test_terrain_neumann_rk2():
-    _assert_bitwise(terrain(33, 29, seed=7, bc=VF.NeumannBC()), 20)
+@pytest.mark.parametrize("kernels", [None, "hybrid", "per-pass"])
+def test_terrain_neumann_rk2(kernels):
+    _assert_bitwise(terrain(33, 29, seed=7, bc=VF.NeumannBC()), 20, kernels=kernels)
 
 
-def test_terrain_forward_euler():
-    _assert_bitwise(terrain(40, 37, seed=9, stepper=VF.ForwardEulerStepper()), 30)
+@pytest.mark.parametrize("kernels", [None, "hybrid", "per-pass"])
+def test_terrain_forward_euler(kernels):
+    _assert_bitwise(terrain(40, 37, seed=9, stepper=VF.ForwardEulerStepper()), 30, kernels=kernels)
+
+
+def test_default_kernels_by_configuration():
+    """RK2 with interior ghost sources takes the march kernels; ForwardEuler
+    and NeumannBC take the hybrid path (march stage-1 right-hand sides,
+    per-pass updates)."""
+    assert make_hip(terrain(20, 20, seed=1)).kernels == "march"
+    assert make_hip(terrain(20, 20, seed=1, bc=VF.NeumannBC())).kernels == "hybrid"
+    assert make_hip(terrain(20, 20, seed=1, stepper=VF.ForwardEulerStepper())).kernels == "hybrid"
+
+
+@pytest.mark.parametrize("case", ["fe_periodic", "fe_neumann", "fe_norec", "neumann_f32", "fe_gc3", "fe_dry_front",
+                                  "neumann_ragged"])
+def test_hybrid_path_cases(case):
+    FE = VF.ForwardEulerStepper()
+    sc = {
+        "fe_periodic": lambda: terrain(61, 45, seed=31, bc=VF.PeriodicBC(), stepper=FE),
+        "fe_neumann": lambda: terrain(70, 33, seed=32, bc=VF.NeumannBC(), stepper=FE),
+        "fe_norec": lambda: terrain(50, 20, seed=33, rec=VF.NoReconstruction(), stepper=FE),
+        "neumann_f32": lambda: terrain(48, 40, seed=34, bc=VF.NeumannBC(), realtype=np.float32),
+        "fe_gc3": lambda: terrain(40, 24, seed=35, gc=3, stepper=FE),
+        "fe_dry_front": lambda: dam_break(128, 8, hl=0.005, hr=0.0, stepper=FE),
+        "neumann_ragged": lambda: terrain(125, 37, seed=36, bc=VF.NeumannBC()),
+    }[case]()
+    _assert_bitwise(sc, 20, kernels="hybrid")
+
+
+@pytest.mark.parametrize("shape", [(3, 2), (5, 3), (3, 70), (130, 3), (61, 17)])
+def test_hybrid_small_and_ragged(shape):
+    _assert_bitwise(terrain(*shape, seed=13, dx=1.0, n_bumps=3, stepper=VF.ForwardEulerStepper()), 12,
+                    kernels="hybrid")
 
 
 def test_no_reconstruction():

@@ -95,6 +95,24 @@ int step_kernels(sfv_handle* h, int slot, int cur) {
     S* U0 = (S*)h->U[cur];
     S* U2 = (S*)h->U[cur ^ 1];
     S* RHS = (S*)h->RHS;
+    if (h->fused == KV_HYBRID) {
+        // the second stage-1 launch also raises this slot's wave-speed maxima,
+        // after dt was taken from them; the slot is re-armed before its next use
+        if (h->cfg.stepper == SFV_FORWARD_EULER) {
+            HIP_TRY(h, launch_march<S>(st, P, h->ctl, slot, 1, U0, nullptr, RHS, h->B));
+            HIP_TRY(h, launch_finish<S>(st, P, h->ctl, slot, 1, U0, U0, RHS, U2, h->B, 1));
+            HIP_TRY(h, launch_bc<S>(st, P, h->ctl, slot, U2, BC_LAST_STAGE));
+            return SFV_OK;
+        }
+        S* U1 = (S*)h->U[2];
+        HIP_TRY(h, launch_march<S>(st, P, h->ctl, slot, 1, U0, nullptr, RHS, h->B));
+        HIP_TRY(h, launch_finish<S>(st, P, h->ctl, slot, 1, U0, U0, RHS, U1, h->B, 1));
+        HIP_TRY(h, launch_bc<S>(st, P, h->ctl, slot, U1, BC_IN_STEP));
+        HIP_TRY(h, launch_march<S>(st, P, h->ctl, slot, 1, U1, nullptr, RHS, h->B));
+        HIP_TRY(h, launch_finish<S>(st, P, h->ctl, slot, 2, U0, U1, RHS, U2, h->B, 1));
+        HIP_TRY(h, launch_bc<S>(st, P, h->ctl, slot, U2, BC_LAST_STAGE));
+        return SFV_OK;
+    }
     if (h->cfg.stepper == SFV_FORWARD_EULER) {
         HIP_TRY(h, launch_flux<S>(st, P, h->ctl, slot, U0, RHS, h->B, 1));
         HIP_TRY(h, launch_finish<S>(st, P, h->ctl, slot, 1, U0, U0, RHS, U2, h->B));
@@ -124,7 +142,7 @@ int step_kernels(sfv_handle* h, int slot, int cur) {
 }
 
 int launch_steps(sfv_handle* h, int64_t nsteps) {
-    if (h->cfg.stepper == SFV_RUNGE_KUTTA2 && h->fused == KV_PER_PASS) {
+    if (h->cfg.stepper == SFV_RUNGE_KUTTA2 && (h->fused == KV_PER_PASS || h->fused == KV_HYBRID)) {
         std::string why;
         const int rc = h->ensure_scratch(why);
         if (rc) return fail(h, rc, why);
@@ -617,8 +635,10 @@ int sfv_device_count(void) {
 
 int sfv_set_fused(sfv_handle* h, int32_t fused) {
     if (!h) return SFV_ERR_ARGUMENT;
-    if (fused < KV_PER_PASS || fused > KV_MARCH) return fail(h, SFV_ERR_ARGUMENT, "unknown kernel variant");
-    if (fused != KV_PER_PASS && !h->fusable())
+    if (fused < KV_PER_PASS || fused > KV_HYBRID) return fail(h, SFV_ERR_ARGUMENT, "unknown kernel variant");
+    if (fused == KV_HYBRID && !h->hybrid_ok())
+        return fail(h, SFV_ERR_ARGUMENT, "the march stage-1 kernel needs gc >= 2");
+    if ((fused == KV_TILED || fused == KV_MARCH) && !h->fusable())
         return fail(h, SFV_ERR_ARGUMENT,
                     "fused kernels need RungeKutta2, gc >= 2 and boundary-condition sources that are interior cells");
     h->fused = fused;
@@ -635,5 +655,7 @@ int sfv_set_march_rows(sfv_handle* h, int32_t rows) {
 }
 
 int32_t sfv_get_march_rows(const sfv_handle* h) { return h ? h->march_rows : -1; }
+
+int32_t sfv_get_fused(const sfv_handle* h) { return h ? h->fused : -1; }
 
 }  // extern "C"

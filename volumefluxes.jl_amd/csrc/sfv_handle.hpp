@@ -14,7 +14,10 @@
 #include "swe_kernels.hpp"
 
 // Kernel variants of an RK2 step (all give identical results).
-enum { KV_PER_PASS = 0, KV_TILED = 1, KV_MARCH = 2 };
+// Kernel variants of a step.  KV_HYBRID: every right-hand side from the march
+// stage-1 kernel, the updates and boundary conditions from the per-pass
+// kernels (ForwardEuler and NeumannBC, whose steps the fused stage 2 cannot do).
+enum { KV_PER_PASS = 0, KV_TILED = 1, KV_MARCH = 2, KV_HYBRID = 3 };
 
 struct sfv_handle {
     sfv_config cfg{};
@@ -136,7 +139,7 @@ struct sfv_handle {
         }
         if (hipStreamSynchronize(stream) != hipSuccess) { why = "initialisation failed"; return SFV_ERR_HIP; }
         build_params();
-        fused = fusable() ? KV_MARCH : KV_PER_PASS;
+        fused = fusable() ? KV_MARCH : hybrid_ok() ? KV_HYBRID : KV_PER_PASS;
         return SFV_OK;
     }
 
@@ -183,6 +186,10 @@ struct sfv_handle {
     }
 
     static void ghost_maps_into(int bc, int gc, int N, int* lo, int* hi, int* flo, int* fhi);
+
+    // The march stage-1 kernel reads a 2-cell halo of the state and writes
+    // the interior right-hand side; it needs gc >= 2 and nothing else.
+    bool hybrid_ok() const { return P.gc >= 2; }
 
     // The fused stage kernels rebuild stage-1 ghost cells from their
     // boundary-condition sources, which must then be interior cells.

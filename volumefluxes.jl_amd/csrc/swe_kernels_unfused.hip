@@ -115,11 +115,14 @@ __global__ void __launch_bounds__(256) k_flux(KParams P, StepCtl* ctl, int slot,
 // rungekutta.jl:38-49 / forwardeuler.jl:41-43, then implicit_substep!
 // (friction.jl:80-84, from the substep input X) and post_proc_substep!
 // (timestepper.jl:53-63), over all cells.  stage 1 computes dt
-// (simulator.jl:147-151); stage 2 averages with U0.
+// (simulator.jl:147-151); stage 2 averages with U0.  With rhs_interior set,
+// RHS holds only the interior cells (the march stage-1 kernel's output) and
+// the other cells take their right-hand side as k_flux gives it there: the
+// non-directional sources on w, zero on hu and hv.
 template <class S>
 __global__ void __launch_bounds__(256) k_finish(KParams P, StepCtl* ctl, int slot, int stage, const S* __restrict__ U0,
                                                 const S* __restrict__ X, const S* __restrict__ RHS,
-                                                S* __restrict__ Out, const double* __restrict__ B) {
+                                                S* __restrict__ Out, const double* __restrict__ B, int rhs_interior) {
     const StepSlot& sl = ctl->slot[slot];
     if (!sl.active) return;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -140,16 +143,25 @@ __global__ void __launch_bounds__(256) k_finish(KParams P, StepCtl* ctl, int slo
     if (i >= P.nxT || j >= P.nyT) return;
     const size_t n = (size_t)P.nxT * P.nyT;
     const size_t k = cid(P, i, j);
-    S x[3], o[3];
+    S x[3], o[3], r[3];
 #pragma unroll
     for (int v = 0; v < 3; ++v) x[v] = X[v * n + k];
+    const bool interior = i >= P.gc && i < P.nxT - P.gc && j >= P.gc && j < P.nyT - P.gc;
+    if (rhs_interior && !interior) {
+        r[0] = apply_sources(P, ctl, S(0), k);
+        r[1] = S(0);
+        r[2] = S(0);
+    } else {
+#pragma unroll
+        for (int v = 0; v < 3; ++v) r[v] = RHS[v * n + k];
+    }
     if (stage == 1) {
 #pragma unroll
-        for (int v = 0; v < 3; ++v) o[v] = (S)(x[v] + dt_d * RHS[v * n + k]);
+        for (int v = 0; v < 3; ++v) o[v] = (S)(x[v] + dt_d * r[v]);
     } else {
 #pragma unroll
         for (int v = 0; v < 3; ++v) {
-            o[v] = (S)(x[v] + dt_s * RHS[v * n + k]);
+            o[v] = (S)(x[v] + dt_s * r[v]);
             o[v] = (S)(0.5 * (U0[v * n + k] + o[v]));
         }
     }
@@ -177,10 +189,10 @@ hipError_t launch_flux(hipStream_t st, const KParams& P, StepCtl* ctl, int slot,
 }
 template <class S>
 hipError_t launch_finish(hipStream_t st, const KParams& P, StepCtl* ctl, int slot, int stage, const S* U0, const S* X,
-                         const S* RHS, S* Out, const double* B) {
+                         const S* RHS, S* Out, const double* B, int rhs_interior) {
     const dim3 block(64, 4);
     const dim3 grid((P.nxT + 63) / 64, (P.nyT + 3) / 4);
-    hipLaunchKernelGGL(k_finish<S>, grid, block, 0, st, P, ctl, slot, stage, U0, X, RHS, Out, B);
+    hipLaunchKernelGGL(k_finish<S>, grid, block, 0, st, P, ctl, slot, stage, U0, X, RHS, Out, B, rhs_interior);
     return hipPeekAtLastError();
 }
 template hipError_t launch_flux<double>(hipStream_t, const KParams&, StepCtl*, int, const double*, double*,
@@ -188,15 +200,15 @@ template hipError_t launch_flux<double>(hipStream_t, const KParams&, StepCtl*, i
 template hipError_t launch_flux<float>(hipStream_t, const KParams&, StepCtl*, int, const float*, float*,
                                        const double*, int);
 template hipError_t launch_finish<double>(hipStream_t, const KParams&, StepCtl*, int, int, const double*,
-                                          const double*, const double*, double*, const double*);
+                                          const double*, const double*, double*, const double*, int);
 template hipError_t launch_finish<float>(hipStream_t, const KParams&, StepCtl*, int, int, const float*, const float*,
-                                         const float*, float*, const double*);
+                                         const float*, float*, const double*, int);
 
 template __global__ void k_flux<double>(KParams, StepCtl*, int, const double*, double*, const double*, int);
 template __global__ void k_flux<float>(KParams, StepCtl*, int, const float*, float*, const double*, int);
 template __global__ void k_finish<double>(KParams, StepCtl*, int, int, const double*, const double*, const double*,
-                                          double*, const double*);
+                                          double*, const double*, int);
 template __global__ void k_finish<float>(KParams, StepCtl*, int, int, const float*, const float*, const float*,
-                                         float*, const double*);
+                                         float*, const double*, int);
 
 }  // namespace sfv

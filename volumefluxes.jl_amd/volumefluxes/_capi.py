@@ -190,6 +190,8 @@ def _declare(L) -> None:
     L.sfv_device_count.restype = c_int32
     L.sfv_set_fused.argtypes = [h, c_int32]
     L.sfv_set_fused.restype = c_int32
+    L.sfv_get_fused.argtypes = [h]
+    L.sfv_get_fused.restype = c_int32
     L.sfv_set_march_rows.argtypes = [h, c_int32]
     L.sfv_set_march_rows.restype = c_int32
     L.sfv_get_march_rows.argtypes = [h]

@@ -132,12 +132,20 @@ class Simulator:
             U = U2
         C.check(self._h, C.lib().sfv_set_state(self._h, U.ctypes.data_as(ctypes.c_void_p), C.SFV_LAYOUT_FULL))
 
-    KERNELS = {"per-pass": 0, "tiled": 1, "march": 2}
+    KERNELS = {"per-pass": 0, "tiled": 1, "march": 2, "hybrid": 3}
 
     def set_kernels(self, variant: str) -> None:
-        """Choose the kernels of an RK2 step: "march" (default), "tiled" or
-        "per-pass".  All give bitwise identical results."""
+        """Choose the kernels of a step: "march" (the RK2 default), "tiled",
+        "hybrid" (march stage-1 right-hand sides with per-pass updates: the
+        default for ForwardEuler and NeumannBC) or "per-pass".  All give
+        bitwise identical results."""
         C.check(self._h, C.lib().sfv_set_fused(self._h, self.KERNELS[variant]))
+
+    @property
+    def kernels(self) -> str:
+        """The kernels a step runs: "march", "tiled", "hybrid" or "per-pass"."""
+        v = C.lib().sfv_get_fused(self._h)
+        return {k: n for n, k in self.KERNELS.items()}[v]
 
     def set_fused(self, fused: bool) -> None:
         self.set_kernels("march" if fused else "per-pass")
