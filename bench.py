@@ -431,7 +431,7 @@ def variant_rain(t, x, y):
 def variants_bench(size: int, seed: int, steps: int, check: bool) -> dict:
     """The other configurations of the step on the terrain workload (same
     grid, bottom, initial state and friction), each timed device-resident on
-    the GPU: ForwardEuler and NeumannBC (the per-pass kernels), PeriodicBC and
+    the GPU: ForwardEuler and NeumannBC (the hybrid kernel path), PeriodicBC and
     NoReconstruction (the march kernels), FunctionalRain (the host evaluates
     the rain function on every cell and uploads the field before each step,
     so its rate includes that work), plus the 1D Stoker dam break (1024

@@ -143,7 +143,7 @@ struct sfv_handle {
         return SFV_OK;
     }
 
-    // U[2] is allocated on first use: only the per-pass RK2 path (its stage-1
+    // U[2] is allocated on first use: only the per-pass and hybrid RK2 paths (their stage-1
     // state) and kernel timing (a scratch output) need it.
     int ensure_scratch(std::string& why) {
         if (U[2]) return SFV_OK;
