@@ -150,3 +150,24 @@ def test_autotuned_height_is_chosen_once_per_size_class():
     sc = terrain(90, 60, seed=29)
     dts, U_hip, U_ref = run_both_compare(sc, 6, kernels="march")
     assert all(dh == dr for dh, dr in dts) and np.array_equal(U_hip, U_ref)
+
+
+def test_hybrid_path_height_tuned_and_results_height_independent():
+    """ForwardEuler / NeumannBC handles tune the march stage-1 height on their
+    first step like march handles, and every height gives the oracle's bits."""
+    sc = terrain(90, 60, seed=37, stepper=VF.ForwardEulerStepper())
+    a = make_hip(sc)
+    VF.perform_step(a, float("inf"))
+    assert a.kernels == "hybrid" and a.march_rows >= 1
+    for rows in (1, 7, 64):
+        sim = make_hip(sc)
+        sim.set_march_rows(rows)
+        o = make_oracle(sc)
+        t = 0.0
+        for k in range(10):
+            dr = o.step(float("inf"), t)
+            dh = VF.perform_step(sim, float("inf"))
+            assert dh == dr, (rows, k)
+            t += dr
+            sim.t = t
+        assert np.array_equal(sim.get_full_state(), o.state()), rows

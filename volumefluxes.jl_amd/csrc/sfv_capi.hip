@@ -179,12 +179,18 @@ int ensure_ready(sfv_handle* h) {
 // timed on the handle itself (full steps from its current state into the
 // spare ping-pong buffer, which the next step overwrites anyway), and the
 // winner is cached per device, precision and grid size class (quarter
-// octaves of nx and ny).  Every height gives bitwise the same results.
+// octaves of nx and ny).  Handles on the hybrid path (ForwardEuler,
+// NeumannBC) run only the stage-1 kernel, so they time it alone and keep
+// their own cache entries.  Every height gives bitwise the same results.
 // SFV_AUTOTUNE=0 in the environment keeps the build default.
 struct TuneKey {
-    int dev, prec, bx, by;
+    int dev, prec, bx, by, kind;   // kind: 0 full march steps, 1 stage 1 only
     bool operator<(const TuneKey& o) const {
-        return dev != o.dev ? dev < o.dev : prec != o.prec ? prec < o.prec : bx != o.bx ? bx < o.bx : by < o.by;
+        if (dev != o.dev) return dev < o.dev;
+        if (prec != o.prec) return prec < o.prec;
+        if (bx != o.bx) return bx < o.bx;
+        if (by != o.by) return by < o.by;
+        return kind < o.kind;
     }
 };
 std::mutex g_tune_mu;
@@ -202,6 +208,7 @@ hipError_t tune_steps(sfv_handle* h, int slot, int n) {
     hipError_t e = hipSuccess;
     for (int k = 0; k < n && e == hipSuccess; ++k) {
         e = launch_march<S>(h->stream, P, h->ctl, slot, 1, U0, nullptr, R, h->B);
+        if (h->fused == KV_HYBRID) continue;
         if (e == hipSuccess) e = launch_march<S>(h->stream, P, h->ctl, slot, 2, U0, R, Us, h->B);
         if (e == hipSuccess) e = launch_bc<S>(h->stream, P, h->ctl, slot, Us, BC_IN_STEP);
     }
@@ -218,7 +225,8 @@ bool autotune_enabled() {
 
 int cached_march_rows(sfv_handle* h) {
     if (h->march_rows > 0) return h->march_rows;
-    const TuneKey key{h->device, h->cfg.precision, size_bucket(h->cfg.nx), size_bucket(h->cfg.ny)};
+    const TuneKey key{h->device, h->cfg.precision, size_bucket(h->cfg.nx), size_bucket(h->cfg.ny),
+                      h->fused == KV_HYBRID ? 1 : 0};
     std::lock_guard<std::mutex> lk(g_tune_mu);
     auto it = g_tuned.find(key);
     if (it == g_tuned.end()) return 0;
@@ -228,7 +236,7 @@ int cached_march_rows(sfv_handle* h) {
 }
 
 int tune_march_rows(sfv_handle* h, std::string& why) {
-    if (h->fused != KV_MARCH || h->march_rows > 0) return SFV_OK;
+    if ((h->fused != KV_MARCH && h->fused != KV_HYBRID) || h->march_rows > 0) return SFV_OK;
     const int fallback = march_rows_per_wave(h->P, 1);
     if (!autotune_enabled()) {
         h->march_rows = fallback;
@@ -236,7 +244,8 @@ int tune_march_rows(sfv_handle* h, std::string& why) {
         return SFV_OK;
     }
     if (cached_march_rows(h) > 0) return SFV_OK;
-    const TuneKey key{h->device, h->cfg.precision, size_bucket(h->cfg.nx), size_bucket(h->cfg.ny)};
+    const TuneKey key{h->device, h->cfg.precision, size_bucket(h->cfg.nx), size_bucket(h->cfg.ny),
+                      h->fused == KV_HYBRID ? 1 : 0};
     std::vector<int> cand;
     for (int c : {1, 2, 3, 4, 6, 8, 12, 16, 24})
         if (c <= h->cfg.ny) cand.push_back(c);
