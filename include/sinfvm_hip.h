@@ -184,16 +184,18 @@ int sfv_device_count(void);
 
 /* Select the kernels of a step for this handle: 2 = fused RK2 stage kernels,
  * one wavefront per 60-column strip marching in y (the default whenever the
- * boundary-condition sources of the ghost cells are interior cells); 1 =
- * fused RK2 stage kernels tiled through LDS; 3 = every right-hand side from
+ * boundary-condition sources of the ghost cells are interior cells; unless a
+ * variant is set here, the first step's tuning switches to 1 when that is
+ * faster for the grid); 1 = fused RK2 stage kernels tiled through LDS; 3 =
+ * every right-hand side from
  * the march stage-1 kernel, the updates and boundary conditions from the
  * per-pass kernels (any stepper and boundary condition with gc >= 2: the
  * default for NeumannBC and ForwardEuler); 0 = one kernel per reference pass
  * (the device-side reference).  All give bitwise identical results.  No
  * counterpart in the reference. */
 int sfv_set_fused(sfv_handle* h, int32_t fused);
-/* The kernel variant a step of this handle runs (2, 1 or 0 as above; -1 for
- * a NULL handle).  No counterpart in the reference. */
+/* The kernel variant a step of this handle runs (0 to 3 as above; -1 for a
+ * NULL handle).  No counterpart in the reference. */
 int32_t sfv_get_fused(const sfv_handle* h);
 
 /* Rows per wave segment of the march kernels (sfv_set_fused(h, 2)).  0 (the

@@ -171,3 +171,20 @@ def test_hybrid_path_height_tuned_and_results_height_independent():
             t += dr
             sim.t = t
         assert np.array_equal(sim.get_full_state(), o.state()), rows
+
+
+def test_tuner_may_pick_tiled_kernels_and_results_stay_bitwise():
+    """A handle whose variant was not set times the LDS-tiled kernels next to
+    the march heights on its first step and takes the faster; a handle set to
+    "march" keeps it.  Either way the results are the oracle's bits."""
+    sc = terrain(300, 280, seed=41)
+    dts, U_hip, U_ref = run_both_compare(sc, 8)
+    assert all(dh == dr for dh, dr in dts) and np.array_equal(U_hip, U_ref)
+    auto = make_hip(sc)
+    VF.perform_step(auto, float("inf"))
+    assert auto.kernels in ("march", "tiled")
+    assert auto.march_rows >= 1
+    forced = make_hip(sc, "march")
+    VF.perform_step(forced, float("inf"))
+    assert forced.kernels == "march"
+    assert np.array_equal(auto.get_full_state(), forced.get_full_state())

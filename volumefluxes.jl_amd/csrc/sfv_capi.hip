@@ -179,7 +179,10 @@ int ensure_ready(sfv_handle* h) {
 // timed on the handle itself (full steps from its current state into the
 // spare ping-pong buffer, which the next step overwrites anyway), and the
 // winner is cached per device, precision and grid size class (quarter
-// octaves of nx and ny).  Handles on the hybrid path (ForwardEuler,
+// octaves of nx and ny).  A march handle whose kernel variant was not set
+// with sfv_set_fused also times the LDS-tiled kernels, which step grids of
+// roughly 200^2 to 512^2 faster (more waves, each with a shorter serial path),
+// and switches to them when they win.  Handles on the hybrid path (ForwardEuler,
 // NeumannBC) run only the stage-1 kernel, so they time it alone and keep
 // their own cache entries.  Every height gives bitwise the same results.
 // SFV_AUTOTUNE=0 in the environment keeps the build default.
@@ -194,12 +197,12 @@ struct TuneKey {
     }
 };
 std::mutex g_tune_mu;
-std::map<TuneKey, int> g_tuned;
+std::map<TuneKey, int> g_tuned;   // the march height; negative: the tiled kernels won (height = its magnitude)
 
 int size_bucket(int n) { return (int)std::lround(4.0 * std::log2((double)std::max(n, 1))); }
 
 template <class S>
-hipError_t tune_steps(sfv_handle* h, int slot, int n) {
+hipError_t tune_steps(sfv_handle* h, int slot, int n, bool tiled) {
     const KParams& P = h->P;
     const int cur = (int)(h->cur & 1);
     const S* U0 = (const S*)h->U[cur];
@@ -207,6 +210,12 @@ hipError_t tune_steps(sfv_handle* h, int slot, int n) {
     S* Us = (S*)h->U[cur ^ 1];
     hipError_t e = hipSuccess;
     for (int k = 0; k < n && e == hipSuccess; ++k) {
+        if (tiled) {
+            e = launch_stage1<S>(h->stream, P, h->ctl, slot, U0, R, h->B);
+            if (e == hipSuccess) e = launch_stage2<S>(h->stream, P, h->ctl, slot, U0, R, Us, h->B);
+            if (e == hipSuccess) e = launch_bc<S>(h->stream, P, h->ctl, slot, Us, BC_IN_STEP);
+            continue;
+        }
         e = launch_march<S>(h->stream, P, h->ctl, slot, 1, U0, nullptr, R, h->B);
         if (h->fused == KV_HYBRID) continue;
         if (e == hipSuccess) e = launch_march<S>(h->stream, P, h->ctl, slot, 2, U0, R, Us, h->B);
@@ -230,13 +239,17 @@ int cached_march_rows(sfv_handle* h) {
     std::lock_guard<std::mutex> lk(g_tune_mu);
     auto it = g_tuned.find(key);
     if (it == g_tuned.end()) return 0;
-    h->march_rows = it->second;
-    h->P.rows = it->second;
-    return it->second;
+    const int rows = it->second < 0 ? -it->second : it->second;
+    h->march_rows = rows;
+    h->P.rows = rows;
+    if (it->second < 0 && !h->variant_forced && h->fused == KV_MARCH) h->fused = KV_TILED;
+    return rows;
 }
 
 int tune_march_rows(sfv_handle* h, std::string& why) {
     if ((h->fused != KV_MARCH && h->fused != KV_HYBRID) || h->march_rows > 0) return SFV_OK;
+    // a march handle whose variant the caller left open also times the tiled kernels
+    const bool try_tiled = h->fused == KV_MARCH && !h->variant_forced;
     const int fallback = march_rows_per_wave(h->P, 1);
     if (!autotune_enabled()) {
         h->march_rows = fallback;
@@ -266,11 +279,11 @@ int tune_march_rows(sfv_handle* h, std::string& why) {
         e = launch_prepare(h->stream, h->P, h->ctl, slot, 0.0, NAN, 0, 0, 0.0, -1, 0, h->cfg.precision);
     // time n steps with `rows`; the first call also sizes n so one
     // measurement covers about a millisecond
-    auto timed = [&](int rows, int n, double& ms_out) -> hipError_t {
+    auto timed = [&](int rows, int n, double& ms_out, bool tiled = false) -> hipError_t {
         h->P.rows = rows;
-        hipError_t r = f64 ? tune_steps<double>(h, slot, 1) : tune_steps<float>(h, slot, 1);     // warm-up
+        hipError_t r = f64 ? tune_steps<double>(h, slot, 1, tiled) : tune_steps<float>(h, slot, 1, tiled);  // warm-up
         if (r == hipSuccess) r = hipEventRecord(e0, h->stream);
-        if (r == hipSuccess) r = f64 ? tune_steps<double>(h, slot, n) : tune_steps<float>(h, slot, n);
+        if (r == hipSuccess) r = f64 ? tune_steps<double>(h, slot, n, tiled) : tune_steps<float>(h, slot, n, tiled);
         if (r == hipSuccess) r = hipEventRecord(e1, h->stream);
         if (r == hipSuccess) r = hipEventSynchronize(e1);
         float ms = 0;
@@ -282,12 +295,19 @@ int tune_march_rows(sfv_handle* h, std::string& why) {
     if (e == hipSuccess) e = timed(fallback, 2, probe);
     const int n = (int)std::min(32.0, std::max(3.0, std::ceil(1.0 / std::max(probe, 1e-3))));
     std::vector<double> best(cand.size(), 1e30);
-    for (int sweep = 0; sweep < 2 && e == hipSuccess; ++sweep)
+    double best_tiled = 1e30;
+    for (int sweep = 0; sweep < 2 && e == hipSuccess; ++sweep) {
         for (size_t c = 0; c < cand.size() && e == hipSuccess; ++c) {
             double ms = 0;
             e = timed(cand[c], n, ms);
             if (e == hipSuccess) best[c] = std::min(best[c], ms);
         }
+        if (try_tiled && e == hipSuccess) {
+            double ms = 0;
+            e = timed(fallback, n, ms, true);
+            if (e == hipSuccess) best_tiled = std::min(best_tiled, ms);
+        }
+    }
     if (e != hipSuccess) return bail(e);
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
@@ -306,17 +326,23 @@ int tune_march_rows(sfv_handle* h, std::string& why) {
     for (size_t c = 1; c < cand.size(); ++c)
         if (best[c] < best[fast]) fast = c;
     int pick = cand[fast];
+    double pick_ms = best[fast];
     for (size_t c = 0; c < cand.size(); ++c)
-        if (cand[c] == fallback && best[c] <= margin * best[fast]) pick = fallback;
+        if (cand[c] == fallback && best[c] <= margin * best[fast]) { pick = fallback; pick_ms = best[c]; }
     h->march_rows = pick;
     h->P.rows = pick;
+    // the tiled kernels replace the march ones only by the same margin
+    const bool tiled_wins = try_tiled && best_tiled * margin < pick_ms;
+    if (tiled_wins) h->fused = KV_TILED;
     if (const char* dbg = std::getenv("SFV_TUNE_REPORT"); dbg && dbg[0] == '1') {
         std::fprintf(stderr, "[sfv autotune] %dx%d %s:", h->cfg.nx, h->cfg.ny, f64 ? "f64" : "f32");
         for (size_t c = 0; c < cand.size(); ++c) std::fprintf(stderr, " %d:%.4f", cand[c], best[c]);
-        std::fprintf(stderr, " -> %d (ms per step, %d steps per timing)\n", pick, n);
+        if (try_tiled) std::fprintf(stderr, " tiled:%.4f", best_tiled);
+        std::fprintf(stderr, " -> %s%d (ms per step, %d steps per timing)\n", tiled_wins ? "tiled, march " : "",
+                     pick, n);
     }
     std::lock_guard<std::mutex> lk(g_tune_mu);
-    g_tuned.emplace(key, pick);
+    g_tuned.emplace(key, tiled_wins ? -pick : pick);
     return SFV_OK;
 }
 }  // namespace sfv
@@ -651,6 +677,7 @@ int sfv_set_fused(sfv_handle* h, int32_t fused) {
         return fail(h, SFV_ERR_ARGUMENT,
                     "fused kernels need RungeKutta2, gc >= 2 and boundary-condition sources that are interior cells");
     h->fused = fused;
+    h->variant_forced = true;
     return SFV_OK;
 }
 

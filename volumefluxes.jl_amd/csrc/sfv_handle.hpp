@@ -46,6 +46,7 @@ struct sfv_handle {
     int fused = KV_MARCH;               // kernel variant, KV_*
     int march_rows = 0;                 // rows per march segment: 0 = not chosen yet (autotuned on first use)
     bool rows_forced = false;           // set by sfv_set_march_rows
+    bool variant_forced = false;        // set by sfv_set_fused: the tuner keeps the variant
     std::string err;
     // Owner-supplied pool (server handles): device blocks (class-rounded),
     // the stream and the pinned mirror come from it and go back to it.

@@ -128,6 +128,39 @@ __device__ __forceinline__ unsigned long long wave_max_u64m(unsigned long long v
     return v;
 }
 
+// Stage 1's wave-speed maxima of one block (its 4 waves) into the step slot.
+// Every wave of the block calls this exactly once per tile, also a wave whose
+// segment lies past the grid (with zeros): the maxima meet in LDS behind a
+// barrier and one lane does the atomics.  Per-wave atomics on the slot's two
+// addresses serialised when hundreds of short waves ended together (small
+// grids); one per block is a quarter of them.  `par` alternates between
+// consecutive tiles of a persistent block so the next tile's writes never
+// meet this tile's reads.  The maxima are reached by a few blocks early on:
+// the atomic is skipped when the slot already holds at least this value.
+__device__ __forceinline__ void block_max_to_slot_m(unsigned long long smx, unsigned long long smy, StepSlot* s,
+                                                    int par) {
+    __shared__ unsigned long long red[2][MWAVES][2];
+    smx = wave_max_u64m(smx);
+    smy = wave_max_u64m(smy);
+    const int lane = threadIdx.x & (MW - 1), w = threadIdx.x / MW;
+    if (lane == 0) {
+        red[par][w][0] = smx;
+        red[par][w][1] = smy;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 1; k < MWAVES; ++k) {
+            smx = red[par][k][0] > smx ? red[par][k][0] : smx;
+            smy = red[par][k][1] > smy ? red[par][k][1] : smy;
+        }
+        const unsigned long long cx = __hip_atomic_load(&s->wsmax[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const unsigned long long cy = __hip_atomic_load(&s->wsmax[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (smx > cx) atomicMax(&s->wsmax[0], smx);
+        if (smy > cy) atomicMax(&s->wsmax[1], smy);
+    }
+}
+
 __device__ __forceinline__ double Bload(const KParams& P, const double* __restrict__ B, int i, int j) {
     return (i <= P.nxT && j <= P.nyT) ? B[(size_t)i + (size_t)(P.nxT + 1) * j] : 0.0;
 }
@@ -163,13 +196,16 @@ __device__ __forceinline__ void stage1_cell(const KParams& P, const S x[3], cons
 template <class S, int STAGE, bool FAST = false>
 __device__ __forceinline__ void march_tile(const KParams& P, StepCtl* ctl, int slot, int my, int strip, int seg,
                                            const S* __restrict__ U0, const S* __restrict__ RHS1,
-                                           S* __restrict__ OUT, const double* __restrict__ B) {
+                                           S* __restrict__ OUT, const double* __restrict__ B, int par = 0) {
     const StepSlot& sl = ctl->slot[slot];
-    if (!sl.active) return;
+    if (!sl.active) return;                        // the same for every wave of the block
     const int lane = threadIdx.x & (MW - 1);
     const int gc = P.gc, nxT = P.nxT, nyT = P.nyT;
     const int j0 = gc + seg * my;
-    if (j0 >= nyT - gc) return;                    // whole wave
+    if (j0 >= nyT - gc) {                          // whole wave: past the grid
+        if (STAGE == 1) block_max_to_slot_m(0ull, 0ull, &ctl->slot[slot], par);
+        return;
+    }
     const int th = min(my, nyT - gc - j0);
     const int i = gc + strip * MX + lane - 2;      // this lane's column (>= 0 since gc >= 2)
     const bool in_dom = i < nxT;
@@ -436,20 +472,7 @@ __device__ __forceinline__ void march_tile(const KParams& P, StepCtl* ctl, int s
             unrolled_rows(body, r, std::make_index_sequence<SFV_MARCH_UNROLL>{});
     for (; r < th + 4; ++r) body(r, std::true_type{});
 
-    if (STAGE == 1) {
-        smx = wave_max_u64m(smx);
-        smy = wave_max_u64m(smy);
-        if (lane == 0) {
-            // The maxima are reached by a few waves early on: skip the atomic
-            // (all waves hit the same two addresses) when the slot already
-            // holds at least this wave's value.
-            StepSlot* s = &ctl->slot[slot];
-            const unsigned long long cx = __hip_atomic_load(&s->wsmax[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            const unsigned long long cy = __hip_atomic_load(&s->wsmax[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            if (smx > cx) atomicMax(&s->wsmax[0], smx);
-            if (smy > cy) atomicMax(&s->wsmax[1], smy);
-        }
-    }
+    if (STAGE == 1) block_max_to_slot_m(smx, smy, &ctl->slot[slot], par);
 
     // The gc x gc corner cells of the domain: never written by the boundary
     // condition (bc.jl:92-109) nor read by a stencil, but part of the state;
@@ -535,7 +558,8 @@ k_march_batch(const BatchMember* __restrict__ mem, const BatchHeader* __restrict
     const int x = blockIdx.x % N_XCD, k = blockIdx.x / N_XCD, per = gridDim.x / N_XCD;
     const int q = tiles / N_XCD, r = tiles % N_XCD;
     const int c0 = x * q + min(x, r), c1 = c0 + q + (x < r ? 1 : 0);
-    for (int t = c0 + k; t < c1; t += per) {
+    int par = 0;
+    for (int t = c0 + k; t < c1; t += per, par ^= 1) {
         const int m = batch_find<0>(mem, members, t);
         const BatchMember& M = mem[m];
         const int local = t - M.tile0;
@@ -544,7 +568,7 @@ k_march_batch(const BatchMember* __restrict__ mem, const BatchHeader* __restrict
         const int seg = sg * MWAVES + threadIdx.x / MW;
         const S* U0 = (const S*)M.U[cur];
         if (STAGE == 1)
-            march_tile<S, 1, FAST>(M.P, M.ctl, slot, M.my, strip, seg, U0, nullptr, (S*)M.RHS, M.B);
+            march_tile<S, 1, FAST>(M.P, M.ctl, slot, M.my, strip, seg, U0, nullptr, (S*)M.RHS, M.B, par);
         else
             march_tile<S, 2, FAST>(M.P, M.ctl, slot, M.my, strip, seg, U0, (const S*)M.RHS, (S*)M.U[cur ^ 1], M.B);
     }
