@@ -63,7 +63,6 @@ int BatchEngine::init(int precision, int capacity, std::string& why) {
     cus_ = cus;
     per_cu_ = std::min(march_batch_blocks_per_cu(precision, 1), march_batch_blocks_per_cu(precision, 2));
     size_grid();
-    blocks_bc_ = std::max(1, cus);
     return grow(std::max(1, capacity), why);
 }
 
@@ -288,19 +287,17 @@ void BatchEngine::plan_tuning() {
 
 int BatchEngine::upload_table(std::string& why) {
     choose_rows();
-    int tiles = 0, lines = 0;
+    int tiles = 0;
     for (size_t r = 0; r < rows_.size(); ++r) {
         BatchMember& M = slot_row_[rows_[r]];
         M.tile0 = tiles;
-        M.bc0 = lines;
         tiles += M.strips * M.seggroups;
-        lines += M.P.nx + M.P.ny;
         h_mem_[r] = M;
     }
     h_hdr_->members = (int)rows_.size();
     h_hdr_->tiles = tiles;
-    h_hdr_->bc_lines = lines;
     h_hdr_->prio = share_ == Share::GUEST ? 1 : 0;
+    h_hdr_->pad = 0;
     if (!rows_.empty())
         ETRY(hipMemcpyAsync(d_mem_, h_mem_, sizeof(BatchMember) * rows_.size(), hipMemcpyHostToDevice, stream_));
     ETRY(hipMemcpyAsync(d_hdr_, h_hdr_, sizeof(BatchHeader), hipMemcpyHostToDevice, stream_));
@@ -308,8 +305,9 @@ int BatchEngine::upload_table(std::string& why) {
     return SFV_OK;
 }
 
-// graph_steps RK2 steps (stage 1, stage 2, boundary condition each), launch
-// parity alternating 0, 1, ... on the engine's stream.
+// graph_steps RK2 steps (stage 1, then stage 2, which also fills the ghosts
+// and closes each member's step), launch parity alternating 0, 1, ... on the
+// engine's stream.
 int BatchEngine::capture(std::string& why) {
     ETRY(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
     hipError_t e = hipSuccess;
@@ -318,11 +316,9 @@ int BatchEngine::capture(std::string& why) {
         if (precision_ == SFV_F64) {
             e = launch_march_batch<double>(stream_, 1, d_mem_, d_hdr_, lp, blocks_stage_, fast_);
             if (e == hipSuccess) e = launch_march_batch<double>(stream_, 2, d_mem_, d_hdr_, lp, blocks_stage_, fast_);
-            if (e == hipSuccess) e = launch_bc_batch<double>(stream_, d_mem_, d_hdr_, lp, blocks_bc_);
         } else {
             e = launch_march_batch<float>(stream_, 1, d_mem_, d_hdr_, lp, blocks_stage_, fast_);
             if (e == hipSuccess) e = launch_march_batch<float>(stream_, 2, d_mem_, d_hdr_, lp, blocks_stage_, fast_);
-            if (e == hipSuccess) e = launch_bc_batch<float>(stream_, d_mem_, d_hdr_, lp, blocks_bc_);
         }
     }
     hipGraph_t g = nullptr;

@@ -119,10 +119,9 @@ int step_kernels(sfv_handle* h, int slot, int cur) {
         HIP_TRY(h, launch_bc<S>(st, P, h->ctl, slot, U2, BC_LAST_STAGE));
         return SFV_OK;
     }
-    if (h->fused == KV_MARCH) {
+    if (h->fused == KV_MARCH) {       // stage 2 fills the ghosts and closes the step
         HIP_TRY(h, launch_march<S>(st, P, h->ctl, slot, 1, U0, nullptr, RHS, h->B));
-        HIP_TRY(h, launch_march<S>(st, P, h->ctl, slot, 2, U0, RHS, U2, h->B));
-        HIP_TRY(h, launch_bc<S>(st, P, h->ctl, slot, U2, BC_LAST_STAGE));
+        HIP_TRY(h, launch_march<S>(st, P, h->ctl, slot, 2, U0, RHS, U2, h->B, 1));
         return SFV_OK;
     }
     if (h->fused == KV_TILED) {
@@ -218,8 +217,7 @@ hipError_t tune_steps(sfv_handle* h, int slot, int n, bool tiled) {
         }
         e = launch_march<S>(h->stream, P, h->ctl, slot, 1, U0, nullptr, R, h->B);
         if (h->fused == KV_HYBRID) continue;
-        if (e == hipSuccess) e = launch_march<S>(h->stream, P, h->ctl, slot, 2, U0, R, Us, h->B);
-        if (e == hipSuccess) e = launch_bc<S>(h->stream, P, h->ctl, slot, Us, BC_IN_STEP);
+        if (e == hipSuccess) e = launch_march<S>(h->stream, P, h->ctl, slot, 2, U0, R, Us, h->B);   // not closing
     }
     return e;
 }
@@ -611,9 +609,11 @@ int sfv_time_kernel(sfv_handle* h, const char* which, int32_t n, double* ms_out)
     }
     const int slot = (int)(h->step_parity & 1), cur = (int)(h->cur & 1);
     // The kernels run as in a step — stage 1, stage 2 (with the dt stage 1's
-    // wave speeds give), boundary condition — against a scratch output so the
-    // state is left as is, with an event between consecutive kernels; the
-    // named kernel's mean duration is returned.  The first step is a warm-up.
+    // wave speeds give), the boundary condition (its own kernel on the tiled
+    // path; inside stage 2 on the march path, where "bc" times 0) — against a
+    // scratch output without closing the step, so the state is left as is,
+    // with an event between consecutive kernels; the named kernel's mean
+    // duration is returned.  The first step is a warm-up.
     HIP_TRY(h, launch_prepare(h->stream, h->P, h->ctl, slot, 0.0, NAN, 0, 0, 0.0, -1, 0, h->cfg.precision));
     auto stage = [&](int which, auto* U0, auto* R, auto* Us) -> hipError_t {
         const KParams& P = h->P;
@@ -633,7 +633,7 @@ int sfv_time_kernel(sfv_handle* h, const char* which, int32_t n, double* ms_out)
         return k == 2 ? launch_bc<float>(h->stream, h->P, h->ctl, slot, Us, BC_IN_STEP) : stage(k + 1, U0, R, Us);
     };
     const bool copy = w == "copy";
-    const int nk = copy ? 1 : 3;
+    const int nk = copy ? 1 : h->fused == KV_MARCH ? 2 : 3;
     const int runs = n + 1;
     std::vector<hipEvent_t> ev((size_t)runs * (nk + 1));
     for (auto& e : ev) HIP_TRY(h, hipEventCreate(&e));

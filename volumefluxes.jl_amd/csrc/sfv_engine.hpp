@@ -106,7 +106,7 @@ private:
     int device_ = 0;
     int capacity_ = 0;
     int graph_steps_ = 8;         // even: launch parity is 0 at every replay start
-    int blocks_stage_ = 0, blocks_bc_ = 0;
+    int blocks_stage_ = 0;
     int per_cu_ = 1;              // resident stage-kernel blocks per CU
     Share share_ = Share::ALONE;
     hipStream_t stream_ = nullptr;

@@ -691,14 +691,13 @@ struct AdjBuffers {
     } while (0)
 
 // One step U0 -> U2 of the forward pass with the march kernels, in step
-// slot `slot` of a loop prepared once (its boundary condition advances t and
-// arms the next slot).
+// slot `slot` of a loop prepared once (stage 2 fills the ghosts, advances t
+// and arms the next slot).
 int march_step(sfv_handle* h, int slot, const double* U0, double* U2) {
     const KParams& P = h->P;
     hipStream_t st = h->stream;
     ATRY(launch_march<double>(st, P, h->ctl, slot, 1, U0, nullptr, (double*)h->RHS, h->B));
-    ATRY(launch_march<double>(st, P, h->ctl, slot, 2, U0, (const double*)h->RHS, U2, h->B));
-    ATRY(launch_bc<double>(st, P, h->ctl, slot, U2, BC_LAST_STAGE));
+    ATRY(launch_march<double>(st, P, h->ctl, slot, 2, U0, (const double*)h->RHS, U2, h->B, 1));
     return SFV_OK;
 }
 

@@ -47,10 +47,16 @@ struct KParams {
 
 // Per-step flags live in two slots indexed by step parity, so the kernel that
 // finishes step n can arm step n+1 while no kernel of step n reads that slot.
+// One step's control: the kernels of a step read only their own slot (and
+// the run-wide fields of StepCtl), and the step's last kernel arms the other
+// slot for the next step, so the step can close while its kernels still run.
 struct StepSlot {
     int32_t active;
     int32_t pad;
     unsigned long long wsmax[2];   // bit patterns of the non-negative max speeds (x, y)
+    double t;             // the step's start time (realtype value)
+    double rain_now;      // compute_rain(rain, t)
+    double infil_tfac;    // fc + (f0 - fc)*exp(-k t) for Horton, the rate for constant
 };
 
 struct StepCtl {
@@ -64,8 +70,6 @@ struct StepCtl {
     int32_t has_max_timestep;
     int64_t steps;        // steps completed in this call
     int64_t max_steps;    // <= 0: unlimited
-    double rain_now;      // compute_rain(rain, t)
-    double infil_tfac;    // fc + (f0 - fc)*exp(-k t) for Horton, the rate for constant
     int32_t nonfinite;    // a step produced a NaN/Inf dt
     int32_t precision;
 };
@@ -85,8 +89,7 @@ struct BatchMember {
     int my;               // rows per wave segment
     int strips, seggroups;
     int tile0;            // first block tile of this member in the batch order
-    int bc0;              // first boundary line of this member
-    int pad;
+    int pad[2];
     // run parameters, read by the batched prepare
     double t0, endtime, max_timestep;
     long long max_steps;
@@ -96,19 +99,18 @@ struct BatchMember {
 struct BatchHeader {
     int members;
     int tiles;            // sum of strips * seggroups
-    int bc_lines;         // sum of nx + ny
     int prio;             // nonzero: the batch kernels raise their waves' issue priority
+    int pad;
 };
 
-// The member owning flattened index t of the batch order: the last member
-// whose first index (tile0, or bc0 for boundary lines) is <= t.  Every member
-// has at least one tile and one line, so the first indices increase strictly.
-template <int LINES>
+// The member owning tile t of the batch order: the last member whose first
+// tile (tile0) is <= t.  Every member has at least one tile, so the first
+// tiles increase strictly.
 __device__ __forceinline__ int batch_find(const BatchMember* __restrict__ mem, int members, int t) {
     int lo = 0, hi = members - 1;
     while (lo < hi) {
         const int mid = (lo + hi + 1) >> 1;
-        const int first = LINES ? mem[mid].bc0 : mem[mid].tile0;
+        const int first = mem[mid].tile0;
         if (first <= t) lo = mid;
         else hi = mid - 1;
     }
@@ -156,24 +158,66 @@ __device__ inline double compute_rain(const KParams& P, double t) {
 // Non-directional sources on the w component, in source-list order
 // (system.jl:72-74): rain adds rate(t), infiltration subtracts its rate.
 template <class S>
-__device__ __forceinline__ S apply_sources(const KParams& P, const StepCtl* ctl, S w, size_t k) {
+__device__ __forceinline__ S apply_sources(const KParams& P, const StepSlot& sl, S w, size_t k) {
     auto infil = [&](S x) -> S {
-        if (P.infil_kind == 2) return (S)(x - P.factor[k] * ctl->infil_tfac);
-        if (P.infil_kind == 1) return (S)(x - ctl->infil_tfac);
+        if (P.infil_kind == 2) return (S)(x - P.factor[k] * sl.infil_tfac);
+        if (P.infil_kind == 1) return (S)(x - sl.infil_tfac);
         return x;
     };
     auto rain = [&](S x) -> S {
         if (P.rain_kind == 3) return (S)(x + P.rain_field[k]);
-        return P.rain_kind ? (S)(x + ctl->rain_now) : x;
+        return P.rain_kind ? (S)(x + sl.rain_now) : x;
     };
     return P.rain_first ? infil(rain(w)) : rain(infil(w));
+}
+
+// The time-dependent source rates of a step starting at t.
+__device__ inline void set_rates(const KParams& P, StepSlot& s, double t) {
+    s.t = t;
+    s.rain_now = compute_rain(P, t);
+    if (P.infil_kind == 2) s.infil_tfac = P.fc + (P.f0 - P.fc) * exp(-P.hk * t);
+    else if (P.infil_kind == 1) s.infil_tfac = P.infil_rate;
+    else s.infil_tfac = 0.0;
+}
+
+// Close the step of `slot` with its dt: t += dt (simulator.jl:205), count
+// it, and arm the next slot (active unless the run ends here).  One thread.
+template <class S, bool NO_INFIL = false>
+__device__ inline void close_step(const KParams& P, StepCtl* c, int slot) {
+    const S dt = (S)c->dt;
+    const S t = (S)((S)c->slot[slot].t + dt);
+    c->t = (double)t;
+    c->steps += 1;
+    if (!__builtin_isfinite((double)dt)) c->nonfinite = 1;
+    bool act = c->max_steps <= 0 || c->steps < c->max_steps;
+    if (c->endtime == c->endtime) act = act && (t < c->endtime);   // NaN endtime: fixed step count
+    StepSlot& nx = c->slot[1 - slot];
+    nx.wsmax[0] = 0ull;
+    nx.wsmax[1] = 0ull;
+    nx.active = act ? 1 : 0;
+    if (NO_INFIL) {             // the caller knows the run has no infiltration
+        nx.t = (double)t;
+        nx.rain_now = compute_rain(P, (double)t);
+        nx.infil_tfac = 0.0;
+    } else {
+        set_rates(P, nx, (double)t);
+    }
+}
+
+
+// The same, out of line, for the persistent batch kernels (their KParams
+// live in the member table): inlined into the tile loop, the rate code
+// costs the whole loop registers.
+template <class S, bool NO_INFIL = false>
+__device__ __attribute__((noinline)) void close_step_call(const KParams& P, StepCtl* c, int slot) {
+    close_step<S, NO_INFIL>(P, c, slot);
 }
 
 // timestep_computer (simulator.jl:147-150) with the max_dt rule of
 // simulate_to_time (simulator.jl:200-203).
 template <class S>
 __device__ __forceinline__ double stage1_dt(const KParams& P, const StepCtl* ctl, const StepSlot& sl) {
-    double max_dt = ctl->match_endtime ? ctl->endtime - (S)ctl->t : __builtin_inf();
+    double max_dt = ctl->match_endtime ? ctl->endtime - (S)sl.t : __builtin_inf();
     if (ctl->has_max_timestep) max_dt = jl_min(max_dt, ctl->max_timestep);
     const double ax = __longlong_as_double((long long)sl.wsmax[0]);
     const double ay = __longlong_as_double((long long)sl.wsmax[1]);

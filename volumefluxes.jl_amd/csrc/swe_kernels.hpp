@@ -53,8 +53,6 @@ hipError_t launch_march_batch(hipStream_t st, int stage, const BatchMember* mem,
                               int blocks, bool fast);
 // The configurations the specialised march kernels cover.
 bool march_fast(const KParams& P);
-template <class S>
-hipError_t launch_bc_batch(hipStream_t st, const BatchMember* mem, const BatchHeader* hdr, int lp, int blocks);
 hipError_t launch_prepare_batch(hipStream_t st, const BatchMember* mem, const BatchHeader* hdr, int members);
 // Resident workgroups per CU of the batched stage kernel (occupancy).
 int march_batch_blocks_per_cu(int precision, int stage);
@@ -72,10 +70,12 @@ hipError_t launch_interior_copy(hipStream_t st, const KParams& P, const void* pa
 template <class S>
 hipError_t launch_stage1(hipStream_t st, const KParams& P, StepCtl* ctl, int slot, const S* U0, S* RHS,
                          const double* B);
-// stage = 1: U0 -> OUT = RHS1 (+ speeds); stage = 2: U0, RHS1 -> OUT = U2.
+// stage = 1: U0 -> OUT = RHS1 (+ speeds); stage = 2: U0, RHS1 -> OUT = U2,
+// ghosts included (the step's boundary condition).  close = 1: stage 2 also
+// closes the step (t += dt, next slot armed), as k_bc's last stage does.
 template <class S>
 hipError_t launch_march(hipStream_t st, const KParams& P, StepCtl* ctl, int slot, int stage, const S* U0,
-                        const S* RHS1, S* OUT, const double* B);
+                        const S* RHS1, S* OUT, const double* B, int close = 0);
 template <class S>
 hipError_t launch_stage2(hipStream_t st, const KParams& P, StepCtl* ctl, int slot, const S* U0, const S* RHS1, S* U2,
                          const double* B);

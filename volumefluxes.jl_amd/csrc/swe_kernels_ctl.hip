@@ -5,12 +5,6 @@
 
 namespace sfv {
 
-__device__ void set_rates(const KParams& P, StepCtl* c, double t) {
-    c->rain_now = compute_rain(P, t);
-    if (P.infil_kind == 2) c->infil_tfac = P.fc + (P.f0 - P.fc) * exp(-P.hk * t);
-    else if (P.infil_kind == 1) c->infil_tfac = P.infil_rate;
-    else c->infil_tfac = 0.0;
-}
 
 // update_bc! (bc.jl:29-110): X ghosts of every interior row, then Y ghosts of
 // every interior column; corners are never written.  One thread per row or
@@ -60,21 +54,7 @@ __device__ __forceinline__ void bc_line(const KParams& P, StepCtl* ctl, int slot
                 U[v * n + cid(P, i, P.nyT - gc + g)] = nh ? (S)(-hi[g][v]) : hi[g][v];
             }
     }
-    if (last_stage && tid == 0) {
-        StepCtl* c = ctl;
-        const S dt = (S)c->dt;
-        const S t = (S)((S)c->t + dt);
-        c->t = (double)t;
-        c->steps += 1;
-        if (!__builtin_isfinite((double)dt)) c->nonfinite = 1;
-        bool act = c->max_steps <= 0 || c->steps < c->max_steps;
-        if (c->endtime == c->endtime) act = act && (t < c->endtime);   // NaN endtime: fixed step count
-        StepSlot& nx = c->slot[1 - slot];
-        nx.wsmax[0] = 0ull;
-        nx.wsmax[1] = 0ull;
-        nx.active = act ? 1 : 0;
-        set_rates(P, c, (double)t);
-    }
+    if (last_stage && tid == 0) close_step<S>(P, ctl, slot);
 }
 
 template <class S>
@@ -102,9 +82,9 @@ __device__ void prepare_ctl(const KParams& P, StepCtl* c, int slot, double t, do
         c->slot[s].wsmax[0] = 0ull;
         c->slot[s].wsmax[1] = 0ull;
         c->slot[s].active = 0;
+        set_rates(P, c->slot[s], t);
     }
     c->slot[slot].active = act ? 1 : 0;
-    set_rates(P, c, t);
 }
 
 __global__ void k_prepare(KParams P, StepCtl* ctl, int slot, double t, double endtime, int match_endtime,
@@ -113,21 +93,6 @@ __global__ void k_prepare(KParams P, StepCtl* ctl, int slot, double t, double en
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     prepare_ctl(P, ctl, slot, t, endtime, match_endtime, has_max_timestep, max_timestep, max_steps, until_endtime,
                 precision);
-}
-
-// Batched boundary condition (last stage of a step): a grid-stride walk over
-// the boundary lines of every member (issue priority as in k_march_batch).
-template <class S>
-__global__ void __launch_bounds__(256) k_bc_batch(const BatchMember* __restrict__ mem,
-                                                  const BatchHeader* __restrict__ hdr, int lp) {
-    const int members = hdr->members, lines = hdr->bc_lines;
-    if (hdr->prio) __builtin_amdgcn_s_setprio(3);
-    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < lines; t += gridDim.x * blockDim.x) {
-        const int m = batch_find<1>(mem, members, t);
-        const BatchMember& M = mem[m];
-        const int cur = (M.cur0 ^ lp) & 1, slot = (M.slot0 ^ lp) & 1;
-        bc_line<S>(M.P, M.ctl, slot, (S*)M.U[cur ^ 1], BC_LAST_STAGE, t - M.bc0);
-    }
 }
 
 // Batched prepare: one thread per member, with the member's run parameters.
@@ -139,17 +104,10 @@ __global__ void k_prepare_batch(const BatchMember* __restrict__ mem, const Batch
                 M.max_steps, M.until_endtime, M.precision);
 }
 
-template <class S>
-hipError_t launch_bc_batch(hipStream_t st, const BatchMember* mem, const BatchHeader* hdr, int lp, int blocks) {
-    hipLaunchKernelGGL(k_bc_batch<S>, dim3(blocks), dim3(256), 0, st, mem, hdr, lp);
-    return hipPeekAtLastError();
-}
 hipError_t launch_prepare_batch(hipStream_t st, const BatchMember* mem, const BatchHeader* hdr, int members) {
     hipLaunchKernelGGL(k_prepare_batch, dim3((members + 63) / 64), dim3(64), 0, st, mem, hdr);
     return hipPeekAtLastError();
 }
-template hipError_t launch_bc_batch<double>(hipStream_t, const BatchMember*, const BatchHeader*, int, int);
-template hipError_t launch_bc_batch<float>(hipStream_t, const BatchMember*, const BatchHeader*, int, int);
 
 template <class S>
 hipError_t launch_bc(hipStream_t st, const KParams& P, StepCtl* ctl, int slot, S* U, int mode) {

@@ -233,7 +233,7 @@ __global__ void __launch_bounds__(F_THREADS) k_stage1(KParams P, StepCtl* ctl, i
         const int cx = c % FT_X, cy = c / FT_X;
         if (cx >= T.tw || cy >= T.th) continue;
         const size_t k = (size_t)(T.i0 + cx) + (size_t)P.nxT * (T.j0 + cy);
-        acc[q][0] = apply_sources(P, ctl, acc[q][0], k);
+        acc[q][0] = apply_sources(P, ctl->slot[slot], acc[q][0], k);
 #pragma unroll
         for (int v = 0; v < 3; ++v) RHS[v * n + k] = acc[q][v];
     }
@@ -316,7 +316,7 @@ __global__ void __launch_bounds__(F_THREADS) k_stage2(KParams P, StepCtl* ctl, i
         if (cx >= T.tw || cy >= T.th) continue;
         const int i = T.i0 + cx, j = T.j0 + cy;
         const size_t k = (size_t)i + (size_t)nxT * j;
-        acc[q][0] = apply_sources(P, ctl, acc[q][0], k);
+        acc[q][0] = apply_sources(P, ctl->slot[slot], acc[q][0], k);
         const S x0 = s.u[0][cy + 2][cx + 2], x1 = s.u[1][cy + 2][cx + 2], x2 = s.u[2][cy + 2][cx + 2];
         S o[3];
         o[0] = (S)(x0 + dt_s * acc[q][0]);
@@ -356,7 +356,7 @@ __global__ void __launch_bounds__(F_THREADS) k_stage2(KParams P, StepCtl* ctl, i
             const int j = hi_y ? nyT - gc + r / gc : r / gc;
             const size_t k = (size_t)i + (size_t)nxT * j;
             const double Bm = B_cell(P, B, i, j);
-            const S src = apply_sources(P, ctl, S(0), k);
+            const S src = apply_sources(P, ctl->slot[slot], S(0), k);
             const S x0 = U0[k], x1 = U0[n + k], x2 = U0[2 * n + k];
             S u1[3];
             finish1<S>(P, x0, x1, x2, src, S(0), S(0), Bm, dt_d, dt_s, u1);

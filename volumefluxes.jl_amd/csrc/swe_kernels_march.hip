@@ -193,13 +193,19 @@ __device__ __forceinline__ void stage1_cell(const KParams& P, const S x[3], cons
 // finite nonzero θ, the bottom source, the 2D cell-bottom lookup,
 // power-of-two spacings, no friction or bh2021, no infiltration and a rain
 // rate uniform in space (march_fast), so those run-time checks fold away.
-template <class S, int STAGE, bool FAST = false>
+template <class S, int STAGE, bool FAST = false, bool BATCH = false>
 __device__ __forceinline__ void march_tile(const KParams& P, StepCtl* ctl, int slot, int my, int strip, int seg,
                                            const S* __restrict__ U0, const S* __restrict__ RHS1,
-                                           S* __restrict__ OUT, const double* __restrict__ B, int par = 0) {
+                                           S* __restrict__ OUT, const double* __restrict__ B, int par = 0,
+                                           int close = 0) {
     const StepSlot& sl = ctl->slot[slot];
-    if (!sl.active) return;                        // the same for every wave of the block
     const int lane = threadIdx.x & (MW - 1);
+    if (!sl.active) {                              // the same for every wave of the block
+        // an inactive step passes inactivity on, so steps launched past the
+        // end of a run stay no-ops
+        if (STAGE == 2 && close && strip == 0 && seg == 0 && lane == 0) ctl->slot[1 - slot].active = 0;
+        return;
+    }
     const int gc = P.gc, nxT = P.nxT, nyT = P.nyT;
     const int j0 = gc + seg * my;
     if (j0 >= nyT - gc) {                          // whole wave: past the grid
@@ -218,8 +224,8 @@ __device__ __forceinline__ void march_tile(const KParams& P, StepCtl* ctl, int s
     auto sdiv = [&](double x, double dd, double inv, int pow2) { return FAST ? x * inv : slope_div(x, dd, inv, pow2); };
     // FAST: no infiltration and a rain rate that is the same for every cell
     auto sources = [&](S w, size_t k) -> S {
-        if (FAST) return P.rain_kind ? (S)(w + ctl->rain_now) : w;
-        return apply_sources(P, ctl, w, k);
+        if (FAST) return P.rain_kind ? (S)(w + sl.rain_now) : w;
+        return apply_sources(P, ctl->slot[slot], w, k);
     };
 
     double dt_d = 0.0;
@@ -227,10 +233,6 @@ __device__ __forceinline__ void march_tile(const KParams& P, StepCtl* ctl, int s
     if (STAGE == 2) {
         dt_d = stage1_dt<S>(P, ctl, sl);
         dt_s = (S)dt_d;
-        if (strip == 0 && seg == 0 && lane == 0) {
-            ctl->dt = (double)dt_s;
-            ctl->dt_stage1 = dt_d;
-        }
     }
 
     // Column remap of a ghost column to its boundary-condition source (stage 2).
@@ -474,6 +476,47 @@ __device__ __forceinline__ void march_tile(const KParams& P, StepCtl* ctl, int s
 
     if (STAGE == 1) block_max_to_slot_m(smx, smy, &ctl->slot[slot], par);
 
+    // Stage 2 also writes the output's ghost cells (update_bc!, bc.jl:29-110):
+    // a ghost takes the value of its source cell, an interior cell here (the
+    // march kernels run only when the maps say so).  The wave whose segment
+    // holds a source cell copies it, reading back what its own lanes wrote
+    // (a lane's column is fixed), so the march loop itself carries no
+    // boundary work; the checks are the same for the whole wave.
+    if (STAGE == 2) {
+        auto copy_cell = [&](int si, int sj, int gi, int gj, int neg) {
+            const size_t ks = (size_t)si + (size_t)nxT * sj, kg = (size_t)gi + (size_t)nxT * gj;
+            const S a = OUT[ks], b = OUT[n + ks], c = OUT[2 * n + ks];
+            OUT[kg] = a;
+            OUT[n + kg] = neg == 1 ? (S)(-b) : b;
+            OUT[2 * n + kg] = neg == 2 ? (S)(-c) : c;
+        };
+        const int c0 = gc + strip * MX, c1 = c0 + MX, r1 = j0 + th;
+        for (int g = 0; g < gc; ++g) {
+            // Y ghosts of every interior column of the strip
+            const int sl_ = P.src_lo_y[g], sh_ = P.src_hi_y[g];
+            if (sl_ >= j0 && sl_ < r1 && out_col) copy_cell(i, sl_, i, g, P.flip_lo_y[g] ? 2 : 0);
+            if (sh_ >= j0 && sh_ < r1 && out_col) copy_cell(i, sh_, i, nyT - gc + g, P.flip_hi_y[g] ? 2 : 0);
+            // X ghosts of every row of the segment
+            const int xl = P.src_lo_x[g], xh = P.src_hi_x[g];
+            if (xl >= c0 && xl < c1 && i == xl)
+                for (int jj = j0; jj < r1; ++jj) copy_cell(xl, jj, g, jj, P.flip_lo_x[g] ? 1 : 0);
+            if (xh >= c0 && xh < c1 && i == xh)
+                for (int jj = j0; jj < r1; ++jj) copy_cell(xh, jj, nxT - gc + g, jj, P.flip_hi_x[g] ? 1 : 0);
+        }
+    }
+
+    // dt for the host and the step's closing (t += dt, next slot armed), by
+    // one lane: the other waves read only this step's slot, and the next
+    // slot is read only by the next launches.
+    if (STAGE == 2 && strip == 0 && seg == 0 && lane == 0) {
+        ctl->dt = (double)dt_s;
+        ctl->dt_stage1 = dt_d;
+        if (close) {
+            if (BATCH) close_step_call<S, FAST>(P, ctl, slot);
+            else close_step<S, FAST>(P, ctl, slot);
+        }
+    }
+
     // The gc x gc corner cells of the domain: never written by the boundary
     // condition (bc.jl:92-109) nor read by a stencil, but part of the state;
     // both stages run sources, RK update, friction and clamp on them.
@@ -533,14 +576,14 @@ template <class S, int STAGE, bool FAST>
 __global__ void __launch_bounds__(MW* MWAVES)
 __attribute__((amdgpu_waves_per_eu(STAGE == 1 ? SFV_MARCH_MIN_WAVES1 : SFV_MARCH_MIN_WAVES)))
 k_march(KParams P, StepCtl* ctl, int slot, int my, int strips, int seggroups, const S* __restrict__ U0,
-        const S* __restrict__ RHS1, S* __restrict__ OUT, const double* __restrict__ B) {
+        const S* __restrict__ RHS1, S* __restrict__ OUT, const double* __restrict__ B, int close) {
     const int t = xcd_tile(blockIdx.x, strips * seggroups);
 #if SFV_MARCH_XCD == 2
     const int strip = t % strips, sg = t / strips;
 #else
     const int strip = t / seggroups, sg = t % seggroups;
 #endif
-    march_tile<S, STAGE, FAST>(P, ctl, slot, my, strip, sg * MWAVES + threadIdx.x / MW, U0, RHS1, OUT, B);
+    march_tile<S, STAGE, FAST>(P, ctl, slot, my, strip, sg * MWAVES + threadIdx.x / MW, U0, RHS1, OUT, B, 0, close);
 }
 
 // Batched stage kernel: a persistent grid (a multiple of 8 workgroups) walks
@@ -560,7 +603,7 @@ k_march_batch(const BatchMember* __restrict__ mem, const BatchHeader* __restrict
     const int c0 = x * q + min(x, r), c1 = c0 + q + (x < r ? 1 : 0);
     int par = 0;
     for (int t = c0 + k; t < c1; t += per, par ^= 1) {
-        const int m = batch_find<0>(mem, members, t);
+        const int m = batch_find(mem, members, t);
         const BatchMember& M = mem[m];
         const int local = t - M.tile0;
         const int strip = local % M.strips, sg = local / M.strips;
@@ -568,9 +611,10 @@ k_march_batch(const BatchMember* __restrict__ mem, const BatchHeader* __restrict
         const int seg = sg * MWAVES + threadIdx.x / MW;
         const S* U0 = (const S*)M.U[cur];
         if (STAGE == 1)
-            march_tile<S, 1, FAST>(M.P, M.ctl, slot, M.my, strip, seg, U0, nullptr, (S*)M.RHS, M.B, par);
+            march_tile<S, 1, FAST, true>(M.P, M.ctl, slot, M.my, strip, seg, U0, nullptr, (S*)M.RHS, M.B, par);
         else
-            march_tile<S, 2, FAST>(M.P, M.ctl, slot, M.my, strip, seg, U0, (const S*)M.RHS, (S*)M.U[cur ^ 1], M.B);
+            march_tile<S, 2, FAST, true>(M.P, M.ctl, slot, M.my, strip, seg, U0, (const S*)M.RHS, (S*)M.U[cur ^ 1], M.B, 0,
+                                   1);
     }
 }
 
@@ -617,7 +661,7 @@ bool march_fast(const KParams& P) {
 
 template <class S>
 hipError_t launch_march(hipStream_t st, const KParams& P, StepCtl* ctl, int slot, int stage, const S* U0,
-                        const S* RHS1, S* OUT, const double* B) {
+                        const S* RHS1, S* OUT, const double* B, int close) {
     const int my = P.rows > 0 ? P.rows : march_rows_per_wave(P, stage);
     const int segs = (P.ny + my - 1) / my;
     const int strips = (P.nx + MX - 1) / MX, seggroups = (segs + MWAVES - 1) / MWAVES;
@@ -626,17 +670,17 @@ hipError_t launch_march(hipStream_t st, const KParams& P, StepCtl* ctl, int slot
     if (march_fast(P)) {
         if (stage == 1)
             hipLaunchKernelGGL((k_march<S, 1, true>), grid, block, 0, st, P, ctl, slot, my, strips, seggroups, U0, RHS1,
-                               OUT, B);
+                               OUT, B, close);
         else
             hipLaunchKernelGGL((k_march<S, 2, true>), grid, block, 0, st, P, ctl, slot, my, strips, seggroups, U0, RHS1,
-                               OUT, B);
+                               OUT, B, close);
     } else {
         if (stage == 1)
             hipLaunchKernelGGL((k_march<S, 1, false>), grid, block, 0, st, P, ctl, slot, my, strips, seggroups, U0,
-                               RHS1, OUT, B);
+                               RHS1, OUT, B, close);
         else
             hipLaunchKernelGGL((k_march<S, 2, false>), grid, block, 0, st, P, ctl, slot, my, strips, seggroups, U0,
-                               RHS1, OUT, B);
+                               RHS1, OUT, B, close);
     }
     return hipPeekAtLastError();
 }
@@ -674,8 +718,8 @@ template hipError_t launch_march_batch<double>(hipStream_t, int, const BatchMemb
 template hipError_t launch_march_batch<float>(hipStream_t, int, const BatchMember*, const BatchHeader*, int, int,
                                               bool);
 template hipError_t launch_march<double>(hipStream_t, const KParams&, StepCtl*, int, int, const double*,
-                                         const double*, double*, const double*);
+                                         const double*, double*, const double*, int);
 template hipError_t launch_march<float>(hipStream_t, const KParams&, StepCtl*, int, int, const float*, const float*,
-                                        float*, const double*);
+                                        float*, const double*, int);
 
 }  // namespace sfv

@@ -104,7 +104,7 @@ __global__ void __launch_bounds__(256) k_flux(KParams P, StepCtl* ctl, int slot,
                 }
             }
         }
-        acc[0] = apply_sources(P, ctl, acc[0], k);
+        acc[0] = apply_sources(P, ctl->slot[slot], acc[0], k);
 #pragma unroll
         for (int v = 0; v < 3; ++v) RHS[v * n + k] = acc[v];
     }
@@ -148,7 +148,7 @@ __global__ void __launch_bounds__(256) k_finish(KParams P, StepCtl* ctl, int slo
     for (int v = 0; v < 3; ++v) x[v] = X[v * n + k];
     const bool interior = i >= P.gc && i < P.nxT - P.gc && j >= P.gc && j < P.nyT - P.gc;
     if (rhs_interior && !interior) {
-        r[0] = apply_sources(P, ctl, S(0), k);
+        r[0] = apply_sources(P, ctl->slot[slot], S(0), k);
         r[1] = S(0);
         r[2] = S(0);
     } else {
