@@ -379,15 +379,23 @@ int BatchEngine::replay_end(std::string& why) {
             // A step's cost drifts as the flow develops, so each trial is set
             // against the rule's height timed first and last, interpolated to
             // the trial's place; another height must be 3% faster than that.
-            const int last = trial_ - 1;
-            int best = trials_[last];
+            // Heights within 1.5% of the best count as tied, and the one
+            // nearest the rule's height (in ratio) wins the tie.
+            const int last = trial_ - 1, r0 = trials_[last];
+            std::vector<float> ratio(trials_.size(), 1.f);
             float best_ratio = 0.97f;
             for (int i = 1; i < last; ++i) {
                 const float ref = trial_ms_[0] + (trial_ms_[last] - trial_ms_[0]) * (float)i / (float)last;
-                const float ratio = trial_ms_[i] / ref;
-                if (ref > 0.f && ratio < best_ratio) {
+                ratio[i] = ref > 0.f ? trial_ms_[i] / ref : 1.f;
+                best_ratio = std::min(best_ratio, ratio[i]);
+            }
+            int best = r0;
+            double best_dist = 1e30;
+            for (int i = 1; i < last; ++i) {
+                const double dist = std::fabs(std::log((double)trials_[i] / r0));
+                if (ratio[i] < 0.97f && ratio[i] <= best_ratio + 0.015f && dist < best_dist) {
                     best = trials_[i];
-                    best_ratio = ratio;
+                    best_dist = dist;
                 }
             }
             rows_override_ = best;

@@ -68,7 +68,8 @@ public:
     // neighbours in the list {1, 2, 3, 4, 6, 8, 12, 16, 24} (one below, two
     // above), one replay each, with the rule's choice first and last, and
     // then keep the fastest: the rule's choice unless another is more than
-    // 3% faster than the rule's time interpolated to the trial's place.
+    // 3% faster than the rule's time interpolated to the trial's place
+    // (heights within 1.5% of the fastest tie, the nearest to the rule's wins).
     // The choice is kept for later runs with the same member sizes.  Every
     // height gives bitwise the same results, so the timed replays are
     // ordinary steps of the run.
