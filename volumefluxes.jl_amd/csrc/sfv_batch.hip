@@ -185,7 +185,17 @@ void BatchEngine::remove(int slot) {
         const StepCtl& c = h_ctl_[slot];
         h->cur = (h->cur + c.steps) & 1;
         h->step_parity = (h->step_parity + c.steps) & 1;
-        if (c.steps > 0) { h->last_dt = c.dt; h->have_dt = true; }
+        if (c.steps > 0) {
+            h->last_dt = c.dt;
+            h->have_dt = true;
+            // the march kernels leave the output's ghost cells unwritten: fill
+            // them on the member's own stream before it is read or stepped alone
+            const int ps = (int)(h->step_parity & 1);
+            if (precision_ == SFV_F64)
+                (void)launch_bc<double>(h->stream, h->P, h->ctl, ps, (double*)h->U[h->cur & 1], BC_STANDALONE);
+            else
+                (void)launch_bc<float>(h->stream, h->P, h->ctl, ps, (float*)h->U[h->cur & 1], BC_STANDALONE);
+        }
     }
     slot_handle_[slot] = nullptr;
     ran_[slot] = 0;

@@ -119,8 +119,10 @@ int step_kernels(sfv_handle* h, int slot, int cur) {
         HIP_TRY(h, launch_bc<S>(st, P, h->ctl, slot, U2, BC_LAST_STAGE));
         return SFV_OK;
     }
-    if (h->fused == KV_MARCH) {       // stage 2 fills the ghosts and closes the step
-        HIP_TRY(h, launch_march<S>(st, P, h->ctl, slot, 1, U0, nullptr, RHS, h->B));
+    if (h->fused == KV_MARCH) {
+        // both stages read ghosts through the boundary maps and stage 2 closes
+        // the step; the state's ghost cells are filled when the loop ends
+        HIP_TRY(h, launch_march<S>(st, P, h->ctl, slot, 1, U0, nullptr, RHS, h->B, 0, 1));
         HIP_TRY(h, launch_march<S>(st, P, h->ctl, slot, 2, U0, RHS, U2, h->B, 1));
         return SFV_OK;
     }
@@ -215,7 +217,7 @@ hipError_t tune_steps(sfv_handle* h, int slot, int n, bool tiled) {
             if (e == hipSuccess) e = launch_bc<S>(h->stream, P, h->ctl, slot, Us, BC_IN_STEP);
             continue;
         }
-        e = launch_march<S>(h->stream, P, h->ctl, slot, 1, U0, nullptr, R, h->B);
+        e = launch_march<S>(h->stream, P, h->ctl, slot, 1, U0, nullptr, R, h->B, 0, h->fused == KV_MARCH ? 1 : 0);
         if (h->fused == KV_HYBRID) continue;
         if (e == hipSuccess) e = launch_march<S>(h->stream, P, h->ctl, slot, 2, U0, R, Us, h->B);   // not closing
     }
@@ -347,6 +349,16 @@ int tune_march_rows(sfv_handle* h, std::string& why) {
 
 namespace {
 
+// update_bc! on the handle's current state (no step closing), on its stream.
+int fill_ghosts(sfv_handle* h) {
+    const int slot = (int)(h->step_parity & 1);
+    if (h->cfg.precision == SFV_F64)
+        HIP_TRY(h, launch_bc<double>(h->stream, h->P, h->ctl, slot, (double*)h->U[h->cur & 1], BC_STANDALONE));
+    else
+        HIP_TRY(h, launch_bc<float>(h->stream, h->P, h->ctl, slot, (float*)h->U[h->cur & 1], BC_STANDALONE));
+    return SFV_OK;
+}
+
 // Run a prepared step loop of at most `nsteps` launched steps, in batches, and
 // finish when the device says the loop ended.
 int run_loop(sfv_handle* h, double t0, double endtime, int match_endtime, int has_max, double max_ts,
@@ -386,6 +398,11 @@ int run_loop(sfv_handle* h, double t0, double endtime, int match_endtime, int ha
     const int64_t done = h->ctl_host->steps;
     h->cur = (h->cur + done) & 1;
     h->step_parity = (h->step_parity + done) & 1;
+    if (done > 0 && h->fused == KV_MARCH) {
+        // the march kernels leave the output's ghost cells unwritten
+        const int rf = fill_ghosts(h);
+        if (rf) return rf;
+    }
     if (done > 0) { h->last_dt = h->ctl_host->dt; h->have_dt = true; }
     if (steps_out) *steps_out = done;
     if (t_out) *t_out = h->ctl_host->t;
@@ -620,7 +637,7 @@ int sfv_time_kernel(sfv_handle* h, const char* which, int32_t n, double* ms_out)
         if (h->fused == KV_TILED)
             return which == 1 ? launch_stage1(h->stream, P, h->ctl, slot, U0, R, h->B)
                               : launch_stage2(h->stream, P, h->ctl, slot, U0, R, Us, h->B);
-        return which == 1 ? launch_march(h->stream, P, h->ctl, slot, 1, U0, (decltype(U0))nullptr, R, h->B)
+        return which == 1 ? launch_march(h->stream, P, h->ctl, slot, 1, U0, (decltype(U0))nullptr, R, h->B, 0, 1)
                           : launch_march(h->stream, P, h->ctl, slot, 2, U0, R, Us, h->B);
     };
     auto kernel = [&](int k) -> hipError_t {      // 0 stage 1, 1 stage 2, 2 boundary condition, 3 copy

@@ -691,12 +691,13 @@ struct AdjBuffers {
     } while (0)
 
 // One step U0 -> U2 of the forward pass with the march kernels, in step
-// slot `slot` of a loop prepared once (stage 2 fills the ghosts, advances t
-// and arms the next slot).
+// slot `slot` of a loop prepared once (stage 2 advances t and arms the next
+// slot; U2's ghost cells are left unwritten and filled where a state leaves
+// the march path).  The adjoint accepts only handles the march path serves.
 int march_step(sfv_handle* h, int slot, const double* U0, double* U2) {
     const KParams& P = h->P;
     hipStream_t st = h->stream;
-    ATRY(launch_march<double>(st, P, h->ctl, slot, 1, U0, nullptr, (double*)h->RHS, h->B));
+    ATRY(launch_march<double>(st, P, h->ctl, slot, 1, U0, nullptr, (double*)h->RHS, h->B, 0, 1));
     ATRY(launch_march<double>(st, P, h->ctl, slot, 2, U0, (const double*)h->RHS, U2, h->B, 1));
     return SFV_OK;
 }
@@ -789,7 +790,11 @@ extern "C" int sfv_adjoint(sfv_handle* h, int64_t nsteps, int32_t checkpoint_eve
     for (int64_t s = 0; s < nsteps; ++s) {
         double* a = seg + (size_t)(s & 1) * n3;
         double* b = seg + (size_t)((s + 1) & 1) * n3;
-        if (s % ce == 0) ATRY(hipMemcpyAsync(ck + (size_t)(s / ce) * n3, a, n3 * sizeof(double), hipMemcpyDeviceToDevice, st));
+        if (s % ce == 0) {
+            // a checkpoint restarts the per-pass recompute, which reads ghosts
+            if (s > 0) ATRY(launch_bc<double>(st, P, h->ctl, 0, a, BC_STANDALONE));
+            ATRY(hipMemcpyAsync(ck + (size_t)(s / ce) * n3, a, n3 * sizeof(double), hipMemcpyDeviceToDevice, st));
+        }
         if ((rc = march_step(h, (int)(s & 1), a, b))) return rc;
         ATRY(hipMemcpyAsync(d_times + s + 1, &h->ctl->t, sizeof(double), hipMemcpyDeviceToDevice, st));
     }
@@ -800,7 +805,8 @@ extern "C" int sfv_adjoint(sfv_handle* h, int64_t nsteps, int32_t checkpoint_eve
     const double t = times[(size_t)nsteps];
     const double last_dt = h->ctl_host->dt;
     double* UT = seg + (size_t)(nsteps & 1) * n3;
-    // the final state becomes the handle's state
+    // the final state, ghosts filled, becomes the handle's state
+    if (nsteps > 0) ATRY(launch_bc<double>(st, P, h->ctl, 0, UT, BC_STANDALONE));
     ATRY(hipMemcpyAsync(cur, UT, n3 * sizeof(double), hipMemcpyDeviceToDevice, st));
 
     // objective and its adjoint

@@ -70,12 +70,15 @@ hipError_t launch_interior_copy(hipStream_t st, const KParams& P, const void* pa
 template <class S>
 hipError_t launch_stage1(hipStream_t st, const KParams& P, StepCtl* ctl, int slot, const S* U0, S* RHS,
                          const double* B);
-// stage = 1: U0 -> OUT = RHS1 (+ speeds); stage = 2: U0, RHS1 -> OUT = U2,
-// ghosts included (the step's boundary condition).  close = 1: stage 2 also
+// stage = 1: U0 -> OUT = RHS1 (+ speeds); stage = 2: U0, RHS1 -> OUT = U2
+// (interior and domain corners).  Stage 2 reads halo ghost cells through the
+// boundary-condition maps; stage 1 too with remap_halo = 1 (only when the
+// maps name interior cells), so U2's ghosts are left unwritten and filled
+// when something outside the march path needs them.  close = 1: stage 2 also
 // closes the step (t += dt, next slot armed), as k_bc's last stage does.
 template <class S>
 hipError_t launch_march(hipStream_t st, const KParams& P, StepCtl* ctl, int slot, int stage, const S* U0,
-                        const S* RHS1, S* OUT, const double* B, int close = 0);
+                        const S* RHS1, S* OUT, const double* B, int close = 0, int remap_halo = 0);
 template <class S>
 hipError_t launch_stage2(hipStream_t st, const KParams& P, StepCtl* ctl, int slot, const S* U0, const S* RHS1, S* U2,
                          const double* B);

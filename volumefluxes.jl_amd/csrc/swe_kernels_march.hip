@@ -193,7 +193,7 @@ __device__ __forceinline__ void stage1_cell(const KParams& P, const S x[3], cons
 // finite nonzero θ, the bottom source, the 2D cell-bottom lookup,
 // power-of-two spacings, no friction or bh2021, no infiltration and a rain
 // rate uniform in space (march_fast), so those run-time checks fold away.
-template <class S, int STAGE, bool FAST = false, bool BATCH = false>
+template <class S, int STAGE, bool FAST = false, bool BATCH = false, bool RMP1 = false>
 __device__ __forceinline__ void march_tile(const KParams& P, StepCtl* ctl, int slot, int my, int strip, int seg,
                                            const S* __restrict__ U0, const S* __restrict__ RHS1,
                                            S* __restrict__ OUT, const double* __restrict__ B, int par = 0,
@@ -235,9 +235,22 @@ __device__ __forceinline__ void march_tile(const KParams& P, StepCtl* ctl, int s
         dt_s = (S)dt_d;
     }
 
-    // Column remap of a ghost column to its boundary-condition source (stage 2).
+    // Column remap of a ghost column to its boundary-condition source: stage
+    // 2 always (it rebuilds U1 there from the source's U0 and RHS), stage 1
+    // in its RMP1 instantiation (the ghost's value is its source's, with the
+    // sign flip, as update_bc! would have stored it; corners, which the
+    // boundary condition never writes, feed no output).  With both stages
+    // reading through the maps, the march path never reads a ghost cell from
+    // memory.
+    constexpr bool rmp = STAGE == 2 || RMP1;
+    // Only the edge strips and segments hold ghost columns or rows; the test
+    // is the same for the whole wave, so interior waves skip the maps.  The
+    // lanes span columns gc + strip*MX - 2 .. gc + strip*MX + 61, the rows
+    // loaded are j0 - 2 .. j0 + th + 1.
+    const bool xedge = rmp && (strip == 0 || gc + strip * MX + MW - 3 >= nxT - gc);
+    const bool yedge = rmp && (j0 - 2 < gc || j0 + th + 1 >= nyT - gc);
     int sci = i, fxc = 0;
-    if (STAGE == 2 && in_dom) {
+    if (xedge && in_dom) {
         if (i < gc) { sci = P.src_lo_x[i]; fxc = P.flip_lo_x[i]; }
         else if (i >= nxT - gc) { sci = P.src_hi_x[i - (nxT - gc)]; fxc = P.flip_hi_x[i - (nxT - gc)]; }
     }
@@ -260,15 +273,19 @@ __device__ __forceinline__ void march_tile(const KParams& P, StepCtl* ctl, int s
             return;
         }
         int sj = jr;
-        if (STAGE == 2) {
+        if (yedge) {
             if (jr < gc) { sj = P.src_lo_y[jr]; o.fy = P.flip_lo_y[jr]; }
             else if (jr >= nyT - gc) { sj = P.src_hi_y[jr - (nyT - gc)]; o.fy = P.flip_hi_y[jr - (nyT - gc)]; }
         }
-        const size_t k = (size_t)(STAGE == 2 ? sci : i) + (size_t)nxT * sj;
+        const size_t k = (size_t)sci + (size_t)nxT * sj;
         o.x[0] = U0[k]; o.x[1] = U0[n + k]; o.x[2] = U0[2 * n + k];
+        if (STAGE == 1) {
+            if (xedge && fxc) o.x[1] = (S)(-o.x[1]);
+            if (o.fy) o.x[2] = (S)(-o.x[2]);
+        }
         if (STAGE == 2) {
             o.r[0] = RHS1[k]; o.r[1] = RHS1[n + k]; o.r[2] = RHS1[2 * n + k];
-            o.remap = sci != i || sj != jr;
+            o.remap = (xedge && sci != i) || sj != jr;
             if (o.remap)   // the source cell's bottom
                 o.bsrc = bc4(Bload(P, B, sci, sj), Bload(P, B, sci + 1, sj), Bload(P, B, sci, sj + 1),
                                 Bload(P, B, sci + 1, sj + 1));
@@ -476,35 +493,6 @@ __device__ __forceinline__ void march_tile(const KParams& P, StepCtl* ctl, int s
 
     if (STAGE == 1) block_max_to_slot_m(smx, smy, &ctl->slot[slot], par);
 
-    // Stage 2 also writes the output's ghost cells (update_bc!, bc.jl:29-110):
-    // a ghost takes the value of its source cell, an interior cell here (the
-    // march kernels run only when the maps say so).  The wave whose segment
-    // holds a source cell copies it, reading back what its own lanes wrote
-    // (a lane's column is fixed), so the march loop itself carries no
-    // boundary work; the checks are the same for the whole wave.
-    if (STAGE == 2) {
-        auto copy_cell = [&](int si, int sj, int gi, int gj, int neg) {
-            const size_t ks = (size_t)si + (size_t)nxT * sj, kg = (size_t)gi + (size_t)nxT * gj;
-            const S a = OUT[ks], b = OUT[n + ks], c = OUT[2 * n + ks];
-            OUT[kg] = a;
-            OUT[n + kg] = neg == 1 ? (S)(-b) : b;
-            OUT[2 * n + kg] = neg == 2 ? (S)(-c) : c;
-        };
-        const int c0 = gc + strip * MX, c1 = c0 + MX, r1 = j0 + th;
-        for (int g = 0; g < gc; ++g) {
-            // Y ghosts of every interior column of the strip
-            const int sl_ = P.src_lo_y[g], sh_ = P.src_hi_y[g];
-            if (sl_ >= j0 && sl_ < r1 && out_col) copy_cell(i, sl_, i, g, P.flip_lo_y[g] ? 2 : 0);
-            if (sh_ >= j0 && sh_ < r1 && out_col) copy_cell(i, sh_, i, nyT - gc + g, P.flip_hi_y[g] ? 2 : 0);
-            // X ghosts of every row of the segment
-            const int xl = P.src_lo_x[g], xh = P.src_hi_x[g];
-            if (xl >= c0 && xl < c1 && i == xl)
-                for (int jj = j0; jj < r1; ++jj) copy_cell(xl, jj, g, jj, P.flip_lo_x[g] ? 1 : 0);
-            if (xh >= c0 && xh < c1 && i == xh)
-                for (int jj = j0; jj < r1; ++jj) copy_cell(xh, jj, nxT - gc + g, jj, P.flip_hi_x[g] ? 1 : 0);
-        }
-    }
-
     // dt for the host and the step's closing (t += dt, next slot armed), by
     // one lane: the other waves read only this step's slot, and the next
     // slot is read only by the next launches.
@@ -570,20 +558,33 @@ __device__ __forceinline__ int xcd_tile(int L, int N) {
 #endif
 }
 
+// Does the wave of (strip, seg) load a ghost column or row?  Its lanes span
+// columns gc + strip*MX - 2 .. gc + strip*MX + 61 and it loads rows
+// j0 - 2 .. j0 + th + 1.  Only such waves run stage 1's remapping code.
+__device__ __forceinline__ bool march_edge(const KParams& P, int my, int strip, int seg) {
+    const int gc = P.gc, j0 = gc + seg * my;
+    const int th = min(my, P.nyT - gc - j0);
+    return strip == 0 || gc + strip * MX + MW - 3 >= P.nxT - gc || j0 - 2 < gc || j0 + th + 1 >= P.nyT - gc;
+}
+
 // 1D grid of strips x seggroups blocks; a block is 4 waves on 4 consecutive
 // segments of one strip.
 template <class S, int STAGE, bool FAST>
 __global__ void __launch_bounds__(MW* MWAVES)
 __attribute__((amdgpu_waves_per_eu(STAGE == 1 ? SFV_MARCH_MIN_WAVES1 : SFV_MARCH_MIN_WAVES)))
 k_march(KParams P, StepCtl* ctl, int slot, int my, int strips, int seggroups, const S* __restrict__ U0,
-        const S* __restrict__ RHS1, S* __restrict__ OUT, const double* __restrict__ B, int close) {
+        const S* __restrict__ RHS1, S* __restrict__ OUT, const double* __restrict__ B, int close, int remap) {
     const int t = xcd_tile(blockIdx.x, strips * seggroups);
 #if SFV_MARCH_XCD == 2
     const int strip = t % strips, sg = t / strips;
 #else
     const int strip = t / seggroups, sg = t % seggroups;
 #endif
-    march_tile<S, STAGE, FAST>(P, ctl, slot, my, strip, sg * MWAVES + threadIdx.x / MW, U0, RHS1, OUT, B, 0, close);
+    const int seg = sg * MWAVES + threadIdx.x / MW;
+    if (STAGE == 1 && remap && march_edge(P, my, strip, seg))
+        march_tile<S, 1, FAST, false, true>(P, ctl, slot, my, strip, seg, U0, RHS1, OUT, B);
+    else
+        march_tile<S, STAGE, FAST>(P, ctl, slot, my, strip, seg, U0, RHS1, OUT, B, 0, close);
 }
 
 // Batched stage kernel: a persistent grid (a multiple of 8 workgroups) walks
@@ -611,7 +612,10 @@ k_march_batch(const BatchMember* __restrict__ mem, const BatchHeader* __restrict
         const int seg = sg * MWAVES + threadIdx.x / MW;
         const S* U0 = (const S*)M.U[cur];
         if (STAGE == 1)
-            march_tile<S, 1, FAST, true>(M.P, M.ctl, slot, M.my, strip, seg, U0, nullptr, (S*)M.RHS, M.B, par);
+            if (march_edge(M.P, M.my, strip, seg))
+                march_tile<S, 1, FAST, true, true>(M.P, M.ctl, slot, M.my, strip, seg, U0, nullptr, (S*)M.RHS, M.B, par);
+            else
+                march_tile<S, 1, FAST, true>(M.P, M.ctl, slot, M.my, strip, seg, U0, nullptr, (S*)M.RHS, M.B, par);
         else
             march_tile<S, 2, FAST, true>(M.P, M.ctl, slot, M.my, strip, seg, U0, (const S*)M.RHS, (S*)M.U[cur ^ 1], M.B, 0,
                                    1);
@@ -661,7 +665,7 @@ bool march_fast(const KParams& P) {
 
 template <class S>
 hipError_t launch_march(hipStream_t st, const KParams& P, StepCtl* ctl, int slot, int stage, const S* U0,
-                        const S* RHS1, S* OUT, const double* B, int close) {
+                        const S* RHS1, S* OUT, const double* B, int close, int remap_halo) {
     const int my = P.rows > 0 ? P.rows : march_rows_per_wave(P, stage);
     const int segs = (P.ny + my - 1) / my;
     const int strips = (P.nx + MX - 1) / MX, seggroups = (segs + MWAVES - 1) / MWAVES;
@@ -670,17 +674,17 @@ hipError_t launch_march(hipStream_t st, const KParams& P, StepCtl* ctl, int slot
     if (march_fast(P)) {
         if (stage == 1)
             hipLaunchKernelGGL((k_march<S, 1, true>), grid, block, 0, st, P, ctl, slot, my, strips, seggroups, U0, RHS1,
-                               OUT, B, close);
+                               OUT, B, close, remap_halo);
         else
             hipLaunchKernelGGL((k_march<S, 2, true>), grid, block, 0, st, P, ctl, slot, my, strips, seggroups, U0, RHS1,
-                               OUT, B, close);
+                               OUT, B, close, remap_halo);
     } else {
         if (stage == 1)
             hipLaunchKernelGGL((k_march<S, 1, false>), grid, block, 0, st, P, ctl, slot, my, strips, seggroups, U0,
-                               RHS1, OUT, B, close);
+                               RHS1, OUT, B, close, remap_halo);
         else
             hipLaunchKernelGGL((k_march<S, 2, false>), grid, block, 0, st, P, ctl, slot, my, strips, seggroups, U0,
-                               RHS1, OUT, B, close);
+                               RHS1, OUT, B, close, remap_halo);
     }
     return hipPeekAtLastError();
 }
@@ -718,8 +722,8 @@ template hipError_t launch_march_batch<double>(hipStream_t, int, const BatchMemb
 template hipError_t launch_march_batch<float>(hipStream_t, int, const BatchMember*, const BatchHeader*, int, int,
                                               bool);
 template hipError_t launch_march<double>(hipStream_t, const KParams&, StepCtl*, int, int, const double*,
-                                         const double*, double*, const double*, int);
+                                         const double*, double*, const double*, int, int);
 template hipError_t launch_march<float>(hipStream_t, const KParams&, StepCtl*, int, int, const float*, const float*,
-                                        float*, const double*, int);
+                                        float*, const double*, int, int);
 
 }  // namespace sfv
