@@ -126,10 +126,9 @@ int step_kernels(sfv_handle* h, int slot, int cur) {
         HIP_TRY(h, launch_march<S>(st, P, h->ctl, slot, 2, U0, RHS, U2, h->B, 1));
         return SFV_OK;
     }
-    if (h->fused == KV_TILED) {
+    if (h->fused == KV_TILED) {      // as the march path: ghosts through the maps, stage 2 closes
         HIP_TRY(h, launch_stage1<S>(st, P, h->ctl, slot, U0, RHS, h->B));
-        HIP_TRY(h, launch_stage2<S>(st, P, h->ctl, slot, U0, RHS, U2, h->B));
-        HIP_TRY(h, launch_bc<S>(st, P, h->ctl, slot, U2, BC_LAST_STAGE));
+        HIP_TRY(h, launch_stage2<S>(st, P, h->ctl, slot, U0, RHS, U2, h->B, 1));
         return SFV_OK;
     }
     S* U1 = (S*)h->U[2];
@@ -213,8 +212,7 @@ hipError_t tune_steps(sfv_handle* h, int slot, int n, bool tiled) {
     for (int k = 0; k < n && e == hipSuccess; ++k) {
         if (tiled) {
             e = launch_stage1<S>(h->stream, P, h->ctl, slot, U0, R, h->B);
-            if (e == hipSuccess) e = launch_stage2<S>(h->stream, P, h->ctl, slot, U0, R, Us, h->B);
-            if (e == hipSuccess) e = launch_bc<S>(h->stream, P, h->ctl, slot, Us, BC_IN_STEP);
+            if (e == hipSuccess) e = launch_stage2<S>(h->stream, P, h->ctl, slot, U0, R, Us, h->B);   // not closing
             continue;
         }
         e = launch_march<S>(h->stream, P, h->ctl, slot, 1, U0, nullptr, R, h->B, 0, h->fused == KV_MARCH ? 1 : 0);
@@ -398,8 +396,8 @@ int run_loop(sfv_handle* h, double t0, double endtime, int match_endtime, int ha
     const int64_t done = h->ctl_host->steps;
     h->cur = (h->cur + done) & 1;
     h->step_parity = (h->step_parity + done) & 1;
-    if (done > 0 && h->fused == KV_MARCH) {
-        // the march kernels leave the output's ghost cells unwritten
+    if (done > 0 && (h->fused == KV_MARCH || h->fused == KV_TILED)) {
+        // the fused kernels leave the output's ghost cells unwritten
         const int rf = fill_ghosts(h);
         if (rf) return rf;
     }
@@ -626,8 +624,8 @@ int sfv_time_kernel(sfv_handle* h, const char* which, int32_t n, double* ms_out)
     }
     const int slot = (int)(h->step_parity & 1), cur = (int)(h->cur & 1);
     // The kernels run as in a step — stage 1, stage 2 (with the dt stage 1's
-    // wave speeds give), the boundary condition (its own kernel on the tiled
-    // path; inside stage 2 on the march path, where "bc" times 0) — against a
+    // wave speeds give; both read ghosts through the boundary maps, so no
+    // boundary kernel runs per step and "bc" times 0) — against a
     // scratch output without closing the step, so the state is left as is,
     // with an event between consecutive kernels; the named kernel's mean
     // duration is returned.  The first step is a warm-up.
@@ -650,7 +648,7 @@ int sfv_time_kernel(sfv_handle* h, const char* which, int32_t n, double* ms_out)
         return k == 2 ? launch_bc<float>(h->stream, h->P, h->ctl, slot, Us, BC_IN_STEP) : stage(k + 1, U0, R, Us);
     };
     const bool copy = w == "copy";
-    const int nk = copy ? 1 : h->fused == KV_MARCH ? 2 : 3;
+    const int nk = copy ? 1 : 2;
     const int runs = n + 1;
     std::vector<hipEvent_t> ev((size_t)runs * (nk + 1));
     for (auto& e : ev) HIP_TRY(h, hipEventCreate(&e));

@@ -81,6 +81,6 @@ hipError_t launch_march(hipStream_t st, const KParams& P, StepCtl* ctl, int slot
                         const S* RHS1, S* OUT, const double* B, int close = 0, int remap_halo = 0);
 template <class S>
 hipError_t launch_stage2(hipStream_t st, const KParams& P, StepCtl* ctl, int slot, const S* U0, const S* RHS1, S* U2,
-                         const double* B);
+                         const double* B, int close = 0);
 
 }  // namespace sfv

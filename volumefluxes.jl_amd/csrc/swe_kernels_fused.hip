@@ -208,13 +208,27 @@ __global__ void __launch_bounds__(F_THREADS) k_stage1(KParams P, StepCtl* ctl, i
     const TileGeom T = tile_geom(P);
     const size_t n = (size_t)P.nxT * P.nyT;
     load_B(P, T, B, s.b);
-    // state box (the stage input is a post-BC state: ghosts are read as is)
+    // state box.  A ghost cell is read through the boundary-condition maps,
+    // as its source cell's value with the normal momentum negated at a wall:
+    // the value update_bc! would have stored (the tiled kernels run only when
+    // the sources are interior cells), so the state's ghost cells in memory
+    // are never read.  The cross-shaped box holds no domain corner.
+    const int gc = P.gc, nxT = P.nxT, nyT = P.nyT;
     for (int e = threadIdx.x; e < BOX_X * BOX_Y; e += F_THREADS) {
         const int x = e % BOX_X, y = e / BOX_X;
         if (!in_cross(T, x, y)) continue;
-        const size_t k = (size_t)(T.i0 - 2 + x) + (size_t)P.nxT * (T.j0 - 2 + y);
+        int i = T.i0 - 2 + x, j = T.j0 - 2 + y;
+        int neg = -1;
+        if (i < gc) { neg = P.flip_lo_x[i] ? 1 : -1; i = P.src_lo_x[i]; }
+        else if (i >= nxT - gc) { neg = P.flip_hi_x[i - (nxT - gc)] ? 1 : -1; i = P.src_hi_x[i - (nxT - gc)]; }
+        else if (j < gc) { neg = P.flip_lo_y[j] ? 2 : -1; j = P.src_lo_y[j]; }
+        else if (j >= nyT - gc) { neg = P.flip_hi_y[j - (nyT - gc)] ? 2 : -1; j = P.src_hi_y[j - (nyT - gc)]; }
+        const size_t k = (size_t)i + (size_t)nxT * j;
 #pragma unroll
-        for (int v = 0; v < 3; ++v) s.u[v][y][x] = U0[v * n + k];
+        for (int v = 0; v < 3; ++v) {
+            const S u = U0[v * n + k];
+            s.u[v][y][x] = v == neg ? (S)(-u) : u;
+        }
     }
     __syncthreads();
     S acc[CPT][3];
@@ -263,9 +277,15 @@ __device__ __forceinline__ void finish1(const KParams& P, S x0, S x1, S x2, S r0
 template <class S>
 __global__ void __launch_bounds__(F_THREADS) k_stage2(KParams P, StepCtl* ctl, int slot, const S* __restrict__ U0,
                                                       const S* __restrict__ RHS1, S* __restrict__ U2,
-                                                      const double* __restrict__ B) {
+                                                      const double* __restrict__ B, int close) {
     const StepSlot& sl = ctl->slot[slot];
-    if (!sl.active) return;
+    const bool closer = close && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0;
+    if (!sl.active) {
+        // an inactive step passes inactivity on, so steps launched past the
+        // end of a run stay no-ops
+        if (closer) ctl->slot[1 - slot].active = 0;
+        return;
+    }
     __shared__ FusedSmem<S> s;
     const TileGeom T = tile_geom(P);
     const size_t n = (size_t)P.nxT * P.nyT;
@@ -274,6 +294,9 @@ __global__ void __launch_bounds__(F_THREADS) k_stage2(KParams P, StepCtl* ctl, i
     if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) {
         ctl->dt = (double)dt_s;
         ctl->dt_stage1 = dt_d;
+        // the step closes here (t += dt, next slot armed): the other threads
+        // read only this step's slot, the next launches the next one
+        if (close) close_step<S>(P, ctl, slot);
     }
     load_B(P, T, B, s.b);
     // U1 on the cross-shaped box.  A ghost cell takes the post-BC value: the
@@ -392,9 +415,9 @@ hipError_t launch_stage1(hipStream_t st, const KParams& P, StepCtl* ctl, int slo
 }
 template <class S>
 hipError_t launch_stage2(hipStream_t st, const KParams& P, StepCtl* ctl, int slot, const S* U0, const S* RHS1, S* U2,
-                         const double* B) {
+                         const double* B, int close) {
     const dim3 grid((P.nx + FT_X - 1) / FT_X, (P.ny + FT_Y - 1) / FT_Y);
-    hipLaunchKernelGGL(k_stage2<S>, grid, dim3(F_THREADS), 0, st, P, ctl, slot, U0, RHS1, U2, B);
+    hipLaunchKernelGGL(k_stage2<S>, grid, dim3(F_THREADS), 0, st, P, ctl, slot, U0, RHS1, U2, B, close);
     return hipPeekAtLastError();
 }
 template hipError_t launch_stage1<double>(hipStream_t, const KParams&, StepCtl*, int, const double*, double*,
@@ -402,8 +425,8 @@ template hipError_t launch_stage1<double>(hipStream_t, const KParams&, StepCtl*,
 template hipError_t launch_stage1<float>(hipStream_t, const KParams&, StepCtl*, int, const float*, float*,
                                          const double*);
 template hipError_t launch_stage2<double>(hipStream_t, const KParams&, StepCtl*, int, const double*, const double*,
-                                          double*, const double*);
+                                          double*, const double*, int);
 template hipError_t launch_stage2<float>(hipStream_t, const KParams&, StepCtl*, int, const float*, const float*,
-                                         float*, const double*);
+                                         float*, const double*, int);
 
 }  // namespace sfv
