@@ -98,6 +98,13 @@ int step_kernels(sfv_handle* h, int slot, int cur) {
     if (h->fused == KV_HYBRID) {
         // the second stage-1 launch also raises this slot's wave-speed maxima,
         // after dt was taken from them; the slot is re-armed before its next use
+        if (h->cfg.stepper == SFV_FORWARD_EULER && h->ghost_sources_interior()) {
+            // two launches: stage 1 reads ghosts through the boundary maps and
+            // the update closes the step; ghosts are filled when the loop ends
+            HIP_TRY(h, launch_march<S>(st, P, h->ctl, slot, 1, U0, nullptr, RHS, h->B, 0, 1));
+            HIP_TRY(h, launch_finish<S>(st, P, h->ctl, slot, 1, U0, U0, RHS, U2, h->B, 1, 1));
+            return SFV_OK;
+        }
         if (h->cfg.stepper == SFV_FORWARD_EULER) {
             HIP_TRY(h, launch_march<S>(st, P, h->ctl, slot, 1, U0, nullptr, RHS, h->B));
             HIP_TRY(h, launch_finish<S>(st, P, h->ctl, slot, 1, U0, U0, RHS, U2, h->B, 1));
@@ -215,7 +222,7 @@ hipError_t tune_steps(sfv_handle* h, int slot, int n, bool tiled) {
             if (e == hipSuccess) e = launch_stage2<S>(h->stream, P, h->ctl, slot, U0, R, Us, h->B);   // not closing
             continue;
         }
-        e = launch_march<S>(h->stream, P, h->ctl, slot, 1, U0, nullptr, R, h->B, 0, h->fused == KV_MARCH ? 1 : 0);
+        e = launch_march<S>(h->stream, P, h->ctl, slot, 1, U0, nullptr, R, h->B, 0, h->ghost_sources_interior() ? 1 : 0);
         if (h->fused == KV_HYBRID) continue;
         if (e == hipSuccess) e = launch_march<S>(h->stream, P, h->ctl, slot, 2, U0, R, Us, h->B);   // not closing
     }
@@ -396,7 +403,7 @@ int run_loop(sfv_handle* h, double t0, double endtime, int match_endtime, int ha
     const int64_t done = h->ctl_host->steps;
     h->cur = (h->cur + done) & 1;
     h->step_parity = (h->step_parity + done) & 1;
-    if (done > 0 && (h->fused == KV_MARCH || h->fused == KV_TILED)) {
+    if (done > 0 && h->leaves_ghosts()) {
         // the fused kernels leave the output's ghost cells unwritten
         const int rf = fill_ghosts(h);
         if (rf) return rf;

@@ -192,10 +192,11 @@ struct sfv_handle {
     // the interior right-hand side; it needs gc >= 2 and nothing else.
     bool hybrid_ok() const { return P.gc >= 2; }
 
-    // The fused stage kernels rebuild stage-1 ghost cells from their
-    // boundary-condition sources, which must then be interior cells.
-    bool fusable() const {
-        if (cfg.stepper != SFV_RUNGE_KUTTA2 || P.gc < 2) return false;
+    // Whether every ghost cell's boundary-condition source is an interior
+    // cell (WallBC, PeriodicBC with nx, ny >= gc): kernels may then read a
+    // ghost through the maps instead of from memory.
+    bool ghost_sources_interior() const {
+        if (P.gc < 2) return false;
         for (int g = 0; g < P.gc; ++g) {
             const int sx[2] = {P.src_lo_x[g], P.src_hi_x[g]}, sy[2] = {P.src_lo_y[g], P.src_hi_y[g]};
             for (int k = 0; k < 2; ++k) {
@@ -205,6 +206,18 @@ struct sfv_handle {
         }
         return true;
     }
+
+    // Does a step leave the output's ghost cells unwritten (they are then
+    // filled where the state leaves the stepping loop)?  The march and tiled
+    // RK2 paths and the two-launch ForwardEuler path do.
+    bool leaves_ghosts() const {
+        return fused == KV_MARCH || fused == KV_TILED ||
+               (fused == KV_HYBRID && cfg.stepper == SFV_FORWARD_EULER && ghost_sources_interior());
+    }
+
+    // The fused stage kernels rebuild stage-1 ghost cells from their
+    // boundary-condition sources, which must then be interior cells.
+    bool fusable() const { return cfg.stepper == SFV_RUNGE_KUTTA2 && ghost_sources_interior(); }
 
     ~sfv_handle() {
         if (stream) (void)hipStreamSynchronize(stream);

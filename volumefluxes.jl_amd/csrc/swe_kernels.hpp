@@ -39,7 +39,7 @@ hipError_t launch_flux(hipStream_t st, const KParams& P, StepCtl* ctl, int slot,
 template <class S>
 hipError_t launch_finish(hipStream_t st, const KParams& P, StepCtl* ctl, int slot, int stage, const S* U0, const S* X,
                          const S* RHS, S* Out, const double* B,
-                         int rhs_interior = 0);
+                         int rhs_interior = 0, int close = 0);
 template <class S>
 hipError_t launch_bc(hipStream_t st, const KParams& P, StepCtl* ctl, int slot, S* U, int mode);
 hipError_t launch_prepare(hipStream_t st, const KParams& P, StepCtl* ctl, int slot, double t, double endtime,

@@ -122,9 +122,14 @@ __global__ void __launch_bounds__(256) k_flux(KParams P, StepCtl* ctl, int slot,
 template <class S>
 __global__ void __launch_bounds__(256) k_finish(KParams P, StepCtl* ctl, int slot, int stage, const S* __restrict__ U0,
                                                 const S* __restrict__ X, const S* __restrict__ RHS,
-                                                S* __restrict__ Out, const double* __restrict__ B, int rhs_interior) {
+                                                S* __restrict__ Out, const double* __restrict__ B, int rhs_interior,
+                                                int close) {
     const StepSlot& sl = ctl->slot[slot];
-    if (!sl.active) return;
+    const bool closer = close && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0 && threadIdx.y == 0;
+    if (!sl.active) {
+        if (closer) ctl->slot[1 - slot].active = 0;   // steps past the end of a run stay no-ops
+        return;
+    }
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y * blockDim.y + threadIdx.y;
     double dt_d;
@@ -135,6 +140,9 @@ __global__ void __launch_bounds__(256) k_finish(KParams P, StepCtl* ctl, int slo
         if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0 && threadIdx.y == 0) {
             ctl->dt = (double)dt_s;
             ctl->dt_stage1 = dt_d;
+            // a one-stage step may close here (t += dt, next slot armed): the
+            // other threads read only this step's slot
+            if (close) close_step<S>(P, ctl, slot);
         }
     } else {
         dt_s = (S)ctl->dt;
@@ -189,10 +197,11 @@ hipError_t launch_flux(hipStream_t st, const KParams& P, StepCtl* ctl, int slot,
 }
 template <class S>
 hipError_t launch_finish(hipStream_t st, const KParams& P, StepCtl* ctl, int slot, int stage, const S* U0, const S* X,
-                         const S* RHS, S* Out, const double* B, int rhs_interior) {
+                         const S* RHS, S* Out, const double* B, int rhs_interior, int close) {
     const dim3 block(64, 4);
     const dim3 grid((P.nxT + 63) / 64, (P.nyT + 3) / 4);
-    hipLaunchKernelGGL(k_finish<S>, grid, block, 0, st, P, ctl, slot, stage, U0, X, RHS, Out, B, rhs_interior);
+    hipLaunchKernelGGL(k_finish<S>, grid, block, 0, st, P, ctl, slot, stage, U0, X, RHS, Out, B, rhs_interior,
+                       close);
     return hipPeekAtLastError();
 }
 template hipError_t launch_flux<double>(hipStream_t, const KParams&, StepCtl*, int, const double*, double*,
@@ -200,15 +209,15 @@ template hipError_t launch_flux<double>(hipStream_t, const KParams&, StepCtl*, i
 template hipError_t launch_flux<float>(hipStream_t, const KParams&, StepCtl*, int, const float*, float*,
                                        const double*, int);
 template hipError_t launch_finish<double>(hipStream_t, const KParams&, StepCtl*, int, int, const double*,
-                                          const double*, const double*, double*, const double*, int);
+                                          const double*, const double*, double*, const double*, int, int);
 template hipError_t launch_finish<float>(hipStream_t, const KParams&, StepCtl*, int, int, const float*, const float*,
-                                         const float*, float*, const double*, int);
+                                         const float*, float*, const double*, int, int);
 
 template __global__ void k_flux<double>(KParams, StepCtl*, int, const double*, double*, const double*, int);
 template __global__ void k_flux<float>(KParams, StepCtl*, int, const float*, float*, const double*, int);
 template __global__ void k_finish<double>(KParams, StepCtl*, int, int, const double*, const double*, const double*,
-                                          double*, const double*, int);
+                                          double*, const double*, int, int);
 template __global__ void k_finish<float>(KParams, StepCtl*, int, int, const float*, const float*, const float*,
-                                         float*, const double*, int);
+                                         float*, const double*, int, int);
 
 }  // namespace sfv
