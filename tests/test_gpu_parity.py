@@ -188,3 +188,43 @@ def test_tuner_may_pick_tiled_kernels_and_results_stay_bitwise():
     VF.perform_step(forced, float("inf"))
     assert forced.kernels == "march"
     assert np.array_equal(auto.get_full_state(), forced.get_full_state())
+
+
+def test_switching_kernel_variants_mid_run_keeps_oracle_bits():
+    """The march and tiled paths read ghost cells through the boundary maps
+    and fill them only when a host call ends; the per-pass and hybrid paths
+    read them from memory.  Switching between them between calls must give
+    the oracle's bits step for step."""
+    import ctypes
+    from volumefluxes import _capi as C
+    sc = terrain(130, 70, seed=61)
+    sim = make_hip(sc)
+    o = make_oracle(sc)
+    t = 0.0
+    for variant in ("march", "per-pass", "tiled", "hybrid", "march", "tiled", "per-pass"):
+        sim.set_kernels(variant)
+        for _ in range(4):
+            dr = o.step(float("inf"), t)
+            dh = VF.perform_step(sim, float("inf"))
+            assert dh == dr, variant
+            t += dr
+            sim.t = t
+        assert np.array_equal(sim.get_full_state(), o.state()), variant
+    # a device-resident run, then a timing call, then the per-pass path again
+    sim.set_kernels("march")
+    VF.advance_steps(sim, 6)
+    for _ in range(6):
+        t += o.step(float("inf"), t)
+    assert np.array_equal(sim.get_full_state(), o.state())
+    before = sim.get_full_state()
+    out = ctypes.c_double()
+    C.check(sim.handle, C.lib().sfv_time_kernel(sim.handle, b"step", 3, ctypes.byref(out)))
+    assert out.value > 0.0
+    assert np.array_equal(sim.get_full_state(), before)       # timing leaves the state alone
+    sim.set_kernels("per-pass")
+    for _ in range(3):
+        dr = o.step(float("inf"), t)
+        dh = VF.perform_step(sim, float("inf"))
+        assert dh == dr
+        t += dr
+    assert np.array_equal(sim.get_full_state(), o.state())
