@@ -228,3 +228,27 @@ def test_switching_kernel_variants_mid_run_keeps_oracle_bits():
         assert dh == dr
         t += dr
     assert np.array_equal(sim.get_full_state(), o.state())
+
+
+def _random_case(k):
+    rng = np.random.default_rng(9000 + k)
+    gc = int(rng.choice([2, 3]))
+    nx, ny = int(rng.integers(max(gc, 3), 140)), int(rng.integers(max(gc, 3), 90))
+    bc = [VF.WallBC(), VF.PeriodicBC(), VF.NeumannBC()][int(rng.integers(0, 3))]
+    stepper = VF.RungeKutta2() if rng.random() < 0.6 else VF.ForwardEulerStepper()
+    rec = VF.NoReconstruction() if rng.random() < 0.2 else None
+    sc = terrain(nx, ny, seed=9100 + k, gc=gc, bc=bc, stepper=stepper, rec=rec,
+                 friction=bool(rng.random() < 0.7), rain=0.036 if rng.random() < 0.6 else 0.0,
+                 n_bumps=int(rng.integers(1, 6)))
+    kernels = None if rng.random() < 0.75 else "per-pass"
+    return sc, kernels
+
+
+@pytest.mark.parametrize("k", range(24))
+def test_random_configurations_bitwise(k):
+    """Seeded random sizes, ghost widths, boundary conditions, steppers,
+    reconstructions and sources, on the default kernel path (march, tiled or
+    hybrid as the handle picks) or the per-pass one: 8 steps, bitwise
+    against the oracle, ghost cells included."""
+    sc, kernels = _random_case(k)
+    _assert_bitwise(sc, 8, kernels=kernels)
