@@ -114,7 +114,7 @@ __device__ __forceinline__ int jmin_pick(double x, double y) {
 // One side of a face: its flux f and eigenvalues e (swe_2D.jl:43-81) and
 // their Jacobians with respect to the side state (h, hu, hv); all zero for
 // a dry side (centralupwind.jl:51-61 leaves them at zero).
-__device__ void side_jacobian(int d, const double s[3], const PhysParams& p, double f[3], double e[2],
+__device__ __forceinline__ void side_jacobian(int d, const double s[3], const PhysParams& p, double f[3], double e[2],
                               double Jf[3][3], double Je[2][3]) {
     for (int v = 0; v < 3; ++v) {
         f[v] = 0.0;
@@ -144,7 +144,7 @@ __device__ void side_jacobian(int d, const double s[3], const PhysParams& p, dou
 // plus side q.  The values and every branch are those of central_upwind on
 // doubles; each side's flux and eigenvalue Jacobians come from a Dual<3>
 // evaluation, and the combination of the two sides is differentiated by hand.
-__device__ void face_vjp(int d, const double m[3], const double q[3], const PhysParams& p, const double lF[3],
+__device__ __forceinline__ void face_vjp(int d, const double m[3], const double q[3], const PhysParams& p, const double lF[3],
                          double lm[3], double lq[3]) {
     double fm[3], em[2], Jfm[3][3], Jem[2][3];
     double fp[3], ep[2], Jfp[3][3], Jep[2][3];
@@ -181,8 +181,15 @@ __device__ void face_vjp(int d, const double m[3], const double q[3], const Phys
     l_ap += l_D;
     l_am -= l_D;
     double l_ep[2] = {0.0, 0.0}, l_em[2] = {0.0, 0.0};
-    if (!pk2) (pk1 ? l_em[0] : l_ep[0]) += l_ap;
-    if (!mk2) (mk1 ? l_em[1] : l_ep[1]) += l_am;
+    // explicit branches (a conditional lvalue would put the arrays in memory)
+    if (!pk2) {
+        if (pk1) l_em[0] += l_ap;
+        else l_ep[0] += l_ap;
+    }
+    if (!mk2) {
+        if (mk1) l_em[1] += l_am;
+        else l_ep[1] += l_am;
+    }
     for (int a = 0; a < 3; ++a) {
         double gm = 0.0, gq = 0.0;
         for (int v = 0; v < 3; ++v) { gm += l_fm[v] * Jfm[v][a]; gq += l_fp[v] * Jfp[v][a]; }
@@ -338,18 +345,21 @@ __device__ void block_sum(double v, double* part, int slot) {
 
 // Adjoint of: pre = in + dt*R (stage 2: pre = 0.5*(U0 + in + dt*R)),
 // friction (scalar from `in`), dry clamp.  Lout: adjoint of the stage
-// output (ghosts already folded).  Adds to Lin (and, stage 2, to LU0),
-// writes LR = adjoint of R, adds the B_cell adjoint, and block partial sums
+// output (ghosts already folded).  Stage 2 writes Lin and LU0 for every
+// cell and resets the step's argmax search (*best); stage 1 adds to Lin.
+// Writes LR = adjoint of R, adds the B_cell adjoint, and block partial sums
 // of the dt and Cz adjoints.
 __global__ void __launch_bounds__(256) k_adj_tail(KParams P, const StepCtl* ctl, int stage, const double* __restrict__ U0,
                                                   const double* __restrict__ Uin, const double* __restrict__ R,
                                                   const double* __restrict__ B, const double* __restrict__ Lout,
                                                   double* __restrict__ Lin, double* __restrict__ LU0,
                                                   double* __restrict__ LR, double* __restrict__ LBc,
-                                                  double* __restrict__ part_dt, double* __restrict__ part_cz) {
+                                                  double* __restrict__ part_dt, double* __restrict__ part_cz,
+                                                  unsigned long long* __restrict__ best) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y * blockDim.y + threadIdx.y;
     const int nblk = gridDim.x * gridDim.y, bid = blockIdx.x + gridDim.x * blockIdx.y;
+    if (stage == 2 && bid == 0 && threadIdx.x == 0 && threadIdx.y == 0) *best = ~0ull;
     double gdt = 0.0, gcz = 0.0;
     if (i < P.nxT && j < P.nyT) {
         const size_t n = (size_t)P.nxT * P.nyT;
@@ -382,6 +392,10 @@ __global__ void __launch_bounds__(256) k_adj_tail(KParams P, const StepCtl* ctl,
         }
         // friction: o1 = pre1 / D, o2 = pre2 / D, D = 1 - dt*fs(x, Bm, Cz)
         double lpre[3] = {lo[0], lo[1] / D, lo[2] / D};
+        // stage 2 is the first writer of Lin and LU0 in a reverse step: it
+        // stores 0.0 + its terms, stage 1 adds to what is there
+        double lin[3];
+        for (int v = 0; v < 3; ++v) lin[v] = stage == 2 ? 0.0 : Lin[v * n + k];
         if (P.friction) {
             const double lD = -(lo[1] * pre[1] + lo[2] * pre[2]) / (D * D);
             const double lfs = lD * (-dt);
@@ -389,7 +403,7 @@ __global__ void __launch_bounds__(256) k_adj_tail(KParams P, const StepCtl* ctl,
             using D5 = Dual<5>;
             const D5 fsd = ad::friction_scalar<D5>(P.friction, D5::var(P.Cz, 4), P.phys.kappa, D5::var(x[0], 0),
                                                D5::var(x[1], 1), D5::var(x[2], 2), D5::var(Bm, 3));
-            for (int v = 0; v < 3; ++v) Lin[v * n + k] += lfs * fsd.d[v];
+            for (int v = 0; v < 3; ++v) lin[v] += lfs * fsd.d[v];
             lBc += lfs * fsd.d[3];
             gcz += lfs * fsd.d[4];
         }
@@ -397,8 +411,8 @@ __global__ void __launch_bounds__(256) k_adj_tail(KParams P, const StepCtl* ctl,
         double lax[3];
         for (int v = 0; v < 3; ++v) {
             lax[v] = stage == 2 ? 0.5 * lpre[v] : lpre[v];
-            if (stage == 2) LU0[v * n + k] += 0.5 * lpre[v];
-            Lin[v * n + k] += lax[v];
+            if (stage == 2) LU0[v * n + k] = 0.0 + 0.5 * lpre[v];
+            Lin[v * n + k] = lin[v] + lax[v];
             LR[v * n + k] = dt * lax[v];
             gdt += lax[v] * r[v];
         }
@@ -547,21 +561,42 @@ __global__ void __launch_bounds__(256) k_adj_argmax(KParams P, const StepCtl* ct
         atomicMin(best, (unsigned long long)cid(P, i, j));
 }
 
-// A step whose dt came from a finite positive wave speed must have a face
-// that reproduces that speed; otherwise the sticky flag is raised.
-__global__ void k_adj_check(const StepCtl* ctl, const unsigned long long* __restrict__ best, int* __restrict__ flag) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    const double ax = __longlong_as_double((long long)ctl->slot[0].wsmax[0]);
-    const double ay = __longlong_as_double((long long)ctl->slot[0].wsmax[1]);
-    const bool armed = __builtin_isfinite(ax) && __builtin_isfinite(ay) && (ax > 0.0 || ay > 0.0);
-    if (armed && *best == ~0ull) *flag = 1;
-}
-
-__global__ void k_adj_dtface(KParams P, const StepCtl* ctl, int d, const double* __restrict__ U,
-                             const double* __restrict__ B, const unsigned long long* __restrict__ best,
-                             const double* __restrict__ sums, double* __restrict__ LfL, double* __restrict__ LfR) {
-    if (threadIdx.x != 0 || blockIdx.x != 0 || d != dt_direction(P, ctl)) return;
+// Once per reverse step, after the stage tails, the argmax search and both
+// directions' face adjoints of stage 1: (1) sum the two stages' block
+// partials of the dt and Cz adjoints in a fixed order (sums[0] = the step's
+// dt adjoint, cz_total += its Cz adjoint); (2) a step whose dt came from a
+// finite positive wave speed must have a face reproducing that speed, else
+// the sticky flag is raised; (3) the dt adjoint goes to the face that set dt
+// (in the face buffers of its direction, d = 0: LfL0/LfR0, d = 1: LfL1/LfR1).
+__global__ void __launch_bounds__(256) k_adj_dtstep(KParams P, const StepCtl* ctl, const double* __restrict__ part_dt,
+                                                    const double* __restrict__ part_cz, int count,
+                                                    double* __restrict__ sums, double* __restrict__ cz_total,
+                                                    const unsigned long long* __restrict__ best, int* __restrict__ flag,
+                                                    const double* __restrict__ U, const double* __restrict__ B,
+                                                    double* __restrict__ LfL0, double* __restrict__ LfR0,
+                                                    double* __restrict__ LfL1, double* __restrict__ LfR1) {
+    __shared__ double red[2][256];
+    double a = 0.0, b = 0.0;
+    for (int k = threadIdx.x; k < count; k += blockDim.x) { a += part_dt[k]; b += part_cz[k]; }
+    red[0][threadIdx.x] = a;
+    red[1][threadIdx.x] = b;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) { red[0][threadIdx.x] += red[0][threadIdx.x + s]; red[1][threadIdx.x] += red[1][threadIdx.x + s]; }
+        __syncthreads();
+    }
+    if (threadIdx.x != 0) return;
+    const double ldt = red[0][0];
+    sums[0] = ldt;
+    *cz_total += red[1][0];
     const unsigned long long kbest = *best;
+    {
+        const double ax = __longlong_as_double((long long)ctl->slot[0].wsmax[0]);
+        const double ay = __longlong_as_double((long long)ctl->slot[0].wsmax[1]);
+        const bool armed = __builtin_isfinite(ax) && __builtin_isfinite(ay) && (ax > 0.0 || ay > 0.0);
+        if (armed && kbest == ~0ull) *flag = 1;
+    }
+    const int d = dt_direction(P, ctl);
     if (kbest >= (unsigned long long)P.nxT * P.nyT) return;
     const int i = (int)(kbest % P.nxT), j = (int)(kbest / P.nxT);
     const double ax = __longlong_as_double((long long)ctl->slot[0].wsmax[d]);
@@ -574,7 +609,7 @@ __global__ void k_adj_dtface(KParams P, const StepCtl* ctl, int d, const double*
     const int ib = ia + di, jb = ja + dj;
     const double Dd = d == 0 ? P.dx : P.dy;
     // dt = cfl * Δ_d / a_d on the selected direction: d dt / d a = -cfl Δ / a^2
-    const double lspeed = sums[0] * P.cfl * (-Dd / (ax * ax));
+    const double lspeed = ldt * P.cfl * (-Dd / (ax * ax));
     const size_t n = (size_t)P.nxT * P.nyT;
     double La[3], Ra[3], Lb[3], Rb[3];
     recon_at<double>(P, U, B, d, ia, ja, -2, false, La, Ra);
@@ -583,30 +618,11 @@ __global__ void k_adj_dtface(KParams P, const StepCtl* ctl, int d, const double*
     D6 m[3], q[3], F[3];
     for (int v = 0; v < 3; ++v) { m[v] = D6::var(Ra[v], v); q[v] = D6::var(Lb[v], 3 + v); }
     const D6 sp = ad::central_upwind<D6>(d, m, q, P.phys, F);
-    for (int a = 0; a < 3; ++a) {
-        LfR[a * n + cid(P, ia, ja)] += lspeed * sp.d[a];
-        LfL[a * n + cid(P, ib, jb)] += lspeed * sp.d[3 + a];
-    }
-}
-
-// Sum the two stages' block partials (dt) and the Cz partials, in order.
-__global__ void k_adj_sum(const double* __restrict__ part_dt, const double* __restrict__ part_cz, int count,
-                          double* __restrict__ sums, double* __restrict__ cz_total,
-                          unsigned long long* __restrict__ best) {
-    if (threadIdx.x == 0) *best = ~0ull;     // argmax search of this step starts afresh
-    __shared__ double red[2][256];
-    double a = 0.0, b = 0.0;
-    for (int k = threadIdx.x; k < count; k += blockDim.x) { a += part_dt[k]; b += part_cz[k]; }
-    red[0][threadIdx.x] = a;
-    red[1][threadIdx.x] = b;
-    __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-        if ((int)threadIdx.x < s) { red[0][threadIdx.x] += red[0][threadIdx.x + s]; red[1][threadIdx.x] += red[1][threadIdx.x + s]; }
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) {
-        sums[0] = red[0][0];
-        *cz_total += red[1][0];
+    double* LfL = d == 0 ? LfL0 : LfL1;
+    double* LfR = d == 0 ? LfR0 : LfR1;
+    for (int c = 0; c < 3; ++c) {
+        LfR[c * n + cid(P, ia, ja)] += lspeed * sp.d[c];
+        LfL[c * n + cid(P, ib, jb)] += lspeed * sp.d[3 + c];
     }
 }
 
@@ -750,8 +766,10 @@ extern "C" int sfv_adjoint(sfv_handle* h, int64_t nsteps, int32_t checkpoint_eve
     double* Lnew = buf.alloc(n3, why, rc);                      // adjoint of its input
     double* L1 = buf.alloc(n3, why, rc);                        // adjoint of U1
     double* LR = buf.alloc(n3, why, rc);
-    double* LfL = buf.alloc(n3, why, rc);
+    double* LfL = buf.alloc(n3, why, rc);                       // face-state adjoints (x, and y of stage 2)
     double* LfR = buf.alloc(n3, why, rc);
+    double* LfL1 = buf.alloc(n3, why, rc);                      // stage 1's y face-state adjoints
+    double* LfR1 = buf.alloc(n3, why, rc);
     double* LBc = buf.alloc(n, why, rc);
     double* LBf = buf.alloc(4 * n, why, rc);
     double* LB = buf.alloc(nB, why, rc);
@@ -850,11 +868,10 @@ extern "C" int sfv_adjoint(sfv_handle* h, int64_t nsteps, int32_t checkpoint_eve
             // fold the output's ghosts (its boundary condition)
             hipLaunchKernelGGL(ad::k_adj_bc, dim3(bcx), dim3(256), 0, st, P, L, 0);
             hipLaunchKernelGGL(ad::k_adj_bc, dim3(bcy), dim3(256), 0, st, P, L, 1);
-            ATRY(hipMemsetAsync(Lnew, 0, n3 * sizeof(double), st));
-            ATRY(hipMemsetAsync(L1, 0, n3 * sizeof(double), st));
-            // stage 2: U2 = clamp(friction(0.5 (U0 + U1 + dt R2)))
+            // stage 2: U2 = clamp(friction(0.5 (U0 + U1 + dt R2))); the tail
+            // writes L1 and Lnew afresh
             hipLaunchKernelGGL(ad::k_adj_tail, cg, cb, 0, st, P, h->ctl, 2, U0, U1, R2, h->B, L, L1, Lnew, LR, LBc,
-                               part_dt, part_cz);
+                               part_dt, part_cz, best);
             for (int d = 0; d < 2; ++d) {
                 hipLaunchKernelGGL(ad::k_adj_faces, faces_grid(d), dim3(256), 0, st, P, d, U1, h->B, LR, LfL, LfR);
                 hipLaunchKernelGGL(ad::k_adj_recon, cg, cb, 0, st, P, d, U1, h->B, LR, LfL, LfR, L1, LBf);
@@ -863,19 +880,16 @@ extern "C" int sfv_adjoint(sfv_handle* h, int64_t nsteps, int32_t checkpoint_eve
             hipLaunchKernelGGL(ad::k_adj_bc, dim3(bcy), dim3(256), 0, st, P, L1, 1);
             // stage 1: U1 = clamp(friction(U0 + dt R1))
             hipLaunchKernelGGL(ad::k_adj_tail, cg, cb, 0, st, P, h->ctl, 1, U0, U0, R1, h->B, L1, Lnew, nullptr, LR,
-                               LBc, part_dt, part_cz);
-            hipLaunchKernelGGL(ad::k_adj_sum, dim3(1), dim3(256), 0, st, part_dt, part_cz, 2 * nblk, sums, sums + 1,
-                               best);
+                               LBc, part_dt, part_cz, best);
             // the face that set dt, found as the reference's maximum(ws) would
             hipLaunchKernelGGL(ad::k_adj_argmax, dim3((P.nx * P.ny + 255) / 256), dim3(256), 0, st, P, h->ctl, U0,
                                h->B, best);
-            hipLaunchKernelGGL(ad::k_adj_check, dim3(1), dim3(64), 0, st, h->ctl, best, flag);
-            for (int d = 0; d < 2; ++d) {
-                hipLaunchKernelGGL(ad::k_adj_faces, faces_grid(d), dim3(256), 0, st, P, d, U0, h->B, LR, LfL, LfR);
-                hipLaunchKernelGGL(ad::k_adj_dtface, dim3(1), dim3(64), 0, st, P, h->ctl, d, U0, h->B, best, sums,
-                                   LfL, LfR);
-                hipLaunchKernelGGL(ad::k_adj_recon, cg, cb, 0, st, P, d, U0, h->B, LR, LfL, LfR, Lnew, LBf);
-            }
+            hipLaunchKernelGGL(ad::k_adj_faces, faces_grid(0), dim3(256), 0, st, P, 0, U0, h->B, LR, LfL, LfR);
+            hipLaunchKernelGGL(ad::k_adj_faces, faces_grid(1), dim3(256), 0, st, P, 1, U0, h->B, LR, LfL1, LfR1);
+            hipLaunchKernelGGL(ad::k_adj_dtstep, dim3(1), dim3(256), 0, st, P, h->ctl, part_dt, part_cz, 2 * nblk,
+                               sums, sums + 1, best, flag, U0, h->B, LfL, LfR, LfL1, LfR1);
+            hipLaunchKernelGGL(ad::k_adj_recon, cg, cb, 0, st, P, 0, U0, h->B, LR, LfL, LfR, Lnew, LBf);
+            hipLaunchKernelGGL(ad::k_adj_recon, cg, cb, 0, st, P, 1, U0, h->B, LR, LfL1, LfR1, Lnew, LBf);
             ATRY(hipPeekAtLastError());
             std::swap(L, Lnew);
         }

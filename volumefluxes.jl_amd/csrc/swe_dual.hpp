@@ -111,31 +111,37 @@ __host__ __device__ inline Dual<N> dcbrt(const Dual<N>& a) {
 }
 __host__ __device__ inline double dcbrt(double a) { return cbrt(a); }
 
-template <int N>
-__host__ __device__ inline Dual<N> dabs(const Dual<N>& a) { return __builtin_signbit(a.v) ? -a : a; }
 __host__ __device__ inline double dabs(double a) { return __builtin_fabs(a); }
 
 __host__ __device__ inline bool is_nan(double a) { return a != a; }
 template <int N> __host__ __device__ inline bool is_nan(const Dual<N>& a) { return a.v != a.v; }
 
+// y if py, else x, component by component (a select of whole operands
+// through references would be lowered to a pointer select and put both
+// operands in scratch memory).
+__host__ __device__ inline double dpick(bool py, double x, double y) { return py ? y : x; }
+template <int N>
+__host__ __device__ inline Dual<N> dpick(bool py, const Dual<N>& x, const Dual<N>& y) {
+    Dual<N> r(py ? y.v : x.v);
+    for (int k = 0; k < N; ++k) r.d[k] = py ? y.d[k] : x.d[k];
+    return r;
+}
+
+template <int N>
+__host__ __device__ inline Dual<N> dabs(const Dual<N>& a) { return dpick(__builtin_signbit(a.v), a, -a); }
+
 // Julia max/min (NaN-propagating, -0.0 < +0.0) choosing one operand whole.
 template <class T>
 __host__ __device__ inline T djl_max(const T& x, const T& y) {
     const double a = val(x), b = val(y);
-    if (a != a) return x;
-    if (b != b) return y;
-    if (a > b) return x;
-    if (b > a) return y;
-    return __builtin_signbit(a) ? y : x;
+    const bool py = !(a != a) && ((b != b) || b > a || (!(a > b) && __builtin_signbit(a)));
+    return dpick(py, x, y);
 }
 template <class T>
 __host__ __device__ inline T djl_min(const T& x, const T& y) {
     const double a = val(x), b = val(y);
-    if (a != a) return x;
-    if (b != b) return y;
-    if (a < b) return x;
-    if (b < a) return y;
-    return __builtin_signbit(a) ? x : y;
+    const bool py = !(a != a) && ((b != b) || b < a || (!(a < b) && !__builtin_signbit(a)));
+    return dpick(py, x, y);
 }
 
 }  // namespace sfv
