@@ -111,45 +111,66 @@ __device__ __forceinline__ int jmin_pick(double x, double y) {
     return __builtin_signbit(x) ? 0 : 1;
 }
 
-// One side of a face: its flux f and eigenvalues e (swe_2D.jl:43-81) and
-// their Jacobians with respect to the side state (h, hu, hv); all zero for
-// a dry side (centralupwind.jl:51-61 leaves them at zero).
-__device__ __forceinline__ void side_jacobian(int d, const double s[3], const PhysParams& p, double f[3], double e[2],
-                              double Jf[3][3], double Je[2][3]) {
-    for (int v = 0; v < 3; ++v) {
-        f[v] = 0.0;
-        for (int a = 0; a < 3; ++a) Jf[v][a] = 0.0;
-    }
-    for (int k = 0; k < 2; ++k) {
-        e[k] = 0.0;
-        for (int a = 0; a < 3; ++a) Je[k][a] = 0.0;
-    }
-    if (!(s[0] > p.cutoff)) return;
-    using D3 = Dual<3>;
-    const D3 h = D3::var(s[0], 0), hu = D3::var(s[1], 1), hv = D3::var(s[2], 2);
-    D3 F[3], E[2];
-    ad::phys(d, h, hu, hv, p, F, E);
-    for (int v = 0; v < 3; ++v) {
-        f[v] = F[v].v;
-        for (int a = 0; a < 3; ++a) Jf[v][a] = F[v].d[a];
-    }
-    for (int k = 0; k < 2; ++k) {
-        e[k] = E[k].v;
-        for (int a = 0; a < 3; ++a) Je[k][a] = E[k].d[a];
-    }
+// d desingularize(h)/dh along the branch the value took (equation.jl:40-67;
+// the operations and picks of ad::desingularize).
+__device__ __forceinline__ double desing_dh(double h, double kappa) {
+    if (!(__builtin_fabs(h) < kappa)) return 1.0;
+    const double inner = h * h / (2 * kappa) + kappa / 2.0;
+    const int mn_k = jmin_pick(inner, kappa);                  // 1: min returned kappa
+    const double mn = mn_k ? kappa : inner;
+    const double dmn = mn_k ? 0.0 : (h + h) * (1.0 / (2 * kappa));
+    const double sg = __builtin_signbit(h) ? -1.0 : 1.0;
+    const int mx_mn = jmax_pick(__builtin_fabs(h), mn);      // 1: max returned the min
+    return sg * (mx_mn ? dmn : sg);
+}
+
+// One side of a face: its flux f and eigenvalues e (swe_2D.jl:43-81), zero
+// for a dry side (centralupwind.jl:51-61 leaves them at zero).
+__device__ __forceinline__ bool side_values(int d, const double s[3], const PhysParams& p, double f[3], double e[2]) {
+    for (int v = 0; v < 3; ++v) f[v] = 0.0;
+    e[0] = e[1] = 0.0;
+    if (!(s[0] > p.cutoff)) return false;
+    ad::phys<double>(d, s[0], s[1], s[2], p, f, e);
+    return true;
+}
+
+// Reverse sweep of one wet side: given the adjoints lf of its flux and le of
+// its eigenvalues, the adjoint of its state (h, hu, hv).  With the normal
+// momentum n = 1 + d and the tangential t = 2 - d, the side computes
+// un = s_n / hs, ut = s_t / hs, hs = desingularize(h), c = sqrt(g h),
+// f = [rho h un, rho h un^2 + 0.5 rho g h^2 (at n), rho h un ut (at t)],
+// e = [un + c, un - c].
+__device__ __forceinline__ void side_vjp(int d, const double s[3], const PhysParams& p, const double lf[3],
+                                         const double le[2], double ls[3]) {
+    const double h = s[0];
+    const double sn = d == 0 ? s[1] : s[2], st = d == 0 ? s[2] : s[1];
+    const double ln = d == 0 ? lf[1] : lf[2], lt = d == 0 ? lf[2] : lf[1];
+    const double hs = ad::desingularize<double>(h, p.kappa);
+    const double un = sn / hs, ut = st / hs;
+    const double c = sqrt(p.g * h);
+    const double rh = p.rho * h;
+    const double l_un = lf[0] * rh + ln * (2.0 * rh * un) + lt * (rh * ut) + le[0] + le[1];
+    const double l_ut = lt * (rh * un);
+    double l_h = lf[0] * (p.rho * un) + ln * (p.rho * un * un + p.rho * p.g * h) + lt * (p.rho * un * ut);
+    if (c > 0.0) l_h += (le[0] - le[1]) * (p.g * (0.5 / c));
+    const double ihs = 1.0 / hs;
+    const double l_hs = -(l_un * un + l_ut * ut) * ihs;
+    ls[0] = l_h + l_hs * desing_dh(h, p.kappa);
+    const double l_sn = l_un * ihs, l_st = l_ut * ihs;
+    ls[1] = d == 0 ? l_sn : l_st;
+    ls[2] = d == 0 ? l_st : l_sn;
 }
 
 // Reverse sweep of the central-upwind flux (centralupwind.jl:48-77) at one
 // face: given the flux adjoint lF, the adjoints of the minus side m and the
 // plus side q.  The values and every branch are those of central_upwind on
-// doubles; each side's flux and eigenvalue Jacobians come from a Dual<3>
-// evaluation, and the combination of the two sides is differentiated by hand.
+// doubles; the combination of the two sides and each side's flux are
+// differentiated by hand (side_vjp).
 __device__ __forceinline__ void face_vjp(int d, const double m[3], const double q[3], const PhysParams& p, const double lF[3],
                          double lm[3], double lq[3]) {
-    double fm[3], em[2], Jfm[3][3], Jem[2][3];
-    double fp[3], ep[2], Jfp[3][3], Jep[2][3];
-    side_jacobian(d, m, p, fm, em, Jfm, Jem);
-    side_jacobian(d, q, p, fp, ep, Jfp, Jep);
+    double fm[3], em[2], fp[3], ep[2];
+    const bool wm = side_values(d, m, p, fm, em);
+    const bool wq = side_values(d, q, p, fp, ep);
     for (int v = 0; v < 3; ++v) lm[v] = lq[v] = 0.0;
     const int pk1 = jmax_pick(ep[0], em[0]);
     const double ap1 = pk1 ? em[0] : ep[0];
@@ -190,12 +211,12 @@ __device__ __forceinline__ void face_vjp(int d, const double m[3], const double 
         if (mk1) l_em[1] += l_am;
         else l_ep[1] += l_am;
     }
+    double gm[3] = {0.0, 0.0, 0.0}, gq[3] = {0.0, 0.0, 0.0};
+    if (wm) side_vjp(d, m, p, l_fm, l_em, gm);
+    if (wq) side_vjp(d, q, p, l_fp, l_ep, gq);
     for (int a = 0; a < 3; ++a) {
-        double gm = 0.0, gq = 0.0;
-        for (int v = 0; v < 3; ++v) { gm += l_fm[v] * Jfm[v][a]; gq += l_fp[v] * Jfp[v][a]; }
-        for (int k = 0; k < 2; ++k) { gm += l_em[k] * Jem[k][a]; gq += l_ep[k] * Jep[k][a]; }
-        lm[a] += gm;
-        lq[a] += gq;
+        lm[a] += gm[a];
+        lq[a] += gq[a];
     }
 }
 
@@ -273,6 +294,65 @@ __device__ void recon_at(const KParams& P, const double* U, const double* B, int
         if (seedB) { Bl.d[0] = 1.0; Br.d[1] = 1.0; }
     }
     ad::reconstruct(cl, cc, cr, P.theta, Bl, Br, L, R);
+}
+
+// Face states of cell (i,j) along d, as recon_at<double> gives them, and the
+// branch each slope took (linearreconstruction.jl:28-43, 81-91): pick[v] is
+// 0 for theta*fwd, 1 for (fwd+bwd)/2, 2 for theta*bwd, 3 for minmod's zero;
+// dry is 0, or 1 for the 2(w - Bl) line, 2 for the 2(Br - w) line.  Ties
+// resolve as Julia's min/max (and djl_min/djl_max) resolve them.  Every
+// branch is linear in its stencil, and variable v's face states depend on
+// variable v only, so the picks give the reconstruction's Jacobian.
+__device__ void recon_pick(const KParams& P, const double* U, const double* B, int d, int i, int j, double L[3],
+                           double R[3], int pick[3], int& dry) {
+    const size_t n = (size_t)P.nxT * P.nyT;
+    const int di = d == 0 ? 1 : 0, dj = d == 0 ? 0 : 1;
+    const double theta = P.theta;
+    double cc[3], slope[3];
+    for (int v = 0; v < 3; ++v) {
+        const double l = U[v * n + cid(P, i - di, j - dj)];
+        const double c = U[v * n + cid(P, i, j)];
+        const double r = U[v * n + cid(P, i + di, j + dj)];
+        cc[v] = c;
+        const double fwd = r - c, bwd = c - l;
+        const double a = theta * fwd, b = (fwd + bwd) / 2.0, e = theta * bwd;
+        int k = 3;
+        double m = 0.0;
+        if (a != a || b != b || e != e) {
+        } else if (a > 0 && b > 0 && e > 0) {        // djl_min(djl_min(a, b), e)
+            k = b <= a ? 1 : 0;
+            m = k ? b : a;
+            if (e <= m) { k = 2; m = e; }
+        } else if (a < 0 && b < 0 && e < 0) {        // djl_max(djl_max(a, b), e)
+            k = b >= a ? 1 : 0;
+            m = k ? b : a;
+            if (e >= m) { k = 2; m = e; }
+        }
+        pick[v] = k;
+        slope[v] = m;
+    }
+    const double Bl = B_face_left(P, B, i, j, d), Br = B_face_right(P, B, i, j, d);
+    const double w = cc[0];
+    dry = 0;
+    if (w - 0.5 * slope[0] < Bl) { slope[0] = 2.0 * (w - Bl); dry = 1; }
+    else if (w + 0.5 * slope[0] < Br) { slope[0] = 2.0 * (Br - w); dry = 2; }
+    for (int v = 0; v < 3; ++v) {
+        L[v] = cc[v] - 0.5 * slope[v];
+        R[v] = cc[v] + 0.5 * slope[v];
+    }
+    L[0] = L[0] - Bl;
+    R[0] = R[0] - Br;
+}
+
+// d slope / d U at stencil position p (-1 left, 0 centre, 1 right) for the
+// branch pick (and, for w, the dry-fix line).
+__device__ __forceinline__ double slope_coef(int pick, int dry, int p, double theta) {
+    if (dry == 1) return p == 0 ? 2.0 : 0.0;
+    if (dry == 2) return p == 0 ? -2.0 : 0.0;
+    if (pick == 0) return p == 1 ? theta : (p == 0 ? -theta : 0.0);
+    if (pick == 1) return p == 1 ? 0.5 : (p == -1 ? -0.5 : 0.0);
+    if (pick == 2) return p == 0 ? theta : (p == -1 ? -theta : 0.0);
+    return 0.0;
 }
 
 // ---------------------------------------------------------------- kernels
@@ -434,7 +514,9 @@ __global__ void __launch_bounds__(256) k_adj_faces(KParams P, int d, const doubl
     const int nT = d == 0 ? P.nxT : P.nyT, nO = d == 0 ? P.ny : P.nx;
     const int nf = nT - 2 * P.gc + 1;
     if (t >= nf * nO) return;
-    const int s = P.gc - 1 + t % nf, o = P.gc + t / nf;
+    // consecutive threads take consecutive i in both directions (coalesced)
+    const int s = d == 0 ? P.gc - 1 + t % nf : P.gc - 1 + t / nO;
+    const int o = d == 0 ? P.gc + t / nf : P.gc + t % nO;
     const int i = d == 0 ? s : o, j = d == 0 ? o : s;
     const int i1 = d == 0 ? i + 1 : i, j1 = d == 0 ? j : j + 1;
     const size_t n = (size_t)P.nxT * P.nyT;
@@ -483,36 +565,40 @@ __global__ void __launch_bounds__(256) k_adj_recon(KParams P, int d, const doubl
             aR[0] += c;
         }
     };
-    using D3 = Dual<3>;
     double acc[3] = {0.0, 0.0, 0.0};
+    double Lc[3], Rc[3];                              // the cell's own face states
+    int dry_c = 0;
     for (int off = -1; off <= 1; ++off) {
         const int kk = s + off;                       // the reconstructed cell along d
         if (kk < P.gc - 1 || kk > nT - P.gc) continue; // its face states are used only in this range
         const int ki = d == 0 ? kk : i, kj = d == 0 ? j : kk;
         double aL[3], aR[3];
         face_adj(kk, ki, kj, aL, aR);
-        D3 L[3], R[3];
-        recon_at<D3>(P, U, B, d, ki, kj, -off, false, L, R);   // this cell sits at offset -off in kk's stencil
-        for (int v = 0; v < 3; ++v)
-            for (int m = 0; m < 3; ++m) acc[v] += aL[m] * L[m].d[v] + aR[m] * R[m].d[v];
+        double L[3], R[3];
+        int pick[3], dry;
+        recon_pick(P, U, B, d, ki, kj, L, R, pick, dry);
+        const int p = -off;                           // this cell's position in kk's stencil
+        const double dc = p == 0 ? 1.0 : 0.0;
+        for (int v = 0; v < 3; ++v) {
+            const double cs = slope_coef(pick[v], v == 0 ? dry : 0, p, P.theta);
+            acc[v] += aL[v] * (dc - 0.5 * cs) + aR[v] * (dc + 0.5 * cs);
+        }
+        if (off == 0) {
+            for (int v = 0; v < 3; ++v) { Lc[v] = L[v]; Rc[v] = R[v]; }
+            dry_c = dry;
+        }
     }
     const size_t k = cid(P, i, j);
     for (int v = 0; v < 3; ++v) LU[v * n + k] += acc[v];
-    // the cell's own face bottoms
+    // the cell's own face bottoms: Bl and Br enter w's face states directly
+    // (L_w -= Bl, R_w -= Br) and through the dry-fix line
     if (s >= P.gc - 1 && s <= nT - P.gc) {
         double aL[3], aR[3];
         face_adj(s, i, j, aL, aR);
-        using D2 = Dual<2>;
-        D2 L[3], R[3];
-        recon_at<D2>(P, U, B, d, i, j, -2, true, L, R);
-        double lBl = 0.0, lBr = 0.0;
-        for (int m = 0; m < 3; ++m) {
-            lBl += aL[m] * L[m].d[0] + aR[m] * R[m].d[0];
-            lBr += aL[m] * L[m].d[1] + aR[m] * R[m].d[1];
-        }
+        const double sbl = dry_c == 1 ? -2.0 : 0.0, sbr = dry_c == 2 ? 2.0 : 0.0;   // d slope_w / d Bl, d Br
+        double lBl = aL[0] * ((0.0 - 0.5 * sbl) - 1.0) + aR[0] * (0.0 + 0.5 * sbl);
+        double lBr = aL[0] * (0.0 - 0.5 * sbr) + aR[0] * ((0.0 + 0.5 * sbr) - 1.0);
         if (P.bottom && s >= P.gc && s < nT - P.gc) {   // bottom source: -g ((Br - Bl)/Δ) ((hR + hL)/2)
-            double Lc[3], Rc[3];
-            recon_at<double>(P, U, B, d, i, j, -2, false, Lc, Rc);
             const double c = LR[(1 + d) * n + k] * (-g) * inv * ((Rc[0] + Lc[0]) / 2.0);
             lBr += c;
             lBl -= c;
