@@ -536,20 +536,18 @@ __global__ void __launch_bounds__(256) k_adj_faces(KParams P, int d, const doubl
     }
 }
 
-// Per cell of direction d: adjoint of the reconstruction (and of the bottom
+// Cell (i,j), direction d: adjoint of the reconstruction (and of the bottom
 // source, source_term_bottom.jl:24-41) gathered from the face-state adjoints
-// of the cell and its two neighbours along d.  Adds to LU; adds the face
-// bottom adjoints of the cell to LBf (planes: x left, x right, y left, y right).
-__global__ void __launch_bounds__(256) k_adj_recon(KParams P, int d, const double* __restrict__ U,
-                                                   const double* __restrict__ B, const double* __restrict__ LR,
-                                                   const double* __restrict__ LfL, const double* __restrict__ LfR,
-                                                   double* __restrict__ LU, double* __restrict__ LBf) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y;
-    if (i >= P.nxT || j >= P.nyT) return;
+// of the cell and its two neighbours along d.  Adds it to acc (has: whether
+// the cell takes part in direction d); adds the face bottom adjoints of the
+// cell to LBf (planes: x left, x right, y left, y right).
+__device__ __forceinline__ bool recon_dir(const KParams& P, int d, int i, int j, const double* __restrict__ U,
+                                          const double* __restrict__ B, const double* __restrict__ LR,
+                                          const double* __restrict__ LfL, const double* __restrict__ LfR,
+                                          double acc[3], double* __restrict__ LBf) {
     const int s = d == 0 ? i : j, o = d == 0 ? j : i;
     const int nT = d == 0 ? P.nxT : P.nyT, oT = d == 0 ? P.nyT : P.nxT;
-    if (o < P.gc || o >= oT - P.gc) return;
+    if (o < P.gc || o >= oT - P.gc) return false;
     const size_t n = (size_t)P.nxT * P.nyT;
     const double g = P.phys.g;
     const double Dd = d == 0 ? P.dx : P.dy, inv = d == 0 ? P.inv_dx : P.inv_dy;
@@ -565,12 +563,18 @@ __global__ void __launch_bounds__(256) k_adj_recon(KParams P, int d, const doubl
             aR[0] += c;
         }
     };
-    double acc[3] = {0.0, 0.0, 0.0};
     double Lc[3], Rc[3];                              // the cell's own face states
     int dry_c = 0;
+    // Branch-free over the three stencil offsets, so that every load can be
+    // issued up front: an offset whose cell lies outside the range where face
+    // states are used is evaluated at a clamped, in-bounds cell (the stencil
+    // of a cell in [gc-1, nT-gc] stays inside the array for gc >= 2) and its
+    // contribution is dropped.
+#pragma unroll
     for (int off = -1; off <= 1; ++off) {
-        const int kk = s + off;                       // the reconstructed cell along d
-        if (kk < P.gc - 1 || kk > nT - P.gc) continue; // its face states are used only in this range
+        const int kr = s + off;                       // the reconstructed cell along d
+        const bool use = kr >= P.gc - 1 && kr <= nT - P.gc;
+        const int kk = kr < P.gc - 1 ? P.gc - 1 : (kr > nT - P.gc ? nT - P.gc : kr);
         const int ki = d == 0 ? kk : i, kj = d == 0 ? j : kk;
         double aL[3], aR[3];
         face_adj(kk, ki, kj, aL, aR);
@@ -581,7 +585,8 @@ __global__ void __launch_bounds__(256) k_adj_recon(KParams P, int d, const doubl
         const double dc = p == 0 ? 1.0 : 0.0;
         for (int v = 0; v < 3; ++v) {
             const double cs = slope_coef(pick[v], v == 0 ? dry : 0, p, P.theta);
-            acc[v] += aL[v] * (dc - 0.5 * cs) + aR[v] * (dc + 0.5 * cs);
+            const double t = aL[v] * (dc - 0.5 * cs) + aR[v] * (dc + 0.5 * cs);
+            if (use) acc[v] += t;
         }
         if (off == 0) {
             for (int v = 0; v < 3; ++v) { Lc[v] = L[v]; Rc[v] = R[v]; }
@@ -589,7 +594,6 @@ __global__ void __launch_bounds__(256) k_adj_recon(KParams P, int d, const doubl
         }
     }
     const size_t k = cid(P, i, j);
-    for (int v = 0; v < 3; ++v) LU[v * n + k] += acc[v];
     // the cell's own face bottoms: Bl and Br enter w's face states directly
     // (L_w -= Bl, R_w -= Br) and through the dry-fix line
     if (s >= P.gc - 1 && s <= nT - P.gc) {
@@ -605,6 +609,31 @@ __global__ void __launch_bounds__(256) k_adj_recon(KParams P, int d, const doubl
         }
         LBf[(2 * d) * n + k] += lBl;
         LBf[(2 * d + 1) * n + k] += lBr;
+    }
+    return true;
+}
+
+// Both directions of the reconstruction adjoint for one cell, one
+// read-modify-write of LU: LU += acc_x, then += acc_y (the x and y face-state
+// adjoints come in their own buffers).
+__global__ void __launch_bounds__(256) k_adj_recon(KParams P, const double* __restrict__ U, const double* __restrict__ B,
+                                                   const double* __restrict__ LR, const double* __restrict__ LfL0,
+                                                   const double* __restrict__ LfR0, const double* __restrict__ LfL1,
+                                                   const double* __restrict__ LfR1, double* __restrict__ LU,
+                                                   double* __restrict__ LBf) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    if (i >= P.nxT || j >= P.nyT) return;
+    double ax[3] = {0.0, 0.0, 0.0}, ay[3] = {0.0, 0.0, 0.0};
+    const bool hx = recon_dir(P, 0, i, j, U, B, LR, LfL0, LfR0, ax, LBf);
+    const bool hy = recon_dir(P, 1, i, j, U, B, LR, LfL1, LfR1, ay, LBf);
+    if (!hx && !hy) return;
+    const size_t n = (size_t)P.nxT * P.nyT, k = cid(P, i, j);
+    for (int v = 0; v < 3; ++v) {
+        double lu = LU[v * n + k];
+        if (hx) lu += ax[v];
+        if (hy) lu += ay[v];
+        LU[v * n + k] = lu;
     }
 }
 
@@ -852,9 +881,9 @@ extern "C" int sfv_adjoint(sfv_handle* h, int64_t nsteps, int32_t checkpoint_eve
     double* Lnew = buf.alloc(n3, why, rc);                      // adjoint of its input
     double* L1 = buf.alloc(n3, why, rc);                        // adjoint of U1
     double* LR = buf.alloc(n3, why, rc);
-    double* LfL = buf.alloc(n3, why, rc);                       // face-state adjoints (x, and y of stage 2)
+    double* LfL = buf.alloc(n3, why, rc);                       // x face-state adjoints
     double* LfR = buf.alloc(n3, why, rc);
-    double* LfL1 = buf.alloc(n3, why, rc);                      // stage 1's y face-state adjoints
+    double* LfL1 = buf.alloc(n3, why, rc);                      // y face-state adjoints
     double* LfR1 = buf.alloc(n3, why, rc);
     double* LBc = buf.alloc(n, why, rc);
     double* LBf = buf.alloc(4 * n, why, rc);
@@ -958,10 +987,9 @@ extern "C" int sfv_adjoint(sfv_handle* h, int64_t nsteps, int32_t checkpoint_eve
             // writes L1 and Lnew afresh
             hipLaunchKernelGGL(ad::k_adj_tail, cg, cb, 0, st, P, h->ctl, 2, U0, U1, R2, h->B, L, L1, Lnew, LR, LBc,
                                part_dt, part_cz, best);
-            for (int d = 0; d < 2; ++d) {
-                hipLaunchKernelGGL(ad::k_adj_faces, faces_grid(d), dim3(256), 0, st, P, d, U1, h->B, LR, LfL, LfR);
-                hipLaunchKernelGGL(ad::k_adj_recon, cg, cb, 0, st, P, d, U1, h->B, LR, LfL, LfR, L1, LBf);
-            }
+            hipLaunchKernelGGL(ad::k_adj_faces, faces_grid(0), dim3(256), 0, st, P, 0, U1, h->B, LR, LfL, LfR);
+            hipLaunchKernelGGL(ad::k_adj_faces, faces_grid(1), dim3(256), 0, st, P, 1, U1, h->B, LR, LfL1, LfR1);
+            hipLaunchKernelGGL(ad::k_adj_recon, cg, cb, 0, st, P, U1, h->B, LR, LfL, LfR, LfL1, LfR1, L1, LBf);
             hipLaunchKernelGGL(ad::k_adj_bc, dim3(bcx), dim3(256), 0, st, P, L1, 0);
             hipLaunchKernelGGL(ad::k_adj_bc, dim3(bcy), dim3(256), 0, st, P, L1, 1);
             // stage 1: U1 = clamp(friction(U0 + dt R1))
@@ -974,8 +1002,7 @@ extern "C" int sfv_adjoint(sfv_handle* h, int64_t nsteps, int32_t checkpoint_eve
             hipLaunchKernelGGL(ad::k_adj_faces, faces_grid(1), dim3(256), 0, st, P, 1, U0, h->B, LR, LfL1, LfR1);
             hipLaunchKernelGGL(ad::k_adj_dtstep, dim3(1), dim3(256), 0, st, P, h->ctl, part_dt, part_cz, 2 * nblk,
                                sums, sums + 1, best, flag, U0, h->B, LfL, LfR, LfL1, LfR1);
-            hipLaunchKernelGGL(ad::k_adj_recon, cg, cb, 0, st, P, 0, U0, h->B, LR, LfL, LfR, Lnew, LBf);
-            hipLaunchKernelGGL(ad::k_adj_recon, cg, cb, 0, st, P, 1, U0, h->B, LR, LfL1, LfR1, Lnew, LBf);
+            hipLaunchKernelGGL(ad::k_adj_recon, cg, cb, 0, st, P, U0, h->B, LR, LfL, LfR, LfL1, LfR1, Lnew, LBf);
             ATRY(hipPeekAtLastError());
             std::swap(L, Lnew);
         }
