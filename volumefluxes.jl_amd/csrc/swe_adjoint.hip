@@ -409,6 +409,41 @@ __global__ void k_adj_bc(KParams P, double* __restrict__ L, int axis) {
     }
 }
 
+// The value k_adj_bc's two folds leave at cell (i,j), read without running
+// them: ghosts and corners read 0; an interior cell reads its own adjoint
+// plus, in the folds' order (x then y, by ghost index, low side then high
+// side), the adjoints of the ghosts copied from it, sign-flipped with the
+// normal momentum.  Exact because the ghosts' sources are interior cells
+// (no ghost feeds another) and the x and y folds touch disjoint ghosts.
+__device__ __forceinline__ void folded_adj(const KParams& P, const double* __restrict__ L, int i, int j, double lo[3]) {
+    const size_t n = (size_t)P.nxT * P.nyT;
+    const int gc = P.gc;
+    lo[0] = lo[1] = lo[2] = 0.0;
+    if (i < gc || i >= P.nxT - gc || j < gc || j >= P.nyT - gc) return;
+    const size_t k = cid(P, i, j);
+    for (int v = 0; v < 3; ++v) lo[v] = L[v * n + k];
+    for (int g = 0; g < gc; ++g) {
+        if (P.src_lo_x[g] == i) {
+            const size_t q = cid(P, g, j);
+            for (int v = 0; v < 3; ++v) lo[v] += ((P.flip_lo_x[g] && v == 1) ? -1.0 : 1.0) * L[v * n + q];
+        }
+        if (P.src_hi_x[g] == i) {
+            const size_t q = cid(P, P.nxT - gc + g, j);
+            for (int v = 0; v < 3; ++v) lo[v] += ((P.flip_hi_x[g] && v == 1) ? -1.0 : 1.0) * L[v * n + q];
+        }
+    }
+    for (int g = 0; g < gc; ++g) {
+        if (P.src_lo_y[g] == j) {
+            const size_t q = cid(P, i, g);
+            for (int v = 0; v < 3; ++v) lo[v] += ((P.flip_lo_y[g] && v == 2) ? -1.0 : 1.0) * L[v * n + q];
+        }
+        if (P.src_hi_y[g] == j) {
+            const size_t q = cid(P, i, P.nyT - gc + g);
+            for (int v = 0; v < 3; ++v) lo[v] += ((P.flip_hi_y[g] && v == 2) ? -1.0 : 1.0) * L[v * n + q];
+        }
+    }
+}
+
 // Block sum of one value into part[blockIdx]; deterministic (fixed tree).
 __device__ void block_sum(double v, double* part, int slot) {
     __shared__ double red[256];
@@ -425,7 +460,7 @@ __device__ void block_sum(double v, double* part, int slot) {
 
 // Adjoint of: pre = in + dt*R (stage 2: pre = 0.5*(U0 + in + dt*R)),
 // friction (scalar from `in`), dry clamp.  Lout: adjoint of the stage
-// output (ghosts already folded).  Stage 2 writes Lin and LU0 for every
+// output, read through its boundary-condition fold (folded_adj).  Stage 2 writes Lin and LU0 for every
 // cell and resets the step's argmax search (*best); stage 1 adds to Lin.
 // Writes LR = adjoint of R, adds the B_cell adjoint, and block partial sums
 // of the dt and Cz adjoints.
@@ -451,8 +486,8 @@ __global__ void __launch_bounds__(256) k_adj_tail(KParams P, const StepCtl* ctl,
             r[v] = R[v * n + k];
             ax[v] = x[v] + dt * r[v];
             pre[v] = stage == 2 ? 0.5 * (U0[v * n + k] + ax[v]) : ax[v];
-            lo[v] = Lout[v * n + k];
         }
+        folded_adj(P, Lout, i, j, lo);
         const double Bm = Bc4(P, B, i, j);
         double fs = 0.0, D = 1.0;
         if (P.friction) {
@@ -980,18 +1015,14 @@ extern "C" int sfv_adjoint(sfv_handle* h, int64_t nsteps, int32_t checkpoint_eve
             } else if ((rc = forward_step(h, times[(size_t)s], U0, U1, R1, R2, Ut))) {
                 return rc;
             }
-            // fold the output's ghosts (its boundary condition)
-            hipLaunchKernelGGL(ad::k_adj_bc, dim3(bcx), dim3(256), 0, st, P, L, 0);
-            hipLaunchKernelGGL(ad::k_adj_bc, dim3(bcy), dim3(256), 0, st, P, L, 1);
             // stage 2: U2 = clamp(friction(0.5 (U0 + U1 + dt R2))); the tail
-            // writes L1 and Lnew afresh
+            // reads L through the boundary condition's fold and writes L1 and
+            // Lnew afresh
             hipLaunchKernelGGL(ad::k_adj_tail, cg, cb, 0, st, P, h->ctl, 2, U0, U1, R2, h->B, L, L1, Lnew, LR, LBc,
                                part_dt, part_cz, best);
             hipLaunchKernelGGL(ad::k_adj_faces, faces_grid(0), dim3(256), 0, st, P, 0, U1, h->B, LR, LfL, LfR);
             hipLaunchKernelGGL(ad::k_adj_faces, faces_grid(1), dim3(256), 0, st, P, 1, U1, h->B, LR, LfL1, LfR1);
             hipLaunchKernelGGL(ad::k_adj_recon, cg, cb, 0, st, P, U1, h->B, LR, LfL, LfR, LfL1, LfR1, L1, LBf);
-            hipLaunchKernelGGL(ad::k_adj_bc, dim3(bcx), dim3(256), 0, st, P, L1, 0);
-            hipLaunchKernelGGL(ad::k_adj_bc, dim3(bcy), dim3(256), 0, st, P, L1, 1);
             // stage 1: U1 = clamp(friction(U0 + dt R1))
             hipLaunchKernelGGL(ad::k_adj_tail, cg, cb, 0, st, P, h->ctl, 1, U0, U0, R1, h->B, L1, Lnew, nullptr, LR,
                                LBc, part_dt, part_cz, best);
