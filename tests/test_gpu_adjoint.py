@@ -94,3 +94,13 @@ def test_adjoint_rejects_unsupported_runs():
     sc = terrain(20, 20, seed=1, bc=VF.NeumannBC())
     with pytest.raises(VF.ArgumentError):
         VF.adjoint_gradient(make_hip(sc), 3, np.zeros((20, 20)))
+
+
+@pytest.mark.parametrize("nx,ny,gc,bc", [(3, 20, 2, "wall"), (26, 3, 2, "wall"), (4, 18, 3, "periodic"),
+                                         (22, 17, 3, "wall"), (19, 23, 3, "periodic")])
+def test_adjoint_narrow_grids_and_wide_halos(nx, ny, gc, bc):
+    """gc = 3, and grids so narrow that one interior cell is the source of a
+    low-side and a high-side ghost: the tails read the boundary condition's
+    fold as a gather, which must add every ghost's adjoint into its source."""
+    b = VF.WallBC() if bc == "wall" else VF.PeriodicBC()
+    _check(smooth_flow(terrain(nx, ny, seed=31 + nx + ny, gc=gc, bc=b)), 8, 3)
