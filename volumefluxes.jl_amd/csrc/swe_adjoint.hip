@@ -163,11 +163,11 @@ __device__ __forceinline__ void side_vjp(int d, const double s[3], const PhysPar
 
 // Reverse sweep of the central-upwind flux (centralupwind.jl:48-77) at one
 // face: given the flux adjoint lF, the adjoints of the minus side m and the
-// plus side q.  The values and every branch are those of central_upwind on
+// plus side q (and the face's speed).  The values and every branch are those of central_upwind on
 // doubles; the combination of the two sides and each side's flux are
 // differentiated by hand (side_vjp).
 __device__ __forceinline__ void face_vjp(int d, const double m[3], const double q[3], const PhysParams& p, const double lF[3],
-                         double lm[3], double lq[3]) {
+                         double lm[3], double lq[3], double& speed) {
     double fm[3], em[2], fp[3], ep[2];
     const bool wm = side_values(d, m, p, fm, em);
     const bool wq = side_values(d, q, p, fp, ep);
@@ -181,6 +181,9 @@ __device__ __forceinline__ void face_vjp(int d, const double m[3], const double 
     const int mk2 = jmin_pick(am1, 0.0);
     const double aminus = mk2 ? 0.0 : am1;
     const double D = aplus - aminus;
+    // the face speed central_upwind returns (centralupwind.jl:48-77)
+    speed = (__builtin_fabs(D) < p.kappa || (m[0] < p.cutoff && q[0] < p.cutoff))
+                ? 0.0 : djl_max(__builtin_fabs(aplus), __builtin_fabs(aminus));
     if (__builtin_fabs(D) < p.kappa) return;       // the flux is the constant zero
     const double pm = (aplus * aminus) / D;
     double l_ap = 0.0, l_am = 0.0, l_D = 0.0, l_pm = 0.0, l_fm[3], l_fp[3];
@@ -537,6 +540,14 @@ __global__ void __launch_bounds__(256) k_adj_tail(KParams P, const StepCtl* ctl,
     block_sum(gcz, part_cz, (stage == 2 ? 0 : nblk) + bid);
 }
 
+// The direction whose wave speed set dt: min(Δx/a_x, Δy/a_y) picks y on a
+// tie (simulator.jl:147-150 with Julia's minimum).
+__device__ __forceinline__ int dt_direction(const KParams& P, const StepCtl* ctl) {
+    const double ax = __longlong_as_double((long long)ctl->slot[0].wsmax[0]);
+    const double ay = __longlong_as_double((long long)ctl->slot[0].wsmax[1]);
+    return (P.dx / ax < P.dy / ay) ? 0 : 1;
+}
+
 // Face fluxes of direction d (numericalflux.jl:21-33): face between cell c
 // and c+1 along d, c from gc-1 to nT-gc-1, other index interior.  The flux
 // adjoint is (1/Δ)(LR(c+1) - LR(c)) over interior cells; its transpose
@@ -544,7 +555,8 @@ __global__ void __launch_bounds__(256) k_adj_tail(KParams P, const StepCtl* ctl,
 // state and (c+1)'s left face state.
 __global__ void __launch_bounds__(256) k_adj_faces(KParams P, int d, const double* __restrict__ U,
                                                    const double* __restrict__ B, const double* __restrict__ LR,
-                                                   double* __restrict__ LfL, double* __restrict__ LfR) {
+                                                   double* __restrict__ LfL, double* __restrict__ LfR,
+                                                   const StepCtl* __restrict__ ctl, unsigned long long* __restrict__ best) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     const int nT = d == 0 ? P.nxT : P.nyT, nO = d == 0 ? P.ny : P.nx;
     const int nf = nT - 2 * P.gc + 1;
@@ -563,11 +575,24 @@ __global__ void __launch_bounds__(256) k_adj_faces(KParams P, int d, const doubl
     double Lc[3], Rc[3], L1[3], R1[3];
     recon_at<double>(P, U, B, d, i, j, -2, false, Lc, Rc);
     recon_at<double>(P, U, B, d, i1, j1, -2, false, L1, R1);
-    double gm[3], gq[3];
-    ad::face_vjp(d, Rc, L1, P.phys, lF, gm, gq);
+    double gm[3], gq[3], speed;
+    ad::face_vjp(d, Rc, L1, P.phys, lF, gm, gq, speed);
     for (int a = 0; a < 3; ++a) {
         LfR[a * n + kc] = gm[a];
         LfL[a * n + k1] = gq[a];
+    }
+    // With best given: the face that set dt.  maximum(wavespeeds) returns the
+    // first interior cell in memory order whose wavespeed, the max of its two
+    // face speeds (numericalflux.jl:29-32), is the step's maximum; a cell's
+    // wavespeed is the maximum exactly when one of its faces' speeds is, so
+    // each face at the maximum offers its interior cells and the smallest
+    // memory index wins.
+    if (best && d == dt_direction(P, ctl) &&
+        (unsigned long long)__double_as_longlong(speed) == ctl->slot[0].wsmax[d]) {
+        unsigned long long c = ~0ull;
+        if (c_in) c = kc;
+        if (c1_in && k1 < c) c = k1;
+        if (c != ~0ull) atomicMin(best, c);
     }
 }
 
@@ -677,8 +702,9 @@ __global__ void __launch_bounds__(256) k_adj_recon(KParams P, const double* __re
 // wavespeed being max(right face speed, left face speed) (numericalflux.jl:29-32);
 // with equal speeds on several faces the derivative depends on which one the
 // maximum returns, so the same one is taken: the first cell in memory order
-// whose wavespeed is the maximum, and its right face unless the left one is
-// strictly faster.  One thread per interior cell; best = its memory index.
+// whose wavespeed is the maximum (found by the stage-1 face kernel: best =
+// its memory index), and its right face unless the left one is strictly
+// faster.
 __device__ inline void face_speeds(const KParams& P, const double* U, const double* B, int d, int i, int j, double& sr,
                                    double& sl) {
     const int di = d == 0 ? 1 : 0, dj = d == 0 ? 0 : 1;
@@ -690,28 +716,9 @@ __device__ inline void face_speeds(const KParams& P, const double* U, const doub
     sl = ad::central_upwind<double>(d, Rl, Lm, P.phys, F);
 }
 
-// The direction whose wave speed set dt: min(Δx/a_x, Δy/a_y) picks y on a
-// tie (simulator.jl:147-150 with Julia's minimum).
-__device__ __forceinline__ int dt_direction(const KParams& P, const StepCtl* ctl) {
-    const double ax = __longlong_as_double((long long)ctl->slot[0].wsmax[0]);
-    const double ay = __longlong_as_double((long long)ctl->slot[0].wsmax[1]);
-    return (P.dx / ax < P.dy / ay) ? 0 : 1;
-}
 
-__global__ void __launch_bounds__(256) k_adj_argmax(KParams P, const StepCtl* ctl, const double* __restrict__ U,
-                                                    const double* __restrict__ B, unsigned long long* __restrict__ best) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= P.nx * P.ny) return;
-    const int dsel = dt_direction(P, ctl);
-    const int i = P.gc + t % P.nx, j = P.gc + t / P.nx;
-    double sr, sl;
-    face_speeds(P, U, B, dsel, i, j, sr, sl);
-    const double ws = djl_max(sr, sl);
-    if ((unsigned long long)__double_as_longlong(ws) == ctl->slot[0].wsmax[dsel])
-        atomicMin(best, (unsigned long long)cid(P, i, j));
-}
 
-// Once per reverse step, after the stage tails, the argmax search and both
+// Once per reverse step, after the stage tails and both
 // directions' face adjoints of stage 1: (1) sum the two stages' block
 // partials of the dt and Cz adjoints in a fixed order (sums[0] = the step's
 // dt adjoint, cz_total += its Cz adjoint); (2) a step whose dt came from a
@@ -1020,17 +1027,20 @@ extern "C" int sfv_adjoint(sfv_handle* h, int64_t nsteps, int32_t checkpoint_eve
             // Lnew afresh
             hipLaunchKernelGGL(ad::k_adj_tail, cg, cb, 0, st, P, h->ctl, 2, U0, U1, R2, h->B, L, L1, Lnew, LR, LBc,
                                part_dt, part_cz, best);
-            hipLaunchKernelGGL(ad::k_adj_faces, faces_grid(0), dim3(256), 0, st, P, 0, U1, h->B, LR, LfL, LfR);
-            hipLaunchKernelGGL(ad::k_adj_faces, faces_grid(1), dim3(256), 0, st, P, 1, U1, h->B, LR, LfL1, LfR1);
+            hipLaunchKernelGGL(ad::k_adj_faces, faces_grid(0), dim3(256), 0, st, P, 0, U1, h->B, LR, LfL, LfR, h->ctl,
+                               nullptr);
+            hipLaunchKernelGGL(ad::k_adj_faces, faces_grid(1), dim3(256), 0, st, P, 1, U1, h->B, LR, LfL1, LfR1, h->ctl,
+                               nullptr);
             hipLaunchKernelGGL(ad::k_adj_recon, cg, cb, 0, st, P, U1, h->B, LR, LfL, LfR, LfL1, LfR1, L1, LBf);
             // stage 1: U1 = clamp(friction(U0 + dt R1))
             hipLaunchKernelGGL(ad::k_adj_tail, cg, cb, 0, st, P, h->ctl, 1, U0, U0, R1, h->B, L1, Lnew, nullptr, LR,
                                LBc, part_dt, part_cz, best);
-            // the face that set dt, found as the reference's maximum(ws) would
-            hipLaunchKernelGGL(ad::k_adj_argmax, dim3((P.nx * P.ny + 255) / 256), dim3(256), 0, st, P, h->ctl, U0,
-                               h->B, best);
-            hipLaunchKernelGGL(ad::k_adj_faces, faces_grid(0), dim3(256), 0, st, P, 0, U0, h->B, LR, LfL, LfR);
-            hipLaunchKernelGGL(ad::k_adj_faces, faces_grid(1), dim3(256), 0, st, P, 1, U0, h->B, LR, LfL1, LfR1);
+            // stage 1's faces also find the face that set dt, as the
+            // reference's maximum(ws) would
+            hipLaunchKernelGGL(ad::k_adj_faces, faces_grid(0), dim3(256), 0, st, P, 0, U0, h->B, LR, LfL, LfR, h->ctl,
+                               best);
+            hipLaunchKernelGGL(ad::k_adj_faces, faces_grid(1), dim3(256), 0, st, P, 1, U0, h->B, LR, LfL1, LfR1, h->ctl,
+                               best);
             hipLaunchKernelGGL(ad::k_adj_dtstep, dim3(1), dim3(256), 0, st, P, h->ctl, part_dt, part_cz, 2 * nblk,
                                sums, sums + 1, best, flag, U0, h->B, LfL, LfR, LfL1, LfR1);
             hipLaunchKernelGGL(ad::k_adj_recon, cg, cb, 0, st, P, U0, h->B, LR, LfL, LfR, LfL1, LfR1, Lnew, LBf);
