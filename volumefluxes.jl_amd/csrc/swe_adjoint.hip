@@ -1013,35 +1013,36 @@ extern "C" int sfv_adjoint(sfv_handle* h, int64_t nsteps, int32_t checkpoint_eve
         }
         for (int64_t s = s1 - 1; s >= s0; --s) {
             const double* U0 = seg + (size_t)(s - s0) * n3;
+            const StepCtl* cs = h->ctl;                 // the step's control block (its dt and maxima)
             if (keep) {
                 const size_t o = (size_t)(s - s0) * n3;
                 U1 = kU1 + o;
                 R1 = kR1 + o;
                 R2 = kR2 + o;
-                ATRY(hipMemcpyAsync(h->ctl, kctl + (s - s0), sizeof(StepCtl), hipMemcpyDeviceToDevice, st));
+                cs = kctl + (s - s0);
             } else if ((rc = forward_step(h, times[(size_t)s], U0, U1, R1, R2, Ut))) {
                 return rc;
             }
             // stage 2: U2 = clamp(friction(0.5 (U0 + U1 + dt R2))); the tail
             // reads L through the boundary condition's fold and writes L1 and
             // Lnew afresh
-            hipLaunchKernelGGL(ad::k_adj_tail, cg, cb, 0, st, P, h->ctl, 2, U0, U1, R2, h->B, L, L1, Lnew, LR, LBc,
+            hipLaunchKernelGGL(ad::k_adj_tail, cg, cb, 0, st, P, cs, 2, U0, U1, R2, h->B, L, L1, Lnew, LR, LBc,
                                part_dt, part_cz, best);
-            hipLaunchKernelGGL(ad::k_adj_faces, faces_grid(0), dim3(256), 0, st, P, 0, U1, h->B, LR, LfL, LfR, h->ctl,
+            hipLaunchKernelGGL(ad::k_adj_faces, faces_grid(0), dim3(256), 0, st, P, 0, U1, h->B, LR, LfL, LfR, cs,
                                nullptr);
-            hipLaunchKernelGGL(ad::k_adj_faces, faces_grid(1), dim3(256), 0, st, P, 1, U1, h->B, LR, LfL1, LfR1, h->ctl,
+            hipLaunchKernelGGL(ad::k_adj_faces, faces_grid(1), dim3(256), 0, st, P, 1, U1, h->B, LR, LfL1, LfR1, cs,
                                nullptr);
             hipLaunchKernelGGL(ad::k_adj_recon, cg, cb, 0, st, P, U1, h->B, LR, LfL, LfR, LfL1, LfR1, L1, LBf);
             // stage 1: U1 = clamp(friction(U0 + dt R1))
-            hipLaunchKernelGGL(ad::k_adj_tail, cg, cb, 0, st, P, h->ctl, 1, U0, U0, R1, h->B, L1, Lnew, nullptr, LR,
+            hipLaunchKernelGGL(ad::k_adj_tail, cg, cb, 0, st, P, cs, 1, U0, U0, R1, h->B, L1, Lnew, nullptr, LR,
                                LBc, part_dt, part_cz, best);
             // stage 1's faces also find the face that set dt, as the
             // reference's maximum(ws) would
-            hipLaunchKernelGGL(ad::k_adj_faces, faces_grid(0), dim3(256), 0, st, P, 0, U0, h->B, LR, LfL, LfR, h->ctl,
+            hipLaunchKernelGGL(ad::k_adj_faces, faces_grid(0), dim3(256), 0, st, P, 0, U0, h->B, LR, LfL, LfR, cs,
                                best);
-            hipLaunchKernelGGL(ad::k_adj_faces, faces_grid(1), dim3(256), 0, st, P, 1, U0, h->B, LR, LfL1, LfR1, h->ctl,
+            hipLaunchKernelGGL(ad::k_adj_faces, faces_grid(1), dim3(256), 0, st, P, 1, U0, h->B, LR, LfL1, LfR1, cs,
                                best);
-            hipLaunchKernelGGL(ad::k_adj_dtstep, dim3(1), dim3(256), 0, st, P, h->ctl, part_dt, part_cz, 2 * nblk,
+            hipLaunchKernelGGL(ad::k_adj_dtstep, dim3(1), dim3(256), 0, st, P, cs, part_dt, part_cz, 2 * nblk,
                                sums, sums + 1, best, flag, U0, h->B, LfL, LfR, LfL1, LfR1);
             hipLaunchKernelGGL(ad::k_adj_recon, cg, cb, 0, st, P, U0, h->B, LR, LfL, LfR, LfL1, LfR1, Lnew, LBf);
             ATRY(hipPeekAtLastError());
