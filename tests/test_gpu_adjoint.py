@@ -79,15 +79,19 @@ def test_adjoint_single_corners_and_friction():
 
 def test_adjoint_recompute_path_equals_kept_path(monkeypatch):
     """Segments either keep their stage-1 states and right-hand sides from the
-    recompute or recompute them per reverse step: the same kernels on the same
-    inputs, so the gradients are identical."""
+    recompute (on a second stream, overlapped with the previous segment's
+    reverse steps, or on one stream) or recompute them per reverse step: the
+    same kernels on the same inputs, so the gradients are identical."""
     sc = terrain(44, 36, seed=25)
     tgt = _target(sc)
     kept = VF.adjoint_gradient(make_hip(sc), 14, tgt, checkpoint_every=4)
-    monkeypatch.setenv("SFV_ADJOINT_KEEP", "0")
-    again = VF.adjoint_gradient(make_hip(sc), 14, tgt, checkpoint_every=4)
-    assert kept["J"] == again["J"] and kept["dJ_dCz"] == again["dJ_dCz"]
-    assert np.array_equal(kept["dJ_dB"], again["dJ_dB"]) and np.array_equal(kept["dJ_dU0"], again["dJ_dU0"])
+    runs = []
+    for env in (("SFV_ADJOINT_OVERLAP", "0"), ("SFV_ADJOINT_KEEP", "0")):
+        monkeypatch.setenv(*env)
+        runs.append(VF.adjoint_gradient(make_hip(sc), 14, tgt, checkpoint_every=4))
+    for again in runs:
+        assert kept["J"] == again["J"] and kept["dJ_dCz"] == again["dJ_dCz"]
+        assert np.array_equal(kept["dJ_dB"], again["dJ_dB"]) and np.array_equal(kept["dJ_dU0"], again["dJ_dU0"])
 
 
 def test_adjoint_rejects_unsupported_runs():

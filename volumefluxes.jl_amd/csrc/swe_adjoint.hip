@@ -854,6 +854,30 @@ struct AdjBuffers {
     }
 };
 
+// A second stream for the recompute and its events; drained and released
+// on every return path.
+struct SideStream {
+    hipStream_t s = nullptr;
+    hipEvent_t fwd = nullptr, ready[2] = {nullptr, nullptr}, done[2] = {nullptr, nullptr};
+    hipError_t create() {
+        hipError_t e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&fwd, hipEventDisableTiming);
+        for (int k = 0; k < 2 && e == hipSuccess; ++k) {
+            e = hipEventCreateWithFlags(&ready[k], hipEventDisableTiming);
+            if (e == hipSuccess) e = hipEventCreateWithFlags(&done[k], hipEventDisableTiming);
+        }
+        return e;
+    }
+    ~SideStream() {
+        if (s) {
+            (void)hipStreamSynchronize(s);
+            (void)hipStreamDestroy(s);
+        }
+        for (hipEvent_t e : {fwd, ready[0], ready[1], done[0], done[1]})
+            if (e) (void)hipEventDestroy(e);
+    }
+};
+
 #define ATRY(expr)                                                                                   \
     do {                                                                                             \
         hipError_t e_ = (expr);                                                                      \
@@ -877,9 +901,9 @@ int march_step(sfv_handle* h, int slot, const double* U0, double* U2) {
 
 // One forward step U0 -> U2 with the per-pass kernels (bitwise the fused
 // path), keeping the stage-1 state and both right-hand sides.
-int forward_step(sfv_handle* h, double t, const double* U0, double* U1, double* R1, double* R2, double* U2) {
+int forward_step(sfv_handle* h, hipStream_t st, double t, const double* U0, double* U1, double* R1, double* R2,
+                 double* U2) {
     const KParams& P = h->P;
-    hipStream_t st = h->stream;
     ATRY(launch_prepare(st, P, h->ctl, 0, t, NAN, 0, 0, 0.0, 1, 0, SFV_F64));
     ATRY(launch_flux<double>(st, P, h->ctl, 0, U0, R1, h->B, 1));
     ATRY(launch_finish<double>(st, P, h->ctl, 0, 1, U0, U0, R1, U1, h->B));
@@ -948,10 +972,26 @@ extern "C" int sfv_adjoint(sfv_handle* h, int64_t nsteps, int32_t checkpoint_eve
     const char* keep_env = std::getenv("SFV_ADJOINT_KEEP");      // "0": always recompute
     const bool keep = !rc && !(keep_env && keep_env[0] == '0') && hipMemGetInfo(&free_b, &total_b) == hipSuccess &&
                       keep_bytes < free_b / 2;
-    double* kU1 = keep ? buf.alloc(n3 * (size_t)ce, why, rc) : nullptr;
-    double* kR1 = keep ? buf.alloc(n3 * (size_t)ce, why, rc) : nullptr;
-    double* kR2 = keep ? buf.alloc(n3 * (size_t)ce, why, rc) : nullptr;
-    StepCtl* kctl = keep ? (StepCtl*)buf.alloc((size_t)ce * (sizeof(StepCtl) / sizeof(double) + 1), why, rc) : nullptr;
+    // When two sets of them (and a second segment of states) fit as well,
+    // segment sg-1 is recomputed on a second stream while the reverse steps
+    // of segment sg run.
+    const char* ovl_env = std::getenv("SFV_ADJOINT_OVERLAP");   // "0": one stream
+    const size_t seg_bytes = n3 * (size_t)(ce + 1) * sizeof(double);
+    const bool overlap = keep && nseg > 1 && !(ovl_env && ovl_env[0] == '0') &&
+                         2 * keep_bytes + seg_bytes < free_b / 2;
+    const size_t ctl_words = (size_t)ce * (sizeof(StepCtl) / sizeof(double) + 1);
+    double* kU1s[2] = {nullptr, nullptr};
+    double* kR1s[2] = {nullptr, nullptr};
+    double* kR2s[2] = {nullptr, nullptr};
+    StepCtl* kctls[2] = {nullptr, nullptr};
+    double* segs[2] = {seg, nullptr};
+    for (int k = 0; k < (overlap ? 2 : keep ? 1 : 0); ++k) {
+        kU1s[k] = buf.alloc(n3 * (size_t)ce, why, rc);
+        kR1s[k] = buf.alloc(n3 * (size_t)ce, why, rc);
+        kR2s[k] = buf.alloc(n3 * (size_t)ce, why, rc);
+        kctls[k] = (StepCtl*)buf.alloc(ctl_words, why, rc);
+    }
+    if (overlap) segs[1] = buf.alloc(n3 * (size_t)(ce + 1), why, rc);
     if (rc) { h->err = why; return rc; }
     ATRY(hipMemcpyAsync(tgt, h_target, sizeof(double) * P.nx * P.ny, hipMemcpyHostToDevice, st));
 
@@ -998,43 +1038,57 @@ extern "C" int sfv_adjoint(sfv_handle* h, int64_t nsteps, int32_t checkpoint_eve
         const int nT = d == 0 ? P.nxT : P.nyT, nO = d == 0 ? P.ny : P.nx;
         return dim3((unsigned)(((nT - 2 * P.gc + 1) * nO + 255) / 256));
     };
-    for (int64_t sg = nseg - 1; sg >= 0; --sg) {
+    // Recompute of segment sg from its checkpoint into buffer set `set`, on
+    // stream rs: with keep, every step's stage-1 state, right-hand sides and
+    // control block are kept; without, only the states.
+    auto recompute = [&](int64_t sg, int set, hipStream_t rs) -> int {
         const int64_t s0 = sg * ce, s1 = std::min<int64_t>(nsteps, s0 + ce);
-        ATRY(hipMemcpyAsync(seg, ck + (size_t)sg * n3, n3 * sizeof(double), hipMemcpyDeviceToDevice, st));
+        double* sb = segs[set];
+        ATRY(hipMemcpyAsync(sb, ck + (size_t)sg * n3, n3 * sizeof(double), hipMemcpyDeviceToDevice, rs));
         for (int64_t s = s0; s < (keep ? s1 : s1 - 1); ++s) {
-            double* a = seg + (size_t)(s - s0) * n3;
+            double* a = sb + (size_t)(s - s0) * n3;
             const size_t o = (size_t)(s - s0) * n3;
+            int r;
             if (keep) {
-                if ((rc = forward_step(h, times[(size_t)s], a, kU1 + o, kR1 + o, kR2 + o, a + n3))) return rc;
-                ATRY(hipMemcpyAsync(kctl + (s - s0), h->ctl, sizeof(StepCtl), hipMemcpyDeviceToDevice, st));
-            } else {
-                if ((rc = forward_step(h, times[(size_t)s], a, U1, R1, R2, a + n3))) return rc;
+                if ((r = forward_step(h, rs, times[(size_t)s], a, kU1s[set] + o, kR1s[set] + o, kR2s[set] + o,
+                                      a + n3)))
+                    return r;
+                ATRY(hipMemcpyAsync(kctls[set] + (s - s0), h->ctl, sizeof(StepCtl), hipMemcpyDeviceToDevice, rs));
+            } else if ((r = forward_step(h, rs, times[(size_t)s], a, U1, R1, R2, a + n3))) {
+                return r;
             }
         }
+        return SFV_OK;
+    };
+    // The reverse steps of segment sg (buffer set `set`), on the main stream.
+    auto reverse = [&](int64_t sg, int set) -> int {
+        const int64_t s0 = sg * ce, s1 = std::min<int64_t>(nsteps, s0 + ce);
         for (int64_t s = s1 - 1; s >= s0; --s) {
-            const double* U0 = seg + (size_t)(s - s0) * n3;
+            const size_t o = (size_t)(s - s0) * n3;
+            const double* U0 = segs[set] + o;
+            const double *u1 = U1, *r1 = R1, *r2 = R2;
             const StepCtl* cs = h->ctl;                 // the step's control block (its dt and maxima)
             if (keep) {
-                const size_t o = (size_t)(s - s0) * n3;
-                U1 = kU1 + o;
-                R1 = kR1 + o;
-                R2 = kR2 + o;
-                cs = kctl + (s - s0);
-            } else if ((rc = forward_step(h, times[(size_t)s], U0, U1, R1, R2, Ut))) {
-                return rc;
+                u1 = kU1s[set] + o;
+                r1 = kR1s[set] + o;
+                r2 = kR2s[set] + o;
+                cs = kctls[set] + (s - s0);
+            } else {
+                const int r = forward_step(h, st, times[(size_t)s], U0, U1, R1, R2, Ut);
+                if (r) return r;
             }
             // stage 2: U2 = clamp(friction(0.5 (U0 + U1 + dt R2))); the tail
             // reads L through the boundary condition's fold and writes L1 and
             // Lnew afresh
-            hipLaunchKernelGGL(ad::k_adj_tail, cg, cb, 0, st, P, cs, 2, U0, U1, R2, h->B, L, L1, Lnew, LR, LBc,
+            hipLaunchKernelGGL(ad::k_adj_tail, cg, cb, 0, st, P, cs, 2, U0, u1, r2, h->B, L, L1, Lnew, LR, LBc,
                                part_dt, part_cz, best);
-            hipLaunchKernelGGL(ad::k_adj_faces, faces_grid(0), dim3(256), 0, st, P, 0, U1, h->B, LR, LfL, LfR, cs,
+            hipLaunchKernelGGL(ad::k_adj_faces, faces_grid(0), dim3(256), 0, st, P, 0, u1, h->B, LR, LfL, LfR, cs,
                                nullptr);
-            hipLaunchKernelGGL(ad::k_adj_faces, faces_grid(1), dim3(256), 0, st, P, 1, U1, h->B, LR, LfL1, LfR1, cs,
+            hipLaunchKernelGGL(ad::k_adj_faces, faces_grid(1), dim3(256), 0, st, P, 1, u1, h->B, LR, LfL1, LfR1, cs,
                                nullptr);
-            hipLaunchKernelGGL(ad::k_adj_recon, cg, cb, 0, st, P, U1, h->B, LR, LfL, LfR, LfL1, LfR1, L1, LBf);
+            hipLaunchKernelGGL(ad::k_adj_recon, cg, cb, 0, st, P, u1, h->B, LR, LfL, LfR, LfL1, LfR1, L1, LBf);
             // stage 1: U1 = clamp(friction(U0 + dt R1))
-            hipLaunchKernelGGL(ad::k_adj_tail, cg, cb, 0, st, P, cs, 1, U0, U0, R1, h->B, L1, Lnew, nullptr, LR,
+            hipLaunchKernelGGL(ad::k_adj_tail, cg, cb, 0, st, P, cs, 1, U0, U0, r1, h->B, L1, Lnew, nullptr, LR,
                                LBc, part_dt, part_cz, best);
             // stage 1's faces also find the face that set dt, as the
             // reference's maximum(ws) would
@@ -1047,6 +1101,37 @@ extern "C" int sfv_adjoint(sfv_handle* h, int64_t nsteps, int32_t checkpoint_eve
             hipLaunchKernelGGL(ad::k_adj_recon, cg, cb, 0, st, P, U0, h->B, LR, LfL, LfR, LfL1, LfR1, Lnew, LBf);
             ATRY(hipPeekAtLastError());
             std::swap(L, Lnew);
+        }
+        return SFV_OK;
+    };
+    SideStream side;
+    if (overlap) {
+        // the side stream starts after the forward pass; segment sg's reverse
+        // waits for its recompute, and a recompute into a buffer set waits
+        // for the reverse that last read it
+        ATRY(side.create());
+        ATRY(hipEventRecord(side.fwd, st));
+        ATRY(hipStreamWaitEvent(side.s, side.fwd, 0));
+        bool used[2] = {false, false};
+        const int last = (int)((nseg - 1) & 1);
+        if ((rc = recompute(nseg - 1, last, side.s))) return rc;
+        ATRY(hipEventRecord(side.ready[last], side.s));
+        for (int64_t sg = nseg - 1; sg >= 0; --sg) {
+            const int set = (int)(sg & 1), other = 1 - set;
+            ATRY(hipStreamWaitEvent(st, side.ready[set], 0));
+            if (sg > 0) {
+                if (used[other]) ATRY(hipStreamWaitEvent(side.s, side.done[other], 0));
+                if ((rc = recompute(sg - 1, other, side.s))) return rc;
+                ATRY(hipEventRecord(side.ready[other], side.s));
+            }
+            if ((rc = reverse(sg, set))) return rc;
+            ATRY(hipEventRecord(side.done[set], st));
+            used[set] = true;
+        }
+    } else {
+        for (int64_t sg = nseg - 1; sg >= 0; --sg) {
+            if ((rc = recompute(sg, 0, st))) return rc;
+            if ((rc = reverse(sg, 0))) return rc;
         }
     }
     // the initial state's own boundary condition
