@@ -11,21 +11,23 @@
 //
 // Memory: the forward pass keeps a checkpoint every `checkpoint_every`
 // steps.  The reverse pass recomputes one segment from its checkpoint (the
-// per-pass kernels, bitwise the forward run), then walks it backwards; one
-// reverse step recomputes that step's stage-1 state U1, the right-hand sides
-// R1 = L(U0) + sources, R2 = L(U1) + sources and dt, and applies, per stage
-// in reverse order: boundary-condition fold, dry clamp, implicit friction,
-// the RK combination, then per direction the face fluxes (dual-number
-// Jacobian of the central-upwind flux at each face) and the reconstruction
-// (dual-number Jacobian of each cell's face states).  All kernels are
-// gathers: every output element is written by one thread.
+// per-pass kernels, bitwise the forward run), keeping each step's stage-1
+// state U1, right-hand sides R1 = L(U0) + sources, R2 = L(U1) + sources and
+// control block when they fit (on a second stream, overlapped with the
+// previous segment's reverse steps, when two sets fit), then walks it
+// backwards.  Per stage in reverse order a reverse step applies the
+// boundary-condition fold (read as a gather), dry clamp, implicit friction
+// and the RK combination, then the face fluxes of both directions (the
+// central-upwind flux swept back by hand at each face) and the
+// reconstruction (its branch-wise linear Jacobian).  All kernels are gathers
+// apart from the dt-face search: every output element is written by one
+// thread.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
 #include <string>
-#include <type_traits>
 #include <vector>
 
 #include "../../include/sinfvm_hip.h"
@@ -273,11 +275,9 @@ __device__ inline double Bc4(const KParams& P, const double* B, int i, int j) {
     return 0.25 * (((Bcorner(P, B, i, j) + Bcorner(P, B, i + 1, j)) + Bcorner(P, B, i, j + 1)) + Bcorner(P, B, i + 1, j + 1));
 }
 
-// cell k's face states in direction d, with duals seeded on cell `seed`'s
-// three variables (seed = -1: none) or on the two face bottoms (seedB).
+// Cell (i,j)'s face states in direction d (linearreconstruction.jl:73-100).
 template <class T>
-__device__ void recon_at(const KParams& P, const double* U, const double* B, int d, int i, int j, int seed_off,
-                         bool seedB, T L[3], T R[3]) {
+__device__ void recon_at(const KParams& P, const double* U, const double* B, int d, int i, int j, T L[3], T R[3]) {
     const size_t n = (size_t)P.nxT * P.nyT;
     const int di = d == 0 ? 1 : 0, dj = d == 0 ? 0 : 1;
     T cl[3], cc[3], cr[3];
@@ -286,16 +286,7 @@ __device__ void recon_at(const KParams& P, const double* U, const double* B, int
         cc[v] = T(U[v * n + cid(P, i, j)]);
         cr[v] = T(U[v * n + cid(P, i + di, j + dj)]);
     }
-    if constexpr (std::is_same<T, Dual<3>>::value) {
-        if (seed_off >= -1 && seed_off <= 1) {
-            T* tgt = seed_off == -1 ? cl : (seed_off == 0 ? cc : cr);
-            for (int v = 0; v < 3; ++v) tgt[v].d[v] = 1.0;
-        }
-    }
-    T Bl(B_face_left(P, B, i, j, d)), Br(B_face_right(P, B, i, j, d));
-    if constexpr (std::is_same<T, Dual<2>>::value) {
-        if (seedB) { Bl.d[0] = 1.0; Br.d[1] = 1.0; }
-    }
+    const T Bl(B_face_left(P, B, i, j, d)), Br(B_face_right(P, B, i, j, d));
     ad::reconstruct(cl, cc, cr, P.theta, Bl, Br, L, R);
 }
 
@@ -573,8 +564,8 @@ __global__ void __launch_bounds__(256) k_adj_faces(KParams P, int d, const doubl
     double lF[3];
     for (int v = 0; v < 3; ++v) lF[v] = inv * ((c1_in ? LR[v * n + k1] : 0.0) - (c_in ? LR[v * n + kc] : 0.0));
     double Lc[3], Rc[3], L1[3], R1[3];
-    recon_at<double>(P, U, B, d, i, j, -2, false, Lc, Rc);
-    recon_at<double>(P, U, B, d, i1, j1, -2, false, L1, R1);
+    recon_at<double>(P, U, B, d, i, j, Lc, Rc);
+    recon_at<double>(P, U, B, d, i1, j1, L1, R1);
     double gm[3], gq[3], speed;
     ad::face_vjp(d, Rc, L1, P.phys, lF, gm, gq, speed);
     for (int a = 0; a < 3; ++a) {
@@ -709,9 +700,9 @@ __device__ inline void face_speeds(const KParams& P, const double* U, const doub
                                    double& sl) {
     const int di = d == 0 ? 1 : 0, dj = d == 0 ? 0 : 1;
     double Lm[3], Rm[3], Ll[3], Rl[3], Lr[3], Rr[3], F[3];
-    recon_at<double>(P, U, B, d, i, j, -2, false, Lm, Rm);
-    recon_at<double>(P, U, B, d, i - di, j - dj, -2, false, Ll, Rl);
-    recon_at<double>(P, U, B, d, i + di, j + dj, -2, false, Lr, Rr);
+    recon_at<double>(P, U, B, d, i, j, Lm, Rm);
+    recon_at<double>(P, U, B, d, i - di, j - dj, Ll, Rl);
+    recon_at<double>(P, U, B, d, i + di, j + dj, Lr, Rr);
     sr = ad::central_upwind<double>(d, Rm, Lr, P.phys, F);
     sl = ad::central_upwind<double>(d, Rl, Lm, P.phys, F);
 }
@@ -769,8 +760,8 @@ __global__ void __launch_bounds__(256) k_adj_dtstep(KParams P, const StepCtl* ct
     const double lspeed = ldt * P.cfl * (-Dd / (ax * ax));
     const size_t n = (size_t)P.nxT * P.nyT;
     double La[3], Ra[3], Lb[3], Rb[3];
-    recon_at<double>(P, U, B, d, ia, ja, -2, false, La, Ra);
-    recon_at<double>(P, U, B, d, ib, jb, -2, false, Lb, Rb);
+    recon_at<double>(P, U, B, d, ia, ja, La, Ra);
+    recon_at<double>(P, U, B, d, ib, jb, Lb, Rb);
     using D6 = Dual<6>;
     D6 m[3], q[3], F[3];
     for (int v = 0; v < 3; ++v) { m[v] = D6::var(Ra[v], v); q[v] = D6::var(Lb[v], 3 + v); }
