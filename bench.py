@@ -505,7 +505,7 @@ def variants_bench(size: int, seed: int, steps: int, check: bool) -> dict:
     el = time.perf_counter() - t0
     n1 = n1.value
     row = {"value": 1024 * n1 / el, "unit": "cell-updates/s", "steps": int(n1), "seconds": el,
-           "ms_per_step": el / max(n1, 1) * 1e3, "t_end": t1.value}
+           "ms_per_step": el / max(n1, 1) * 1e3, "t_end": t1.value, "kernels": sim.kernels}
     if check:
         g2, s2 = VF.embed_1d(s1, g1)
         cfg = VF.build_config(s2, VF.RungeKutta2(), g2, 0.25, C.SFV_F64)
