@@ -164,12 +164,13 @@ __device__ __forceinline__ void side_vjp(int d, const double s[3], const PhysPar
 }
 
 // Reverse sweep of the central-upwind flux (centralupwind.jl:48-77) at one
-// face: given the flux adjoint lF, the adjoints of the minus side m and the
-// plus side q (and the face's speed).  The values and every branch are those of central_upwind on
+// face: given the flux adjoint lF (and the adjoint l_speed of the face's
+// speed), the adjoints of the minus side m and the plus side q, and the
+// face's speed.  The values and every branch are those of central_upwind on
 // doubles; the combination of the two sides and each side's flux are
 // differentiated by hand (side_vjp).
 __device__ __forceinline__ void face_vjp(int d, const double m[3], const double q[3], const PhysParams& p, const double lF[3],
-                         double lm[3], double lq[3], double& speed) {
+                         double lm[3], double lq[3], double& speed, double l_speed = 0.0) {
     double fm[3], em[2], fp[3], ep[2];
     const bool wm = side_values(d, m, p, fm, em);
     const bool wq = side_values(d, q, p, fp, ep);
@@ -189,6 +190,13 @@ __device__ __forceinline__ void face_vjp(int d, const double m[3], const double 
     if (__builtin_fabs(D) < p.kappa) return;       // the flux is the constant zero
     const double pm = (aplus * aminus) / D;
     double l_ap = 0.0, l_am = 0.0, l_D = 0.0, l_pm = 0.0, l_fm[3], l_fp[3];
+    // the speed's own adjoint: speed = max(|a+|, |a-|) unless both sides are dry
+    if (l_speed != 0.0 && !(m[0] < p.cutoff && q[0] < p.cutoff)) {
+        if (jmax_pick(__builtin_fabs(aplus), __builtin_fabs(aminus)))
+            l_am += l_speed * (__builtin_signbit(aminus) ? -1.0 : 1.0);
+        else
+            l_ap += l_speed * (__builtin_signbit(aplus) ? -1.0 : 1.0);
+    }
     for (int v = 0; v < 3; ++v) {
         const double N = aplus * fm[v] - aminus * fp[v];
         const double lN = lF[v] / D;
@@ -762,15 +770,14 @@ __global__ void __launch_bounds__(256) k_adj_dtstep(KParams P, const StepCtl* ct
     double La[3], Ra[3], Lb[3], Rb[3];
     recon_at<double>(P, U, B, d, ia, ja, La, Ra);
     recon_at<double>(P, U, B, d, ib, jb, Lb, Rb);
-    using D6 = Dual<6>;
-    D6 m[3], q[3], F[3];
-    for (int v = 0; v < 3; ++v) { m[v] = D6::var(Ra[v], v); q[v] = D6::var(Lb[v], 3 + v); }
-    const D6 sp = ad::central_upwind<D6>(d, m, q, P.phys, F);
+    const double lF0[3] = {0.0, 0.0, 0.0};
+    double gm[3], gq[3], sp;
+    ad::face_vjp(d, Ra, Lb, P.phys, lF0, gm, gq, sp, lspeed);
     double* LfL = d == 0 ? LfL0 : LfL1;
     double* LfR = d == 0 ? LfR0 : LfR1;
     for (int c = 0; c < 3; ++c) {
-        LfR[c * n + cid(P, ia, ja)] += lspeed * sp.d[c];
-        LfL[c * n + cid(P, ib, jb)] += lspeed * sp.d[3 + c];
+        LfR[c * n + cid(P, ia, ja)] += gm[c];
+        LfL[c * n + cid(P, ib, jb)] += gq[c];
     }
 }
 
