@@ -26,8 +26,11 @@
 //   rain / infiltration       test_rain.jl:56-73 KATs and 1.03 closed form,
 //                             test_infiltration.jl:58-72
 //   steppers                  convergence order 1 / 2 (test_timesteppers.jl:76-100)
-//   friction bh2021           closed form of one implicit step on a uniform flow
-//   friction bsa2012/fcg2016  parity unpinned at ulp level (cbrt of the host libm)
+//   friction (all 3 laws)     closed forms of the implicit update on a uniform periodic
+//                             flow: FE and RK2 steps (stage input), CFL dt, speed 0, one
+//                             flow direction, desingularised thin layer, 1D, to 1e-14
+//                             relative (tests/test_oracle_friction.py); the cbrt of
+//                             bsa2012/fcg2016 stays unpinned at ulp level (host libm)
 //   Horton exp                parity unpinned at ulp level (exp of the host libm),
 //                             f(0)=f0 and f(1e6)=fc pinned (test_infiltration.jl:60-61)
 //   Neumann BC                restates the reference's right-side index map as written
