@@ -32,7 +32,8 @@ import numpy as np  # noqa: E402
 
 import volumefluxes as VF  # noqa: E402
 from volumefluxes import _capi as C  # noqa: E402
-from volumefluxes.scenarios import terrain_initial_state, terrain_topography  # noqa: E402
+from volumefluxes.scenarios import (serving_requests, terrain_initial_state, terrain_topography,  # noqa: E402
+                                   terrain_workload)
 
 HBM_PEAK_GBS = 8000.0                 # MI355X HBM3E, 8 TB/s
 BYTES_PER_CELL_STEP = {"f64": 136, "f32": 80}   # compulsory HBM bytes per RK2 cell-update (B stays f64)
@@ -41,16 +42,7 @@ METRIC = "cell-updates/s + HBM GB/s % peak (2D SWE RK2 FP64); ensemble p50/p99 v
 
 
 def build_scenario(n: int, precision: str, seed: int):
-    grid = VF.CartesianGrid(n, n, gc=2, boundary=VF.WallBC(), extent=[[0.0, float(n)], [0.0, float(n)]])
-    B = terrain_topography(grid, seed=seed, n_bumps=32, slope=1e-3, amp=(0.5, 5.0), width=(20.0, 200.0))
-    eq = VF.ShallowWaterEquations(VF.BottomTopography2D(B, None, grid), desingularizing_kappa=1e-4)
-    manning_n = 0.03
-    system = VF.ConservedSystem(None, VF.LinearReconstruction(), VF.CentralUpwind(eq), eq, grid,
-                                [VF.SourceTermBottom(), VF.ConstantRain(1e-5 * 3600)],
-                                VF.ImplicitFriction(Cz=manning_n * manning_n))
-    realtype = np.float64 if precision == "f64" else np.float32
-    U0 = terrain_initial_state(grid, B, 2.0)
-    return grid, system, realtype, U0, B
+    return terrain_workload(n, precision, seed)
 
 
 def read_traffic(kernel: str, workload: str):
@@ -197,36 +189,7 @@ def ensemble_bench(members: int, steps: int, seed: int):
 
 
 def serving_scenarios(rate: float, duration: float, t_range, seed: int):
-    """Serving requests: Poisson arrivals (rate per s over `duration` s); each
-    a square grid of side log-uniform in [64, 2048] (dx = 1 m), half dam
-    breaks (2 m | 1 m, flat bottom) and half terrains (seeded bumps as in the
-    single-grid workload), Manning n = 0.03, f64, RK2, t_end U[t_range]."""
-    rng = np.random.default_rng(seed)
-    reqs, t = [], 0.0
-    while True:
-        t += float(rng.exponential(1.0 / rate))
-        if t > duration:
-            break
-        side = int(round(float(np.exp(rng.uniform(np.log(64), np.log(2048))))))
-        grid = VF.CartesianGrid(side, side, gc=2, boundary=VF.WallBC(), extent=[[0.0, float(side)], [0.0, float(side)]])
-        friction = VF.ImplicitFriction(Cz=0.03 * 0.03)
-        if rng.uniform() < 0.5:
-            eq = VF.ShallowWaterEquations(desingularizing_kappa=1e-4)
-            system = VF.ConservedSystem(None, VF.LinearReconstruction(), VF.CentralUpwind(eq), eq, grid, [], friction)
-            from volumefluxes.scenarios import dam_break_initial_state
-            U0 = dam_break_initial_state(grid, side / 2.0, 2.0, 1.0)
-            kind = "dam_break"
-        else:
-            B = terrain_topography(grid, seed=int(rng.integers(1 << 30)), n_bumps=32, slope=1e-3, amp=(0.5, 5.0),
-                                   width=(20.0, 200.0))
-            eq = VF.ShallowWaterEquations(VF.BottomTopography2D(B, None, grid), desingularizing_kappa=1e-4)
-            system = VF.ConservedSystem(None, VF.LinearReconstruction(), VF.CentralUpwind(eq), eq, grid,
-                                        [VF.SourceTermBottom(), VF.ConstantRain(1e-5 * 3600)], friction)
-            U0 = terrain_initial_state(grid, B, 2.0)
-            kind = "terrain"
-        reqs.append({"arrival": t, "grid": grid, "system": system, "U0": U0, "kind": kind,
-                     "t_end": float(rng.uniform(*t_range))})
-    return reqs
+    return serving_requests(rate, duration, t_range, seed)
 
 
 def serving_bench(rate: float, duration: float, t_range, budget_gb: float, seed: int):

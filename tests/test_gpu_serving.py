@@ -179,3 +179,78 @@ def test_destroying_a_busy_server_fails_its_requests_cleanly():
         U, _ = sv2.fetch(rid)
     assert np.array_equal(U, _interior(_alone(sc, 0.2)))
     assert len(ids) == 3
+
+
+@pytest.mark.timeout(600)
+def test_serving_workload_mix_against_the_oracle():
+    """The serving workload's request mix on a short horizon: Poisson arrivals
+    at 50 per second (seeded), square grids from 64 to 1100 cells a side at
+    dx = 1 m (half dam breaks 2 m | 1 m, half seeded terrains with rain and
+    friction), t_end 10-15 s (the full workload draws U[10, 600] s).  The
+    HBM budget holds the 1024^2 terrain or the 1100^2 dam break but not both,
+    so the second waits in the queue until the first retires, and the requests
+    behind it wait too (admission is in arrival order).  Three results (the
+    1024^2 terrain, an 80^2 dam break and a 90^2 terrain) are compared bitwise
+    with the oracle's simulate_to_time (simulator.jl:179-212) and the others
+    with the same request run alone on the HIP backend."""
+    import ctypes
+    import threading
+
+    import oracle as O
+    from volumefluxes import _capi as C
+    from volumefluxes.scenarios import serving_request
+
+    spec = [(1024, "terrain", 11, 10.0, True), (1100, "dam_break", 0, 10.0, False),
+            (80, "dam_break", 0, 10.0, True), (90, "terrain", 12, 10.0, True),
+            (200, "terrain", 13, 12.0, False), (300, "dam_break", 0, 15.0, False),
+            (64, "terrain", 5, 10.0, False), (512, "dam_break", 0, 11.0, False)]
+    reqs = [serving_request(side, kind, seed) + (te, ora) for side, kind, seed, te, ora in spec]
+    rng = np.random.default_rng(50)
+    arrivals = np.cumsum(rng.exponential(1.0 / 50.0, len(reqs)))
+    sizes = [C.lib().sfv_device_bytes_for(ctypes.byref(VF.build_config(s, VF.RungeKutta2(), g, 0.25, 0)))
+             for g, s, _, _, _ in reqs]
+    budget = int(1.2 * max(sizes[0], sizes[1])) + sum(sizes[2:])
+    assert sizes[0] + sizes[1] > budget            # the two large grids cannot run together
+
+    ids = [None] * len(reqs)
+    with VF.Server(budget) as sv:
+        def submitter():
+            t_start = time.perf_counter()
+            for k, (g, s, U0, te, _) in enumerate(reqs):
+                delay = arrivals[k] - (time.perf_counter() - t_start)
+                if delay > 0:
+                    time.sleep(delay)
+                ids[k] = sv.submit(s, g, U0, te)
+        th = threading.Thread(target=submitter)
+        th.start()
+        th.join()
+        outs = []
+        for k, rid in enumerate(ids):
+            info = sv.wait(rid, timeout=300)
+            assert info["status"] == 2, (k, info)
+            outs.append(sv.fetch(rid))
+        st = sv.stats()
+    print(f"[serving mix] budget {budget / 1e6:.1f} MB, peak {st['bytes_peak'] / 1e6:.1f} MB, "
+          f"replays {st['replays']}", flush=True)
+    assert st["completed"] == len(reqs) and st["failed"] == 0
+    assert st["bytes_peak"] <= budget
+    info0 = outs[0][1]
+    for k in range(1, len(reqs)):
+        assert outs[k][1]["admit_s"] >= info0["done_s"], k        # queued behind the budget
+    for k, ((g, s, U0, te, ora), (U, info)) in enumerate(zip(reqs, outs)):
+        assert info["t"] == te, k
+        if ora:
+            cfg = VF.build_config(s, VF.RungeKutta2(), g, 0.25, 0)
+            o = O.Oracle2D(cfg, np.asfortranarray(s.equation.B.corners(g), dtype=np.float64))
+            o.set_interior(U0)
+            n = o.simulate_to_time(te)
+            print(f"[serving mix] request {k} ({g.nx}^2): {info['steps']} steps served, {n} on the oracle",
+                  flush=True)
+            assert info["steps"] == n and o.t == info["t"], k
+            assert np.array_equal(U, o.state()[2:-2, 2:-2, :]), k
+        else:
+            sim = VF.Simulator(VF.HIPBackend(np.float64), s, VF.RungeKutta2(), g, cfl=0.25)
+            VF.set_current_state(sim, U0)
+            VF.simulate_to_time(sim, te)
+            assert sim.t == info["t"], k
+            assert np.array_equal(U, _interior(sim)), k
