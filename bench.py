@@ -20,6 +20,7 @@ import argparse
 import json
 import os
 import statistics
+import subprocess
 import sys
 import time
 
@@ -58,6 +59,70 @@ def read_traffic(kernel: str, workload: str):
             if "bytes_per_launch" in kv:
                 return float(kv["bytes_per_launch"])
     return None
+
+
+VALU_WAVE_INSTR_PER_S = 256 * 2.4e9   # MI355X: 256 CUs, each issuing one 64-lane VALU instruction per clock
+
+
+def read_valu(kernel: str, workload: str):
+    """SQ_INSTS_VALU (64-lane wave-instructions per launch) of `kernel` from the
+    committed rocprofv3 PMC summary, or None."""
+    path = os.path.join(ROOT, "profiles", f"valu_{workload}.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        parts = line.split()
+        if parts and parts[0] == kernel:
+            kv = dict(p.split("=") for p in parts[1:] if "=" in p)
+            if "SQ_INSTS_VALU" in kv and "dispatches" in kv:
+                return float(kv["SQ_INSTS_VALU"])
+    return None
+
+
+def host_cpu():
+    """(logical CPUs this process may use, CPU model name) of this host."""
+    model = None
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except (OSError, subprocess.SubprocessError):
+        pass
+    if model is None and os.path.exists("/proc/cpuinfo"):
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    return len(os.sched_getaffinity(0)), model
+
+
+def cpu_stoker_1d(threads: int):
+    """The 1D CPU workload on the oracle: Stoker dam break, 1024 cells on
+    [0, 1000] m, flat bottom, h = 2 m | 1 m, wall BCs, RK2, CFL 0.25, to t = 20 s
+    with match_endtime.  Median of 3 whole runs."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O  # the reference restatement: CPU baseline only
+    O.set_threads(threads)
+    g1 = VF.CartesianGrid(1024, gc=2, boundary=VF.WallBC(), extent=[[0.0, 1000.0]])
+    eq1 = VF.ShallowWaterEquations1D(VF.ConstantBottomTopography())
+    s1 = VF.ConservedSystem(None, VF.LinearReconstruction(), VF.CentralUpwind(eq1), eq1, g1, [], None)
+    g2, s2 = VF.embed_1d(s1, g1)
+    cfg = VF.build_config(s2, VF.RungeKutta2(), g2, 0.25, C.SFV_F64)
+    cfg.ny, cfg.dy = 1, cfg.dx
+    U1 = np.zeros((1024, 2))
+    U1[:, 0] = np.where(g1.cell_centers() < 500.0, 2.0, 1.0)
+    times, steps = [], 0
+    for _ in range(3):
+        o = O.Oracle1D(cfg, np.ascontiguousarray(s1.equation.B.corners(g1), dtype=np.float64))
+        o.set_interior(U1)
+        t0 = time.perf_counter()
+        steps = o.simulate_to_time(20.0)
+        times.append(time.perf_counter() - t0)
+    med = statistics.median(times)
+    return {"value": 1024 * steps / med, "unit": "cell-updates/s", "steps": int(steps), "seconds": med,
+            "threads": threads, "workload": "1D dam break, 1024 cells on [0, 1000] m, RK2, CFL 0.25, to t = 20 s"}
 
 
 def cpu_baseline(grid, system, realtype, U0, B, steps_per_run: int, runs: int, threads: int):
@@ -589,6 +654,14 @@ def main():
     traffic = read_traffic(dominant, workload)
     step_gbs = value * BYTES_PER_CELL_STEP[args.precision] / 1e9
     first_gpu, finite, copy_gbs = leg["first_gpu"], leg["finite"], leg["copy_gbs"]
+    # the FP64 VALU side: committed SQ_INSTS_VALU per launch against this run's kernel times
+    valu = {k: read_valu(k, workload) for k in ("stage1", "stage2")}
+    valu_frac = lane_cell = lane_update = None
+    if valu[dominant] is not None:
+        valu_frac = (valu[dominant] / VALU_WAVE_INSTR_PER_S) / (ms[dominant] * 1e-3)
+        lane_cell = valu[dominant] * 64 / cells
+    if None not in valu.values():
+        lane_update = (valu["stage1"] + valu["stage2"]) * 64 / cells
 
     result = {
         "metric": METRIC,
@@ -614,6 +687,10 @@ def main():
         },
         "roofline": {
             "bound": "hbm",
+            "binding": "valu_f64" if valu_frac is not None and valu_frac > achieved / HBM_PEAK_GBS else "hbm",
+            "valu_issue_frac": valu_frac,
+            "lane_instr_per_cell": lane_cell,
+            "lane_instr_per_cell_update": lane_update,
             "kernel": dominant,
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
@@ -638,12 +715,15 @@ def main():
         first_cpu, cpu_value, cpu_med = cpu_baseline(grid, system, realtype, U0, B, args.cpu_steps, args.cpu_runs,
                                                      threads)
         _, cpu1_value, _ = cpu_baseline(grid, system, realtype, U0, B, 2, 1, 1)
+        nproc, cpu_model = host_cpu()
         result["cpu_baseline"] = {
             "value": cpu_value, "unit": "cell-updates/s", "cores": threads, "kind": "port",
+            "nproc": nproc, "cpu_model": cpu_model,
             "sample": (f"C++ oracle (loop-faithful restatement of the reference CPU path), same workload: "
                        f"median of {args.cpu_runs} runs of {args.cpu_steps} RK2 steps after 1 warm-up step, "
                        f"OpenMP over rows"),
             "single_thread_value": cpu1_value,
+            "stoker_1d": cpu_stoker_1d(threads),
         }
         result["parity_first_step"] = "bitwise" if np.array_equal(first_gpu, first_cpu) else "MISMATCH"
         result["gpu_over_cpu"] = value / cpu_value

@@ -50,3 +50,57 @@ def test_shim_symbols_are_exported():
     L = C.lib()
     missing = sorted({name for name, _ in shim_ccalls() if not hasattr(L, name)})
     assert not missing, missing
+
+
+def _code(text):
+    """The shim without comments and docstrings."""
+    text = re.sub(r'"""[\s\S]*?"""', "", text)
+    text = re.sub(r'"[^"\n]*"', '""', text)
+    return "\n".join(line.split("#", 1)[0] for line in text.splitlines())
+
+
+def test_shim_binds_the_reference_module():
+    # The reference's module is VolumeFluxes (src/SinFVM.jl:21); every reference
+    # test does `using VolumeFluxes`.
+    code = _code(open(SHIM).read())
+    assert re.search(r"^\s*using VolumeFluxes\s*$", code, flags=re.M)
+    assert "SinFVM" not in code
+    assert re.search(r"<:\s*VolumeFluxes\.Backend", code)
+    for name in ("perform_step!", "simulate_to_time", "current_state", "set_current_state!"):
+        assert re.search(rf"function VolumeFluxes\.{re.escape(name)}\(|VolumeFluxes\.{re.escape(name)}\(sim::", code), name
+
+
+def test_shim_keys_handles_by_identity():
+    # sim.t is an MVector mutated every step (simulator.jl:205): a table that
+    # hashes it by content loses the handle after the first step and merges
+    # simulators at the same t.  The registry must key on identity.
+    code = _code(open(SHIM).read())
+    assert "WeakKeyDict" not in code
+    decl = re.search(r"const HANDLES\s*=\s*(.+)", code).group(1)
+    assert "IdDict" in decl or "UInt" in decl, decl
+    stores = re.findall(r"HANDLES\[(.+?)\]\s*=", code)
+    assert stores and all("objectid" in k for k in stores), stores
+    assert re.search(r"get\(HANDLES,\s*objectid\(", code)
+    assert re.search(r"finalizer\(.*release!", code)      # the handle dies with sim.t
+
+
+def test_shim_runs_1d_systems_through_the_embedding():
+    code = _code(open(SHIM).read())
+    assert "ShallowWaterEquations1D" in code
+    assert re.search(r"c\.embedded_1d\s*=\s*one_d\s*\?\s*1\s*:\s*0", code)
+    assert "2.0^60" in code                        # dy of the embedding (volumefluxes.model.embed_1d)
+    from volumefluxes.model import EMBED_ROWS
+    assert re.search(rf"const EMBED_ROWS\s*=\s*{EMBED_ROWS}\b", code)
+
+
+def test_shim_config_struct_matches_the_c_layout():
+    # field names and order of `mutable struct SfvConfig` against ctypes' sfv_config
+    text = open(SHIM).read()
+    body = re.search(r"mutable struct SfvConfig\n(.*?)\n\s*SfvConfig\(\) = new\(\)", text, flags=re.S).group(1)
+    names = []
+    for line in body.splitlines():
+        for part in line.split(";"):
+            m = re.match(r"\s*(\w+)::", part)
+            if m:
+                names.append(m.group(1))
+    assert names == [f[0] for f in C.SfvConfig._fields_]

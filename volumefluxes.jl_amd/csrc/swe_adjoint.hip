@@ -833,8 +833,13 @@ namespace {
 
 using namespace sfv;
 
+// Every buffer starts at zero: the clear is queued on the handle's stream,
+// ahead of every kernel of the call (the side stream's work waits for an
+// event recorded on that stream after the forward pass).
 struct AdjBuffers {
+    hipStream_t stream;
     std::vector<void*> owned;
+    explicit AdjBuffers(hipStream_t s) : stream(s) {}
     double* alloc(size_t count, std::string& why, int& rc) {
         void* p = nullptr;
         if (rc) return nullptr;
@@ -844,7 +849,11 @@ struct AdjBuffers {
             return nullptr;
         }
         owned.push_back(p);
-        (void)hipMemset(p, 0, count * sizeof(double));
+        if (hipMemsetAsync(p, 0, count * sizeof(double), stream) != hipSuccess) {
+            rc = SFV_ERR_HIP;
+            why = "clearing the adjoint buffers failed";
+            return nullptr;
+        }
         return (double*)p;
     }
     ~AdjBuffers() {
@@ -934,7 +943,7 @@ extern "C" int sfv_adjoint(sfv_handle* h, int64_t nsteps, int32_t checkpoint_eve
     const int64_t nseg = (nsteps + ce - 1) / ce;
     std::string why;
     int rc = SFV_OK;
-    AdjBuffers buf;
+    AdjBuffers buf(st);
     double* ck = buf.alloc(n3 * (size_t)nseg, why, rc);         // checkpoint states
     double* seg = buf.alloc(n3 * (size_t)(ce + 1), why, rc);    // the states of one segment
     double* U1 = buf.alloc(n3, why, rc);
