@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -184,6 +185,12 @@ struct sfv_handle {
         ghost_maps_into(c.bc, c.gc, p.nxT, p.src_lo_x, p.src_hi_x, p.flip_lo_x, p.flip_hi_x);
         ghost_maps_into(c.bc, c.gc, p.nyT, p.src_lo_y, p.src_hi_y, p.flip_lo_y, p.flip_hi_y);
         p.rows = march_rows;
+        // central_upwind_rng's parameter conditions; SFV_RANGED=0 turns it off
+        auto in = [](double v, double lo, double hi) { return v >= lo && v <= hi; };
+        const char* rng_env = std::getenv("SFV_RANGED");
+        p.ranged = !(rng_env && rng_env[0] == '0') && (p.rec != 1 || in(c.theta, 0x1p-10, 0x1p10)) &&
+                   in(c.rho, 0x1p-10, 0x1p10) && in(c.g, 0x1p-10, 0x1p10) && in(c.depth_cutoff, 0x1p-39, 0x1p10) &&
+                   in(c.desingularizing_kappa, 0x1p-39, 0x1p10);
     }
 
     static void ghost_maps_into(int bc, int gc, int N, int* lo, int* hi, int* flo, int* fhi);

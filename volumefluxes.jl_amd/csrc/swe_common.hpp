@@ -34,6 +34,7 @@ struct KParams {
     const double* factor;      // device (nxT*nyT) Horton factor field
     const double* rain_field;  // device (nxT*nyT) FunctionalRain rates of the current step
     int bcell_1d;   // B_cell = 0.5*(B[i,j] + B[i+1,j]) (a CartesianGrid{1} run embedded in (nx, 3))
+    int ranged;     // θ, ρ, g, depth_cutoff and κ allow the ranged face evaluation (central_upwind_rng)
     int bc;
     // Ghost-cell source maps for the boundary condition, by side: low-side
     // ghost g is cell g, high-side ghost g is cell N-gc+g (N = nxT or nyT);

@@ -79,7 +79,7 @@ namespace sfv {
 
 template <class F, size_t... I>
 __device__ __forceinline__ void unrolled_rows(F& body, int r, std::index_sequence<I...>) {
-    (body(r + (int)I, std::true_type{}), ...);
+    (body(r + (int)I, std::true_type{}, std::true_type{}), ...);
 }
 
 constexpr int MW = 64;                // lanes
@@ -302,13 +302,15 @@ __device__ __forceinline__ void march_tile(const KParams& P, StepCtl* ctl, int s
     };
 
     // corner window: rows jr-1 (bm1), jr (b0), jr+1 (bp1); n* = column i+1
-    double bm1 = 0.0, nm1 = 0.0, b0, n0, bp1, np1, e_;
-    fetch_b(j0 - 2, b0, e_);
-    n0 = next_col(b0, e_);
-    fetch_b(j0 - 1, bp1, e_);
-    np1 = next_col(bp1, e_);
+    double bm1 = 0.0, nm1 = 0.0, b0, n0, bp1, np1, e0_, e1_;
+    fetch_b(j0 - 2, b0, e0_);
+    n0 = next_col(b0, e0_);
+    fetch_b(j0 - 1, bp1, e1_);
+    np1 = next_col(bp1, e1_);
     Raw cur, nxt;
     if (SFV_MARCH_PREFETCH) fetch(j0 - 2, cur);
+    // ranged face evaluation (central_upwind_rng) while the wave's inputs allow it
+    bool band = P.ranged && __all(in_band(b0) && in_band(e0_) && in_band(bp1) && in_band(e1_));
 
     S cA[3] = {S(0), S(0), S(0)}, cB[3] = {S(0), S(0), S(0)};
     // stage 2 with SFV_MARCH_KEEPU0: U0 of rows jr-2, jr-1 as loaded, for the
@@ -324,9 +326,17 @@ __device__ __forceinline__ void march_tile(const KParams& P, StepCtl* ctl, int s
     // x faces this lane evaluates (its left face) that border an output cell
     const bool face_x_ok = in_dom && lane >= 2 && lane <= MX + 2 && (lane == 2 ? i < nxT - gc : i - 1 < nxT - gc);
 
-    auto body = [&](int r, auto full_tag) {
+    // Work on row q = jr-1 is done only where it feeds an output row: its x
+    // part (x reconstruction, x faces, x flux difference and bottom term) when
+    // q is an output row, j0 <= q < j0+th, i.e. 3 <= r <= th+2; the y face
+    // below it when it borders one, j0-1 < q <= j0+th, i.e. r >= 3.  The row
+    // loop is r = 0..th+3: the first four iterations and the last are peeled,
+    // so the loop proper (4 <= r <= th+2) does everything in one basic block.
+    auto body = [&](int r, auto full_tag, auto x_tag) {
         constexpr bool FULL = decltype(full_tag)::value;   // r >= 4: every step applies
         const int jr = j0 - 2 + r;
+        const bool xrow = FULL ? decltype(x_tag)::value : r == 3;
+        const bool yface = FULL || r >= 3;
         double nb, ne;
         fetch_b(jr + 2, nb, ne);                   // next corner row (Bload bounds-checks row nyT+1)
         if (SFV_MARCH_PREFETCH) {
@@ -347,6 +357,11 @@ __device__ __forceinline__ void march_tile(const KParams& P, StepCtl* ctl, int s
         } else {
             cC[0] = cC[1] = cC[2] = S(0);
         }
+        // the ranged face forms apply while every stage input and bottom
+        // corner this wave has taken in lies in band (central_upwind_rng);
+        // the faces of this iteration read rows up to jr and corner rows up
+        // to jr, the corner row jr+2 taken in here is read two rows later
+        band = band && __all(in_band(cC[0]) && in_band(cC[1]) && in_band(cC[2]) && in_band(nb) && in_band(ne));
 
         // 1. y reconstruction of row m = jr-1 (rows jr-2, jr-1, jr) and x
         //    reconstruction of row q = jr-1 (this lane's cell cB and its x
@@ -356,58 +371,100 @@ __device__ __forceinline__ void march_tile(const KParams& P, StepCtl* ctl, int s
         const double bcq = bc4(bm1, nm1, b0, n0);                 // B_cell of row jr-1
         S Lx[3] = {S(0), S(0), S(0)}, Rx[3] = {S(0), S(0), S(0)}, Rxl[3] = {S(0), S(0), S(0)};
         const double Blx = 0.5 * (bm1 + b0), Brx = 0.5 * (nm1 + n0);
-        if (FULL || r >= 1) {
-            S cl[3], cr[3];
-#pragma unroll
-            for (int v = 0; v < 3; ++v) { cl[v] = shfl_up1(cB[v]); cr[v] = shfl_dn1(cB[v]); }
+        if (FULL || r >= 2) {
             if (rec_linear) {
 #if SFV_MARCH_BRANCHFREE
-                if (FULL || r >= 2) reconstruct_cell_nb(cA, cB, cC, P.theta, Bly, Bry, Ly, Ry);
+                reconstruct_cell_nb(cA, cB, cC, P.theta, Bly, Bry, Ly, Ry);
+#else
+                reconstruct_cell(cA, cB, cC, P.theta, Bly, Bry, Ly, Ry, FAST);
+#endif
+            } else {
+#pragma unroll
+                for (int v = 0; v < 3; ++v) { Ly[v] = Ry[v] = cB[v]; }
+                Ly[0] = Ry[0] = (S)(cB[0] - bcq);
+            }
+        }
+        if (xrow) {
+            if (rec_linear) {
+                S cl[3], cr[3];
+#pragma unroll
+                for (int v = 0; v < 3; ++v) { cl[v] = shfl_up1(cB[v]); cr[v] = shfl_dn1(cB[v]); }
+#if SFV_MARCH_BRANCHFREE
                 reconstruct_cell_nb(cl, cB, cr, P.theta, Blx, Brx, Lx, Rx);
 #else
-                if (FULL || r >= 2) reconstruct_cell(cA, cB, cC, P.theta, Bly, Bry, Ly, Ry, FAST);
                 reconstruct_cell(cl, cB, cr, P.theta, Blx, Brx, Lx, Rx, FAST);
 #endif
             } else {
 #pragma unroll
-                for (int v = 0; v < 3; ++v) { Ly[v] = Ry[v] = Lx[v] = Rx[v] = cB[v]; }
-                Ly[0] = Ry[0] = Lx[0] = Rx[0] = (S)(cB[0] - bcq);
+                for (int v = 0; v < 3; ++v) { Lx[v] = Rx[v] = cB[v]; }
+                Lx[0] = Rx[0] = (S)(cB[0] - bcq);
             }
 #pragma unroll
             for (int v = 0; v < 3; ++v) Rxl[v] = shfl_up1(Rx[v]);
         }
-        // 2. both faces: y face between rows jr-2 and jr-1, and this lane's
-        //    left x face in row jr-1 (independent chains, one basic block)
+        // 2. the faces: the y face between rows jr-2 and jr-1, and this lane's
+        //    left x face in row jr-1 (independent chains, one basic block when
+        //    both apply)
         double Fy[3] = {0.0, 0.0, 0.0}, Fxm[3] = {0.0, 0.0, 0.0};
-        if (FULL || r >= 1) {
-#if SFV_MARCH_BRANCHFREE
-            const S* hp[4] = {&Ryp[0], &Ly[0], &Rxl[0], &Lx[0]};
-            double hs[4];
-            desingularize_n<S, 4>(hp, P.phys.kappa, hs);
-            const double spy = central_upwind_nb<1>(Ryp, Ly, hs[0], hs[1], P.phys, Fy);
-            const double spx = central_upwind_nb<0>(Rxl, Lx, hs[2], hs[3], P.phys, Fxm);
-#elif SFV_MARCH_WETFAST
+        auto face_y = [&](auto rng_tag) {
+#if SFV_MARCH_WETFAST
             const double spy = central_upwind_auto<1>(Ryp, Ly, P.phys, Fy);
-            const double spx = central_upwind_auto<0>(Rxl, Lx, P.phys, Fxm);
 #else
-            const double spy = central_upwind(1, Ryp, Ly, P.phys, Fy);
-            const double spx = central_upwind(0, Rxl, Lx, P.phys, Fxm);
+            const double spy = decltype(rng_tag)::value ? central_upwind_rng(1, Ryp, Ly, P.phys, Fy)
+                                                        : central_upwind(1, Ryp, Ly, P.phys, Fy);
 #endif
             if (STAGE == 1) {
                 const int m = jr - 1;
-                if ((FULL || r >= 3) && out_col && m >= j0 && m <= j0 + th) {
+                if (out_col && m >= j0 && m <= j0 + th) {
                     const unsigned long long bits = (unsigned long long)__double_as_longlong((double)(S)spy);
                     smy = bits > smy ? bits : smy;
                 }
-                if (face_x_ok && m >= j0 && m < j0 + th) {
+            }
+        };
+        auto face_x = [&](auto rng_tag) {
+#if SFV_MARCH_WETFAST
+            const double spx = central_upwind_auto<0>(Rxl, Lx, P.phys, Fxm);
+#else
+            const double spx = decltype(rng_tag)::value ? central_upwind_rng(0, Rxl, Lx, P.phys, Fxm)
+                                                        : central_upwind(0, Rxl, Lx, P.phys, Fxm);
+#endif
+            if (STAGE == 1 && face_x_ok) {       // xrow: row jr-1 is an output row
+                const unsigned long long bits = (unsigned long long)__double_as_longlong((double)(S)spx);
+                smx = bits > smx ? bits : smx;
+            }
+        };
+#if SFV_MARCH_BRANCHFREE
+        if (yface || xrow) {
+            const S* hp[4] = {&Ryp[0], &Ly[0], &Rxl[0], &Lx[0]};
+            double hs[4];
+            desingularize_n<S, 4>(hp, P.phys.kappa, hs);
+            if (yface) {
+                const double spy = central_upwind_nb<1>(Ryp, Ly, hs[0], hs[1], P.phys, Fy);
+                if (STAGE == 1 && out_col && jr - 1 >= j0 && jr - 1 <= j0 + th) {
+                    const unsigned long long bits = (unsigned long long)__double_as_longlong((double)(S)spy);
+                    smy = bits > smy ? bits : smy;
+                }
+            }
+            if (xrow) {
+                const double spx = central_upwind_nb<0>(Rxl, Lx, hs[2], hs[3], P.phys, Fxm);
+                if (STAGE == 1 && face_x_ok) {
                     const unsigned long long bits = (unsigned long long)__double_as_longlong((double)(S)spx);
                     smx = bits > smx ? bits : smx;
                 }
             }
         }
+#else
+        if (band) {                                // every input of this row's faces is in band
+            if (yface) face_y(std::true_type{});
+            if (xrow) face_x(std::true_type{});
+        } else {
+            if (yface) face_y(std::false_type{});
+            if (xrow) face_x(std::false_type{});
+        }
+#endif
         // 3. x part of row jr-1 (finished next iteration)
         S accx_n[3] = {S(0), S(0), S(0)};
-        if (FULL || r >= 1) {
+        if (xrow) {
             double Fxp[3];
 #pragma unroll
             for (int v = 0; v < 3; ++v) Fxp[v] = shfl_dn1(Fxm[v]);
@@ -481,15 +538,20 @@ __device__ __forceinline__ void march_tile(const KParams& P, StepCtl* ctl, int s
         if (SFV_MARCH_PREFETCH) cur = nxt;
     };
 
-    const int rmain = min(4, th + 4);
-    for (int r = 0; r < rmain; ++r) body(r, std::false_type{});
+    // th >= 1, so there are th+4 >= 5 iterations: r = 0..3 peeled, the loop
+    // proper r = 4..th+2 (x part and y face), and r = th+3 (y face only).
+    body(0, std::false_type{}, std::false_type{});
+    body(1, std::false_type{}, std::false_type{});
+    body(2, std::false_type{}, std::false_type{});
+    body(3, std::false_type{}, std::false_type{});
     // The row loop is unrolled by hand (its cross-lane shifts are convergent
     // operations, which the compiler will not unroll with a runtime trip count).
-    int r = rmain;
+    int r = 4;
     if (SFV_MARCH_UNROLL > 1)
-        for (; r + SFV_MARCH_UNROLL <= th + 4; r += SFV_MARCH_UNROLL)
+        for (; r + SFV_MARCH_UNROLL <= th + 3; r += SFV_MARCH_UNROLL)
             unrolled_rows(body, r, std::make_index_sequence<SFV_MARCH_UNROLL>{});
-    for (; r < th + 4; ++r) body(r, std::true_type{});
+    for (; r < th + 3; ++r) body(r, std::true_type{}, std::true_type{});
+    body(th + 3, std::true_type{}, std::false_type{});
 
     if (STAGE == 1) block_max_to_slot_m(smx, smy, &ctl->slot[slot], par);
 

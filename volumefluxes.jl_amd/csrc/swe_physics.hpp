@@ -332,6 +332,129 @@ __device__ __forceinline__ double central_upwind_wet(const S m[3], const S q[3],
     return degenerate ? 0.0 : sp;
 }
 
+// ---------------------------------------------------------------------------
+// Face evaluation on operands in a proven range.
+//
+// The compiler's correctly rounded FP64 quotient x/d on gfx950 is
+//   ds = div_scale(d; d, x), xs = div_scale(x; d, x, flag),
+//   r = rcp(ds), then twice r = fma(r, fma(-ds, r, 1), r),
+//   q = xs*r, e = fma(-ds, q, xs), q = div_fmas(e, r, q, flag),
+//   result = div_fixup(q, d, x).
+// div_scale returns its operand unchanged and clears the flag, and div_fmas
+// is then a plain fma, unless an operand or the quotient nears the ends of
+// the exponent range (d or 1/d subnormal, x/d subnormal, |x| < 2^-969, or
+// exponent(x) - exponent(d) >= 768).  div_fixup decides the result from x
+// and d alone when either is zero, infinite or NaN.  So for d in
+// [2^-40, 2^94] and x = 0 or |x| in [2^-800, 2^340] (or x, d special), the
+// steps without the scaling give the hardware quotient bit for bit, and one
+// refined reciprocal serves every numerator over the same d: 4 instructions
+// per quotient after 5 for the reciprocal, against 11 per quotient.
+//
+// The compiler's sqrt scales x < 2^-767 by 2^256 before its rsq-based
+// Newton steps (and the result back) and passes +-0 and +Inf through; for x
+// in [2^-767, DBL_MAX] both are no-ops, and sqrt_rng is the same steps
+// without them: 10 instructions against 17.
+//
+// central_upwind_rng is central_upwind with these forms.  It is used only by
+// a wave whose stage inputs (w, hu, hv of every cell the face stencils read)
+// and bottom corners all lie in {0} U [2^-100, 2^40] in magnitude (checked as
+// the rows arrive, in_band), with θ (linear reconstruction), ρ and g in
+// [2^-10, 2^10] and depth_cutoff and κ in [2^-39, 2^10] (KParams::ranged).
+// Then (exact differences of such values are multiples of 2^-152, rounded
+// products keep their exponent sums, and cancellation in u +- c leaves at
+// least an ulp of c >= sqrt(g*cutoff)):
+//   face values:   0 or [2^-215, 2^51]; wet depths h > cutoff >= 2^-39
+//   velocity quotient hu/h*:  x in {0} U [2^-215, 2^51], d = h* in [2^-40, 2^51]
+//   c = sqrt(g h):  g h in [2^-49, 2^61]
+//   a+, a-:         0 or [2^-78, 2^92];  da = a+ - a- in [κ, 2^93] unless degenerate
+//   flux quotients: numerators 0 or [2^-712, 2^336], (a+ a-) in [2^-156, 2^184]
+// which is inside the range above.  A degenerate face (|da| < κ) discards
+// its quotients, as the reference's early return does.
+struct Recip {
+    double d, r;
+};
+__device__ __forceinline__ Recip recip_rng(double d) {
+    double r = __builtin_amdgcn_rcp(d);
+    double t = __builtin_fma(-d, r, 1.0);
+    r = __builtin_fma(r, t, r);
+    t = __builtin_fma(-d, r, 1.0);
+    r = __builtin_fma(r, t, r);
+    return {d, r};
+}
+__device__ __forceinline__ double div_rng(double x, const Recip& R) {
+    const double q = x * R.r;
+    const double e = __builtin_fma(-R.d, q, x);
+    return __builtin_amdgcn_div_fixup(__builtin_fma(e, R.r, q), R.d, x);
+}
+__device__ __forceinline__ double sqrt_rng(double x) {
+    double y = __builtin_amdgcn_rsq(x);
+    double g = x * y;
+    double h = y * 0.5;
+    const double r = __builtin_fma(-h, g, 0.5);
+    g = __builtin_fma(g, r, g);
+    h = __builtin_fma(h, r, h);
+    double d = __builtin_fma(-g, g, x);
+    g = __builtin_fma(d, h, g);
+    d = __builtin_fma(-g, g, x);
+    return __builtin_fma(d, h, g);
+}
+// |v| is 0 or in [2^-100, 2^40] (false for NaN, Inf and subnormals).
+__device__ __forceinline__ bool in_band(double v) {
+    const double a = __builtin_fabs(v);
+    return a == 0.0 || (a >= 0x1p-100 && a <= 0x1p40);
+}
+
+// swe_2D.jl:43-81 with the ranged quotients and square root (h > cutoff).
+template <class S>
+__device__ __forceinline__ void phys_flux_rng(int d, S h, S hu, S hv, const PhysParams& p, double f[3],
+                                              double e[2]) {
+    const double hs = desingularize(h, p.kappa);
+    const Recip R = recip_rng(hs);
+    const double u = div_rng(hu, R);
+    const double v = div_rng(hv, R);
+    const S h2 = h * h;
+    const double c = sqrt_rng(p.g * h);
+    if (d == 0) {
+        f[0] = p.rho * h * u;
+        f[1] = p.rho * h * u * u + 0.5 * p.rho * p.g * h2;
+        f[2] = p.rho * h * u * v;
+        e[0] = u + c;
+        e[1] = u - c;
+    } else {
+        f[0] = p.rho * h * v;
+        f[1] = p.rho * h * u * v;
+        f[2] = p.rho * h * v * v + 0.5 * p.rho * p.g * h2;
+        e[0] = v + c;
+        e[1] = v - c;
+    }
+}
+
+// centralupwind.jl:48-77 as central_upwind, with the ranged forms.
+template <class S>
+__device__ __forceinline__ double central_upwind_rng(int d, const S m[3], const S q[3], const PhysParams& p,
+                                                     double F[3]) {
+    double fm[3] = {0.0, 0.0, 0.0}, em[2] = {0.0, 0.0};
+    double fp[3] = {0.0, 0.0, 0.0}, ep[2] = {0.0, 0.0};
+    if (m[0] > p.cutoff) phys_flux_rng(d, m[0], m[1], m[2], p, fm, em);
+    if (q[0] > p.cutoff) phys_flux_rng(d, q[0], q[1], q[2], p, fp, ep);
+    const double aplus = nz_max(nz_max(ep[0], em[0]), 0.0);
+    const double aminus = nz_min(nz_min(ep[1], em[1]), 0.0);
+    const double da = aplus - aminus;
+    if (__builtin_fabs(da) < p.kappa) {
+        F[0] = F[1] = F[2] = 0.0;
+        return 0.0;
+    }
+    const Recip R = recip_rng(da);
+    const double pm = div_rng(aplus * aminus, R);
+#pragma unroll
+    for (int v = 0; v < 3; ++v) {
+        const S diff = q[v] - m[v];
+        F[v] = div_rng(aplus * fm[v] - aminus * fp[v], R) + pm * diff;
+    }
+    if (m[0] < p.cutoff && q[0] < p.cutoff) return 0.0;
+    return nz_max(__builtin_fabs(aplus), __builtin_fabs(aminus));
+}
+
 // Both sides of a face wet with |h| >= κ: central_upwind_wet applies.
 template <class S>
 __device__ __forceinline__ bool face_is_wet(const S m[3], const S q[3], const PhysParams& p) {
