@@ -48,6 +48,12 @@ namespace sfv {
 #ifndef SFV_MARCH_PREFETCH
 #define SFV_MARCH_PREFETCH 0
 #endif
+#ifndef SFV_MARCH_PREFETCH1
+#define SFV_MARCH_PREFETCH1 SFV_MARCH_PREFETCH
+#endif
+#ifndef SFV_MARCH_PREFETCH2
+#define SFV_MARCH_PREFETCH2 SFV_MARCH_PREFETCH
+#endif
 #ifndef SFV_MARCH_BRANCHFREE
 #define SFV_MARCH_BRANCHFREE 0
 #endif
@@ -206,6 +212,7 @@ __device__ __forceinline__ void march_tile(const KParams& P, StepCtl* ctl, int s
         if (STAGE == 2 && close && strip == 0 && seg == 0 && lane == 0) ctl->slot[1 - slot].active = 0;
         return;
     }
+    constexpr bool PREFETCH = STAGE == 1 ? SFV_MARCH_PREFETCH1 : SFV_MARCH_PREFETCH2;
     const int gc = P.gc, nxT = P.nxT, nyT = P.nyT;
     const int j0 = gc + seg * my;
     if (j0 >= nyT - gc) {                          // whole wave: past the grid
@@ -308,7 +315,7 @@ __device__ __forceinline__ void march_tile(const KParams& P, StepCtl* ctl, int s
     fetch_b(j0 - 1, bp1, e1_);
     np1 = next_col(bp1, e1_);
     Raw cur, nxt;
-    if (SFV_MARCH_PREFETCH) fetch(j0 - 2, cur);
+    if (PREFETCH) fetch(j0 - 2, cur);
     // ranged face evaluation (central_upwind_rng) while the wave's inputs allow it
     bool band = P.ranged && __all(in_band(b0) && in_band(e0_) && in_band(bp1) && in_band(e1_));
 
@@ -339,7 +346,7 @@ __device__ __forceinline__ void march_tile(const KParams& P, StepCtl* ctl, int s
         const bool yface = FULL || r >= 3;
         double nb, ne;
         fetch_b(jr + 2, nb, ne);                   // next corner row (Bload bounds-checks row nyT+1)
-        if (SFV_MARCH_PREFETCH) {
+        if (PREFETCH) {
             if (r + 1 < th + 4) fetch(jr + 1, nxt);    // rows up to j0+th+1 <= nyT-1 (gc >= 2)
         } else {
             fetch(jr, cur);
@@ -535,7 +542,7 @@ __device__ __forceinline__ void march_tile(const KParams& P, StepCtl* ctl, int s
         bm1 = b0; nm1 = n0;
         b0 = bp1; n0 = np1;
         bp1 = nb; np1 = next_col(nb, ne);
-        if (SFV_MARCH_PREFETCH) cur = nxt;
+        if (PREFETCH) cur = nxt;
     };
 
     // th >= 1, so there are th+4 >= 5 iterations: r = 0..3 peeled, the loop
