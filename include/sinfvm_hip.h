@@ -113,7 +113,7 @@ int sfv_create(const sfv_config* cfg, sfv_handle** out);
 void sfv_destroy(sfv_handle* h);
 /* Text of the last error on this handle ("" if none).  Valid until the next call. */
 const char* sfv_last_error(const sfv_handle* h);
-/* Text of the last error of sfv_create (thread-local), for a NULL handle. */
+/* Text of the last error of sfv_create or sfv_advance_noalloc (thread-local). */
 const char* sfv_create_error(void);
 
 /* BottomTopography2D(B, backend, grid) (bottom_topography.jl:51-57): corner
@@ -157,6 +157,33 @@ int sfv_run(sfv_handle* h, double t_end, int32_t match_endtime, double maximum_t
 /* nsteps calls of perform_step!(sim, max_dt) with t += dt after each
  * (the fixed-step-count workload), device-resident.  *t_io in/out. */
 int sfv_advance(sfv_handle* h, int64_t nsteps, double max_dt, double* t_io);
+
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (tight serving loops, harnesses that forbid hipMalloc).  sfv_advance
+ * on buffers the caller owns, with no handle and no allocation: nsteps calls
+ * of perform_step!(sim, max_dt) with t += dt after each, device-resident, on
+ * the full state U_dev ((nxT, nyT, 3) in the configuration's precision, in
+ * device memory, 16-byte aligned; replaced by the state after nsteps steps,
+ * ghost cells included, as sfv_get_state with SFV_LAYOUT_FULL would return
+ * it) over the corner topography B_dev ((nxT+1, nyT+1) double in device
+ * memory).  *t_io is the start time on entry and the end time on exit;
+ * *dt_out (may be NULL) receives the last step's dt.  The scratch the steps
+ * need (the stage right-hand side, the second state of the ping-pong pair and
+ * the step-control block) is one static device array of
+ * SFV_NOALLOC_SCRATCH_BYTES that each GPU the library runs on holds until the
+ * process ends: grids of at most 1,398,085 cells ghosts included (f64;
+ * 2,796,170 with f32 storage) are accepted, larger ones return
+ * SFV_ERR_DOMAIN.  The configuration must suit the fused RK2 kernels:
+ * RungeKutta2, WallBC or PeriodicBC with nx, ny >= gc >= 2, rain none or
+ * constant, infiltration none or constant (no device tables); other ones
+ * return SFV_ERR_ARGUMENT.  Work goes on `stream` (a hipStream_t; NULL: the
+ * null stream) and the call returns when it has finished.  Results are
+ * bitwise those of sfv_advance.  Every call on a GPU shares that GPU's
+ * scratch: calls that overlap on one GPU race.  The message of a failed call
+ * is sfv_create_error()'s. */
+#define SFV_NOALLOC_SCRATCH_BYTES (64ull * 1024 * 1024)
+int sfv_advance_noalloc(const sfv_config* cfg, const double* B_dev, void* U_dev, int64_t nsteps, double max_dt,
+                        double* t_io, double* dt_out, void* stream);
 
 /* dt of the most recent step (current_timestep(sim), simulator.jl:141). */
 int sfv_current_timestep(sfv_handle* h, double* dt_out);

@@ -37,57 +37,6 @@ int fail(sfv_handle* h, int code, const std::string& msg) {
             return fail((h), SFV_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));        \
     } while (0)
 
-// Replay bc.jl's per-ghost assignments (for g = 1..gc, low side then high
-// side) on an index map, so each ghost ends up naming the cell whose pre-BC
-// value it receives and how many times its normal momentum was negated.
-void ghost_maps(int bc, int gc, int N, int* lo, int* hi, int* flo, int* fhi) {
-    std::vector<int> cur(N), neg(N, 0);
-    for (int x = 0; x < N; ++x) cur[x] = x;
-    const int f = bc == SFV_BC_WALL ? 1 : 0;
-    for (int g = 1; g <= gc; ++g) {
-        int dl, sl, dr, sr;
-        if (bc == SFV_BC_WALL) {            // bc.jl:92-109
-            dl = g - 1; sl = 2 * gc - g; dr = N - gc + g - 1; sr = N - gc - g;
-        } else if (bc == SFV_BC_PERIODIC) { // bc.jl:31-39
-            dl = g - 1; sl = N + g - 2 * gc - 1; dr = N - (gc - g) - 1; sr = gc + g - 1;
-        } else {                            // bc.jl:51-59
-            dl = g - 1; sl = 2 * gc - g; dr = N - (gc - g) - 1; sr = N - g;
-        }
-        cur[dl] = cur[sl]; neg[dl] = neg[sl] ^ f;
-        cur[dr] = cur[sr]; neg[dr] = neg[sr] ^ f;
-    }
-    for (int g = 0; g < gc; ++g) {
-        lo[g] = cur[g]; flo[g] = neg[g];
-        hi[g] = cur[N - gc + g]; fhi[g] = neg[N - gc + g];
-    }
-}
-
-int validate(const sfv_config* c, std::string& why) {
-    if (!c) { why = "config is NULL"; return SFV_ERR_ARGUMENT; }
-    if (c->nx < 1 || c->ny < 1) { why = "nx and ny must be >= 1"; return SFV_ERR_DOMAIN; }
-    if (c->gc < 1 || c->gc > MAX_GC) { why = "gc must be in [1, 8]"; return SFV_ERR_DOMAIN; }
-    if (c->reconstruction == SFV_LINEAR_RECONSTRUCTION && !(c->gc > 1)) {
-        why = "LinearReconstruction needs grid.ghostcells > 1 (linearreconstruction.jl:59)";
-        return SFV_ERR_ASSERTION;
-    }
-    if (c->reconstruction != SFV_LINEAR_RECONSTRUCTION && c->reconstruction != SFV_NO_RECONSTRUCTION) {
-        why = "unknown reconstruction"; return SFV_ERR_ARGUMENT;
-    }
-    if (c->bc < SFV_BC_WALL || c->bc > SFV_BC_NEUMANN) { why = "unknown boundary condition"; return SFV_ERR_ARGUMENT; }
-    if (c->stepper != SFV_FORWARD_EULER && c->stepper != SFV_RUNGE_KUTTA2) { why = "unknown stepper"; return SFV_ERR_ARGUMENT; }
-    if (c->friction < SFV_FRICTION_NONE || c->friction > SFV_FRICTION_FCG2016) { why = "unknown friction law"; return SFV_ERR_ARGUMENT; }
-    if (c->rain < SFV_RAIN_NONE || c->rain > SFV_RAIN_FIELD) { why = "unknown rain kind"; return SFV_ERR_ARGUMENT; }
-    if (c->rain == SFV_RAIN_TIME_TABLE && (c->rain_table_len < 1 || !c->rain_table_rates || !c->rain_table_times)) {
-        why = "TimeDependentRain needs rates and times of equal, non-zero length (source_term_rain.jl:36-38)";
-        return SFV_ERR_DOMAIN;
-    }
-    if (c->infiltration < SFV_INFILTRATION_NONE || c->infiltration > SFV_INFILTRATION_HORTON) {
-        why = "unknown infiltration kind"; return SFV_ERR_ARGUMENT;
-    }
-    if (c->precision != SFV_F64 && c->precision != SFV_F32_STORAGE) { why = "unknown precision"; return SFV_ERR_ARGUMENT; }
-    return SFV_OK;
-}
-
 template <class S>
 int step_kernels(sfv_handle* h, int slot, int cur) {
     const KParams& P = h->P;
@@ -230,6 +179,74 @@ hipError_t tune_steps(sfv_handle* h, int slot, int n, bool tiled) {
 }
 
 }  // namespace
+
+namespace sfv {
+void set_call_error(const std::string& msg) { g_create_error = msg; }
+
+// Replay bc.jl's per-ghost assignments (for g = 1..gc, low side then high
+// side) on an index map, so each ghost ends up naming the cell whose pre-BC
+// value it receives and how many times its normal momentum was negated.
+void ghost_maps(int bc, int gc, int N, int* lo, int* hi, int* flo, int* fhi) {
+    // Every index the assignments touch lies within gc of an edge's ghost
+    // layer: in [0, 2gc) or [N-2gc, N).  The map is kept for those only, in a
+    // fixed array (no heap): slot x for x < 2gc, 2gc + (x - (N-2gc)) above.
+    constexpr int W = 4 * MAX_GC;
+    int cur[W], neg[W];
+    const bool direct = N <= W;
+    auto at = [&](int x) { return direct || x < 2 * gc ? x : 2 * gc + (x - (N - 2 * gc)); };
+    const int span = direct ? N : 4 * gc;
+    for (int k = 0; k < span; ++k) neg[k] = 0;
+    if (direct) {
+        for (int x = 0; x < N; ++x) cur[x] = x;
+    } else {
+        for (int x = 0; x < 2 * gc; ++x) { cur[at(x)] = x; cur[at(N - 2 * gc + x)] = N - 2 * gc + x; }
+    }
+    const int f = bc == SFV_BC_WALL ? 1 : 0;
+    for (int g = 1; g <= gc; ++g) {
+        int dl, sl, dr, sr;
+        if (bc == SFV_BC_WALL) {            // bc.jl:92-109
+            dl = g - 1; sl = 2 * gc - g; dr = N - gc + g - 1; sr = N - gc - g;
+        } else if (bc == SFV_BC_PERIODIC) { // bc.jl:31-39
+            dl = g - 1; sl = N + g - 2 * gc - 1; dr = N - (gc - g) - 1; sr = gc + g - 1;
+        } else {                            // bc.jl:51-59
+            dl = g - 1; sl = 2 * gc - g; dr = N - (gc - g) - 1; sr = N - g;
+        }
+        cur[at(dl)] = cur[at(sl)]; neg[at(dl)] = neg[at(sl)] ^ f;
+        cur[at(dr)] = cur[at(sr)]; neg[at(dr)] = neg[at(sr)] ^ f;
+    }
+    for (int g = 0; g < gc; ++g) {
+        lo[g] = cur[at(g)]; flo[g] = neg[at(g)];
+        hi[g] = cur[at(N - gc + g)]; fhi[g] = neg[at(N - gc + g)];
+    }
+}
+
+int validate_config(const sfv_config* c, std::string& why) {
+    if (!c) { why = "config is NULL"; return SFV_ERR_ARGUMENT; }
+    if (c->nx < 1 || c->ny < 1) { why = "nx and ny must be >= 1"; return SFV_ERR_DOMAIN; }
+    if (c->gc < 1 || c->gc > MAX_GC) { why = "gc must be in [1, 8]"; return SFV_ERR_DOMAIN; }
+    if (c->reconstruction == SFV_LINEAR_RECONSTRUCTION && !(c->gc > 1)) {
+        why = "LinearReconstruction needs grid.ghostcells > 1 (linearreconstruction.jl:59)";
+        return SFV_ERR_ASSERTION;
+    }
+    if (c->reconstruction != SFV_LINEAR_RECONSTRUCTION && c->reconstruction != SFV_NO_RECONSTRUCTION) {
+        why = "unknown reconstruction"; return SFV_ERR_ARGUMENT;
+    }
+    if (c->bc < SFV_BC_WALL || c->bc > SFV_BC_NEUMANN) { why = "unknown boundary condition"; return SFV_ERR_ARGUMENT; }
+    if (c->stepper != SFV_FORWARD_EULER && c->stepper != SFV_RUNGE_KUTTA2) { why = "unknown stepper"; return SFV_ERR_ARGUMENT; }
+    if (c->friction < SFV_FRICTION_NONE || c->friction > SFV_FRICTION_FCG2016) { why = "unknown friction law"; return SFV_ERR_ARGUMENT; }
+    if (c->rain < SFV_RAIN_NONE || c->rain > SFV_RAIN_FIELD) { why = "unknown rain kind"; return SFV_ERR_ARGUMENT; }
+    if (c->rain == SFV_RAIN_TIME_TABLE && (c->rain_table_len < 1 || !c->rain_table_rates || !c->rain_table_times)) {
+        why = "TimeDependentRain needs rates and times of equal, non-zero length (source_term_rain.jl:36-38)";
+        return SFV_ERR_DOMAIN;
+    }
+    if (c->infiltration < SFV_INFILTRATION_NONE || c->infiltration > SFV_INFILTRATION_HORTON) {
+        why = "unknown infiltration kind"; return SFV_ERR_ARGUMENT;
+    }
+    if (c->precision != SFV_F64 && c->precision != SFV_F32_STORAGE) { why = "unknown precision"; return SFV_ERR_ARGUMENT; }
+    return SFV_OK;
+}
+
+}  // namespace sfv
 
 namespace sfv {
 bool autotune_enabled() {
@@ -417,14 +434,10 @@ int run_loop(sfv_handle* h, double t0, double endtime, int match_endtime, int ha
 
 }  // namespace
 
-void sfv_handle::ghost_maps_into(int bc, int gc, int N, int* lo, int* hi, int* flo, int* fhi) {
-    ghost_maps(bc, gc, N, lo, hi, flo, fhi);
-}
-
 namespace sfv {
 int create_handle(const sfv_config* cfg, DevicePool* pool, sfv_handle** out, std::string& why) {
     *out = nullptr;
-    const int vr = validate(cfg, why);
+    const int vr = validate_config(cfg, why);
     if (vr) return vr;
     sfv_handle* h = new sfv_handle();
     h->pool = pool;

@@ -14,6 +14,59 @@
 #include "sfv_pool.hpp"
 #include "swe_kernels.hpp"
 
+namespace sfv {
+// The reference constructors' checks of a configuration (sfv_capi.hip).
+int validate_config(const sfv_config* c, std::string& why);
+// The message sfv_create_error() returns on this thread (sfv_capi.hip).
+void set_call_error(const std::string& msg);
+// bc.jl's ghost-cell assignments replayed on an index map (sfv_capi.hip).
+void ghost_maps(int bc, int gc, int N, int* lo, int* hi, int* flo, int* fhi);
+
+// The kernel parameter block of a configuration, pointing at the device
+// copies of its tables (rain table: rates then times; Horton factor field;
+// FunctionalRain field), each NULL when the configuration has none.
+// Allocates nothing.
+inline void build_kparams(const sfv_config& c, KParams& p, const double* d_rain, int rain_n, const double* d_factor,
+                          const double* d_rain_field, int march_rows) {
+    std::memset(&p, 0, sizeof(p));
+    p.nx = c.nx; p.ny = c.ny; p.gc = c.gc;
+    p.nxT = c.nx + 2 * c.gc; p.nyT = c.ny + 2 * c.gc;
+    p.dx = c.dx; p.dy = c.dy;
+    p.inv_dx = 1 / c.dx; p.inv_dy = 1 / c.dy;
+    auto pow2 = [](double d) { int e; return std::isfinite(d) && d > 0 && std::frexp(d, &e) == 0.5; };
+    p.dx_pow2 = pow2(c.dx) ? 1 : 0;
+    p.dy_pow2 = pow2(c.dy) ? 1 : 0;
+    p.phys.g = c.g; p.phys.rho = c.rho; p.phys.cutoff = c.depth_cutoff; p.phys.kappa = c.desingularizing_kappa;
+    p.theta = c.theta; p.cfl = c.cfl; p.Cz = c.Cz;
+    p.rec = c.reconstruction == SFV_LINEAR_RECONSTRUCTION ? 1 : 0;
+    p.friction = c.friction;
+    p.bottom = c.bottom_source ? 1 : 0;
+    p.rain_kind = c.rain;
+    p.infil_kind = c.infiltration;
+    p.rain_first = c.rain_before_infiltration ? 1 : 0;
+    p.stepper = c.stepper;
+    p.rain_n = rain_n;
+    p.rain_rates = d_rain;
+    p.rain_times = d_rain ? d_rain + rain_n : nullptr;
+    p.rain_const = c.rain_rate / 3600;
+    p.infil_rate = c.infiltration_rate;
+    p.f0 = c.horton_f0; p.fc = c.horton_fc; p.hk = c.horton_k;
+    p.factor = d_factor;
+    p.rain_field = d_rain_field;
+    p.bcell_1d = c.embedded_1d ? 1 : 0;
+    p.bc = c.bc;
+    ghost_maps(c.bc, c.gc, p.nxT, p.src_lo_x, p.src_hi_x, p.flip_lo_x, p.flip_hi_x);
+    ghost_maps(c.bc, c.gc, p.nyT, p.src_lo_y, p.src_hi_y, p.flip_lo_y, p.flip_hi_y);
+    p.rows = march_rows;
+    // central_upwind_rng's parameter conditions; SFV_RANGED=0 turns it off
+    auto in = [](double v, double lo, double hi) { return v >= lo && v <= hi; };
+    const char* rng_env = std::getenv("SFV_RANGED");
+    p.ranged = !(rng_env && rng_env[0] == '0') && (p.rec != 1 || in(c.theta, 0x1p-10, 0x1p10)) &&
+               in(c.rho, 0x1p-10, 0x1p10) && in(c.g, 0x1p-10, 0x1p10) && in(c.depth_cutoff, 0x1p-39, 0x1p10) &&
+               in(c.desingularizing_kappa, 0x1p-39, 0x1p10);
+}
+}  // namespace sfv
+
 // Kernel variants of an RK2 step (all give identical results).
 // Kernel variants of a step.  KV_HYBRID: every right-hand side from the march
 // stage-1 kernel, the updates and boundary conditions from the per-pass
@@ -153,47 +206,8 @@ struct sfv_handle {
     }
 
     void build_params() {
-        const sfv_config& c = cfg;
-        sfv::KParams& p = P;
-        std::memset(&p, 0, sizeof(p));
-        p.nx = c.nx; p.ny = c.ny; p.gc = c.gc;
-        p.nxT = c.nx + 2 * c.gc; p.nyT = c.ny + 2 * c.gc;
-        p.dx = c.dx; p.dy = c.dy;
-        p.inv_dx = 1 / c.dx; p.inv_dy = 1 / c.dy;
-        auto pow2 = [](double d) { int e; return std::isfinite(d) && d > 0 && std::frexp(d, &e) == 0.5; };
-        p.dx_pow2 = pow2(c.dx) ? 1 : 0;
-        p.dy_pow2 = pow2(c.dy) ? 1 : 0;
-        p.phys.g = c.g; p.phys.rho = c.rho; p.phys.cutoff = c.depth_cutoff; p.phys.kappa = c.desingularizing_kappa;
-        p.theta = c.theta; p.cfl = c.cfl; p.Cz = c.Cz;
-        p.rec = c.reconstruction == SFV_LINEAR_RECONSTRUCTION ? 1 : 0;
-        p.friction = c.friction;
-        p.bottom = c.bottom_source ? 1 : 0;
-        p.rain_kind = c.rain;
-        p.infil_kind = c.infiltration;
-        p.rain_first = c.rain_before_infiltration ? 1 : 0;
-        p.stepper = c.stepper;
-        p.rain_n = (int)rain_rates.size();
-        p.rain_rates = d_rain;
-        p.rain_times = d_rain ? d_rain + rain_rates.size() : nullptr;
-        p.rain_const = c.rain_rate / 3600;
-        p.infil_rate = c.infiltration_rate;
-        p.f0 = c.horton_f0; p.fc = c.horton_fc; p.hk = c.horton_k;
-        p.factor = d_factor;
-        p.rain_field = d_rain_field;
-        p.bcell_1d = c.embedded_1d ? 1 : 0;
-        p.bc = c.bc;
-        ghost_maps_into(c.bc, c.gc, p.nxT, p.src_lo_x, p.src_hi_x, p.flip_lo_x, p.flip_hi_x);
-        ghost_maps_into(c.bc, c.gc, p.nyT, p.src_lo_y, p.src_hi_y, p.flip_lo_y, p.flip_hi_y);
-        p.rows = march_rows;
-        // central_upwind_rng's parameter conditions; SFV_RANGED=0 turns it off
-        auto in = [](double v, double lo, double hi) { return v >= lo && v <= hi; };
-        const char* rng_env = std::getenv("SFV_RANGED");
-        p.ranged = !(rng_env && rng_env[0] == '0') && (p.rec != 1 || in(c.theta, 0x1p-10, 0x1p10)) &&
-                   in(c.rho, 0x1p-10, 0x1p10) && in(c.g, 0x1p-10, 0x1p10) && in(c.depth_cutoff, 0x1p-39, 0x1p10) &&
-                   in(c.desingularizing_kappa, 0x1p-39, 0x1p10);
+        sfv::build_kparams(cfg, P, d_rain, (int)rain_rates.size(), d_factor, d_rain_field, march_rows);
     }
-
-    static void ghost_maps_into(int bc, int gc, int N, int* lo, int* hi, int* flo, int* fhi);
 
     // The march stage-1 kernel reads a 2-cell halo of the state and writes
     // the interior right-hand side; it needs gc >= 2 and nothing else.

@@ -176,6 +176,9 @@ def _declare(L) -> None:
     L.sfv_run.restype = c_int32
     L.sfv_advance.argtypes = [h, c_int64, c_double, POINTER(c_double)]
     L.sfv_advance.restype = c_int32
+    L.sfv_advance_noalloc.argtypes = [cfgp, c_void_p, c_void_p, c_int64, c_double, POINTER(c_double),
+                                      POINTER(c_double), c_void_p]
+    L.sfv_advance_noalloc.restype = c_int32
     L.sfv_current_timestep.argtypes = [h, POINTER(c_double)]
     L.sfv_current_timestep.restype = c_int32
     L.sfv_synchronize.argtypes = [h]
