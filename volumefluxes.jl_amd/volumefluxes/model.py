@@ -207,25 +207,59 @@ def cell_center_coordinates(grid: CartesianGrid):
     return xs, ys
 
 
-def rain_field_values(rain: FunctionalRain, grid: CartesianGrid, t: float) -> np.ndarray:
+_RAIN_POOL = None
+
+
+def _rain_pool(workers: int):
+    global _RAIN_POOL
+    if _RAIN_POOL is None or _RAIN_POOL._max_workers != workers:
+        from concurrent.futures import ThreadPoolExecutor
+        _RAIN_POOL = ThreadPoolExecutor(max_workers=workers)
+    return _RAIN_POOL
+
+
+def rain_field_values(rain: FunctionalRain, grid: CartesianGrid, t: float, out=None, workers=None) -> np.ndarray:
     """compute_rain(rain, t, I) (source_term_rain.jl:64-68) for every cell, as
-    a (nxT, nyT) float64 array.  The function is called once with array
-    arguments when it accepts them, else once per cell."""
+    a (nxT, nyT) float64 array (written into `out` when given, e.g. a pinned
+    host buffer).  The function is called with array arguments when it
+    accepts them, on blocks of columns evaluated by a pool of host threads
+    (numpy releases the interpreter lock in its element-wise kernels; the
+    rate of a cell depends on (t, x, y) alone, as the reference evaluates it
+    per cell), else once per cell."""
+    import os
     xs, ys = cell_center_coordinates(grid)
-    X = np.broadcast_to(xs[:, None], grid.totalcells)
-    Y = np.broadcast_to(ys[None, :], grid.totalcells)
-    try:
+    nxT, nyT = grid.totalcells
+    if out is None:
+        out = np.empty((nxT, nyT), order="F")
+    if workers is None:
+        workers = max(1, min(16, len(os.sched_getaffinity(0))))
+
+    def block(j0, j1):
+        X = np.broadcast_to(xs[:, None], (nxT, j1 - j0))
+        Y = np.broadcast_to(ys[None, j0:j1], (nxT, j1 - j0))
         v = np.asarray(rain.rain_function(t, X, Y), dtype=np.float64)
         if v.shape == ():
-            return np.full(grid.totalcells, float(v), order="F")
-        if v.shape == tuple(grid.totalcells):
-            return np.asfortranarray(v)
+            out[:, j0:j1] = float(v)
+        elif v.shape == (nxT, j1 - j0):
+            out[:, j0:j1] = v
+        else:
+            return False
+        return True
+
+    try:
+        if workers <= 1 or nxT * nyT < (1 << 16):
+            ok = block(0, nyT)
+        else:
+            n = min(nyT, 4 * workers)
+            edges = [nyT * k // n for k in range(n + 1)]
+            ok = all(_rain_pool(workers).map(lambda k: block(edges[k], edges[k + 1]), range(n)))
+        if ok:
+            return out
     except (TypeError, ValueError):
         pass
-    out = np.empty(grid.totalcells, order="F")
     f = rain.rain_function
-    for j in range(grid.totalcells[1]):
-        for i in range(grid.totalcells[0]):
+    for j in range(nyT):
+        for i in range(nxT):
             out[i, j] = float(f(t, float(xs[i]), float(ys[j])))
     return out
 

@@ -91,11 +91,16 @@ class Simulator:
         C.check(self._h, L.sfv_set_topography(self._h, C.dptr(B), B.shape[0], B.shape[1]))
         fr = [s for s in self._system2.source_terms if isinstance(s, FunctionalRain)]
         self._functional_rain = fr[0] if fr else None
+        self._rain_buf = None
+        if self._functional_rain is not None:
+            from ._pinned import PinnedArray
+            self._rain_buf = PinnedArray(self._grid2.totalcells)   # page-locked: the upload runs at DMA speed
 
     def _refresh_rain(self) -> None:
-        """FunctionalRain: the rates at this step's start time on every cell."""
+        """FunctionalRain: the rates at this step's start time on every cell,
+        evaluated by host threads into the page-locked buffer and uploaded."""
         if self._functional_rain is not None:
-            F = rain_field_values(self._functional_rain, self._grid2, self.t)
+            F = rain_field_values(self._functional_rain, self._grid2, self.t, out=self._rain_buf.array)
             C.check(self._h, C.lib().sfv_set_rain_field(self._h, C.dptr(F)))
 
     def __del__(self):
@@ -103,6 +108,10 @@ class Simulator:
         if h:
             C.lib().sfv_destroy(h)
             self._h = None
+        buf = getattr(self, "_rain_buf", None)
+        if buf is not None:
+            buf.free()
+            self._rain_buf = None
 
     # -- state transfer ----------------------------------------------------
     @property
