@@ -257,19 +257,34 @@ def serving_scenarios(rate: float, duration: float, t_range, seed: int):
     return serving_requests(rate, duration, t_range, seed)
 
 
-def serving_bench(rate: float, duration: float, t_range, budget_gb: float, seed: int):
+def _progress(msg: str) -> None:
+    """A progress line on stderr (stdout carries the one JSON result line)."""
+    print(msg, file=sys.stderr, flush=True)
+
+
+def serving_bench(rate: float, duration: float, t_range, budget_gb: float, seed: int, cap_s: float = 0.0):
     """Continuous batching (Server: arrivals submitted in real time, admitted
     under the HBM budget, advanced together) against plain one-at-a-time
     FIFO service of the same requests (each a device-resident
     simulate_to_time; its queue is replayed from the measured service times
-    and the same arrival times).  Time-to-result = completion - arrival."""
+    and the same arrival times).  Time-to-result = completion - arrival; it
+    splits into the queue wait (admission - arrival) and the time on the
+    device (completion - admission).
+
+    With cap_s > 0 (a workload beyond what the device can finish in a
+    measurement) each path runs for cap_s seconds from the first arrival and
+    stops: the figures are the cell-updates delivered in that time (finished
+    requests and the progress of running ones), the requests finished, and
+    time-to-result over the finished ones; the others are reported as
+    unfinished."""
     import threading
     reqs = serving_scenarios(rate, duration, t_range, seed)
     n = len(reqs)
+    budget = int(budget_gb * (1 << 30))
     backend = VF.make_hip_backend(np.float64)
     # warm-up of both paths
     w = reqs[0]
-    with VF.Server(int(budget_gb * (1 << 30))) as sv:
+    with VF.Server(budget) as sv:
         rid = sv.submit(w["system"], w["grid"], w["U0"], 0.1)
         sv.wait(rid, timeout=120)
         sv.fetch(rid)
@@ -277,35 +292,56 @@ def serving_bench(rate: float, duration: float, t_range, budget_gb: float, seed:
     VF.set_current_state(sim, w["U0"])
     VF.simulate_to_time(sim, 0.1)
     del sim
+    _progress(f"[serving] {n} requests over {duration} s of arrivals, t_end U{tuple(t_range)}, budget {budget_gb} GB")
 
-    # plain: service times of one-at-a-time solo runs
+    # plain: one request at a time in arrival order, each a device-resident
+    # simulate_to_time (in chunks of steps, so a cap can stop a long one);
+    # the FIFO clock starts at the first arrival and idles until the next
+    # arrival when the queue is empty
     service, solo_states, cells = [], {}, []
     picks = set(range(0, n, max(1, n // 6))) | {n - 1}
+    t_plain = time.perf_counter()
+    clock, done, plain_cells = reqs[0]["arrival"], [], 0.0
+    L0 = C.lib()
     for k, r in enumerate(reqs):
+        if cap_s > 0 and clock >= cap_s:
+            break
+        clock = max(clock, r["arrival"])
         sim = VF.Simulator(backend, r["system"], VF.RungeKutta2(), r["grid"])
         VF.set_current_state(sim, r["U0"])
-        C.check(sim.handle, C.lib().sfv_synchronize(sim.handle))
-        t0 = time.perf_counter()
-        L0 = C.lib()
+        C.check(sim.handle, L0.sfv_synchronize(sim.handle))
         tt = ctypes.c_double(0.0)
-        steps = ctypes.c_int64()
-        C.check(sim.handle, L0.sfv_run(sim.handle, r["t_end"], 1, 0.0, 0, ctypes.byref(tt), ctypes.byref(steps)))
-        service.append(time.perf_counter() - t0)
-        cells.append(r["grid"].nx * r["grid"].ny * steps.value)
-        if k in picks:
-            g = r["grid"].gc
-            solo_states[k] = sim.get_full_state()[g:-g, g:-g, :]
+        steps, t0, finished = 0, time.perf_counter(), False
+        while True:
+            st_ = ctypes.c_int64()
+            chunk = 0 if cap_s <= 0 else 20000
+            C.check(sim.handle, L0.sfv_run(sim.handle, r["t_end"], 1, 0.0, chunk, ctypes.byref(tt), ctypes.byref(st_)))
+            steps += st_.value
+            if tt.value >= r["t_end"] or st_.value == 0:
+                finished = True
+                break
+            if cap_s > 0 and clock + (time.perf_counter() - t0) >= cap_s:
+                break
+        el = time.perf_counter() - t0
+        clock += el
+        plain_cells += r["grid"].nx * r["grid"].ny * steps
+        if finished:
+            service.append(el)
+            cells.append(r["grid"].nx * r["grid"].ny * steps)
+            done.append(clock)
+            if k in picks:
+                g = r["grid"].gc
+                solo_states[k] = sim.get_full_state()[g:-g, g:-g, :]
         del sim
-    done, clock = [], 0.0
-    for r, sv_t in zip(reqs, service):
-        clock = max(clock, r["arrival"]) + sv_t
-        done.append(clock)
+        if (k + 1) % 25 == 0 or k + 1 == n:
+            _progress(f"[serving] plain: {k + 1}/{n} requests run alone, {time.perf_counter() - t_plain:.0f} s")
+    plain_done = len(done)
     plain_lat = [d - r["arrival"] for d, r in zip(done, reqs)]
-    plain_span = done[-1] - reqs[0]["arrival"]
+    plain_span = (cap_s if cap_s > 0 else done[-1]) - reqs[0]["arrival"]
 
     # served: real-time arrivals into the server
     ids = [None] * n
-    with VF.Server(int(budget_gb * (1 << 30))) as sv:
+    with VF.Server(budget) as sv:
         def submitter():
             t_start = time.perf_counter()
             for k, r in enumerate(reqs):
@@ -313,41 +349,88 @@ def serving_bench(rate: float, duration: float, t_range, budget_gb: float, seed:
                 if delay > 0:
                     time.sleep(delay)
                 ids[k] = sv.submit(r["system"], r["grid"], r["U0"], r["t_end"])
+        t_served = time.perf_counter()
         th = threading.Thread(target=submitter)
         th.start()
         th.join()
-        infos, same = [], True
+        infos, same, served_cells, unfinished, compared = [], True, 0.0, 0, 0
         for k, rid in enumerate(ids):
-            sv.wait(rid, timeout=1200)
+            while True:
+                left = cap_s - (time.perf_counter() - t_served) if cap_s > 0 else 30.0
+                if left <= 0:
+                    break
+                try:
+                    sv.wait(rid, timeout=min(30.0, left))
+                    break
+                except TimeoutError:
+                    st = sv.stats()
+                    _progress(f"[serving] served: {st['completed']}/{n} done, {st['running']} running, "
+                              f"{st['queued']} queued, {st['bytes_in_use'] / 1e9:.2f} GB in use, "
+                              f"{time.perf_counter() - t_served:.0f} s")
+            info = sv.poll(rid)
+            served_cells += info["cell_updates"]
+            if info["status"] != 2:
+                unfinished += 1
+                continue
             U, info = sv.fetch(rid)
             infos.append(info)
             if k in solo_states:
                 same = same and np.array_equal(U, solo_states[k])
+                compared += 1
         st = sv.stats()
+        span_cap = time.perf_counter() - t_served
+    _progress(f"[serving] served: {len(infos)} of {n} done in {span_cap:.0f} s")
     lat = [i["done_s"] - i["submit_s"] for i in infos]
     wait = [i["admit_s"] - i["submit_s"] for i in infos]
-    span = max(i["done_s"] for i in infos) - min(i["submit_s"] for i in infos)
-    total = float(sum(cells))
+    on_dev = [i["done_s"] - i["admit_s"] for i in infos]
+    span = (span_cap if cap_s > 0 else max(i["done_s"] for i in infos) - min(i["submit_s"] for i in infos))
+    total = float(sum(cells)) if cap_s <= 0 else served_cells
 
     def pct(v, q):
-        return float(np.percentile(np.asarray(v), q))
+        return float(np.percentile(np.asarray(v), q)) if len(v) else None
+    served_rate = total / span
+    plain_rate = (plain_cells if cap_s > 0 else total) / plain_span
+    if cap_s > 0:
+        return {
+            "workload": (f"Poisson arrivals {rate}/s for {duration} s ({n} requests), square grids side log-uniform "
+                         f"[64, 2048], half dam breaks / half terrains, f64 RK2, t_end U[{t_range[0]}, {t_range[1]}] s, "
+                         f"HBM budget {budget_gb} GB; each path stopped {cap_s:.0f} s after the first arrival"),
+            "requests": n,
+            "load": (f"the requests need more than one GPU can deliver in {cap_s:.0f} s: the served path finished "
+                     f"{len(infos)} and the plain FIFO path {plain_done} of {n}; offered cell-updates per second "
+                     f"of arrivals are at least {served_cells / duration:.3g} (what the server delivered, divided by "
+                     f"the {duration} s arrival window), i.e. at least {span / duration:.0f}x the served rate"),
+            "served": {"finished": len(infos), "unfinished": unfinished,
+                       "cell_updates_per_s": served_rate, "cell_updates": served_cells,
+                       "p50_s_finished": pct(lat, 50), "p99_s_finished": pct(lat, 99),
+                       "queue_wait_p50_s": pct(wait, 50), "queue_wait_p99_s": pct(wait, 99),
+                       "on_device_p50_s": pct(on_dev, 50), "on_device_p99_s": pct(on_dev, 99),
+                       "busy_s": st["busy_s"], "peak_bytes": st["bytes_peak"], "budget_bytes": budget,
+                       "replays": st["replays"]},
+            "plain_fifo": {"finished": plain_done, "cell_updates_per_s": plain_rate, "cell_updates": plain_cells,
+                           "p50_s_finished": pct(plain_lat, 50), "p99_s_finished": pct(plain_lat, 99)},
+            "throughput_ratio": served_rate / plain_rate,
+            "parity_served_vs_alone": ("bitwise" if same else "MISMATCH") + f" ({compared} requests compared)",
+        }
     return {
         "workload": (f"Poisson arrivals {rate}/s for {duration} s ({n} requests), square grids side log-uniform "
-                     f"[64, 2048], half dam breaks / half terrains, f64 RK2, t_end U[{t_range[0]}, {t_range[1]}] s "
-                     f"(the full workload's U[10, 600] s scaled down to keep the run short), HBM budget {budget_gb} GB"),
+                     f"[64, 2048], half dam breaks / half terrains, f64 RK2, t_end U[{t_range[0]}, {t_range[1]}] s, "
+                     f"HBM budget {budget_gb} GB"),
         "requests": n,
         "cell_updates_total": total,
         "offered_cell_updates_per_s": total / duration,
+        "offered_over_served_capacity": (total / duration) / served_rate,
         "served": {"p50_s": pct(lat, 50), "p99_s": pct(lat, 99), "queue_wait_p50_s": pct(wait, 50),
-                   "queue_wait_p99_s": pct(wait, 99), "cell_updates_per_s": total / span, "makespan_s": span,
+                   "queue_wait_p99_s": pct(wait, 99), "on_device_p50_s": pct(on_dev, 50),
+                   "on_device_p99_s": pct(on_dev, 99), "cell_updates_per_s": served_rate, "makespan_s": span,
                    "busy_s": st["busy_s"], "host_admit_s": st["host_admit_s"], "host_retire_s": st["host_retire_s"],
-                   "peak_bytes": st["bytes_peak"], "replays": st["replays"]},
+                   "peak_bytes": st["bytes_peak"], "budget_bytes": budget, "replays": st["replays"]},
         "plain_fifo": {"p50_s": pct(plain_lat, 50), "p99_s": pct(plain_lat, 99),
                        "cell_updates_per_s": total / plain_span, "makespan_s": plain_span,
                        "service_s_total": float(sum(service))},
-        "throughput_ratio": (total / span) / (total / plain_span),
+        "throughput_ratio": served_rate / plain_rate,
         "p99_ratio": pct(plain_lat, 99) / pct(lat, 99),
-        "parity_served_vs_alone": ("bitwise" if same else "MISMATCH") + f" ({len(solo_states)} requests compared)",
+        "parity_served_vs_alone": ("bitwise" if same else "MISMATCH") + f" ({compared} requests compared)",
     }
 
 
@@ -631,6 +714,8 @@ def main():
     ap.add_argument("--serving-duration", type=float, default=1.0, help="seconds of arrivals")
     ap.add_argument("--serving-tend", type=float, nargs=2, default=[1.0, 30.0], help="t_end range of a request (s)")
     ap.add_argument("--serving-budget-gb", type=float, default=64.0)
+    ap.add_argument("--serving-cap", type=float, default=0.0,
+                    help="stop each serving path this many seconds after the first arrival (0: run to completion)")
     ap.add_argument("--no-f32", dest="f32_leg", action="store_false",
                     help="skip the Float32-storage run of the same workload")
     ap.add_argument("--adjoint-size", type=int, default=512, help="adjoint leg grid side (0: skip)")
@@ -733,7 +818,7 @@ def main():
         result["ensemble"] = ensemble_bench(args.ensemble, args.ensemble_steps, args.seed)
     if rank == 0 and args.serving_rate > 0:
         result["serving"] = serving_bench(args.serving_rate, args.serving_duration, tuple(args.serving_tend),
-                                          args.serving_budget_gb, args.seed)
+                                          args.serving_budget_gb, args.seed, args.serving_cap)
     if rank == 0 and args.adjoint_size > 0:
         result["adjoint"] = adjoint_bench(args.seed, args.adjoint_size)
     if rank == 0 and args.variant_steps > 0:
