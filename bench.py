@@ -396,10 +396,10 @@ def serving_bench(rate: float, duration: float, t_range, budget_gb: float, seed:
                          f"[64, 2048], half dam breaks / half terrains, f64 RK2, t_end U[{t_range[0]}, {t_range[1]}] s, "
                          f"HBM budget {budget_gb} GB; each path stopped {cap_s:.0f} s after the first arrival"),
             "requests": n,
-            "load": (f"the requests need more than one GPU can deliver in {cap_s:.0f} s: the served path finished "
-                     f"{len(infos)} and the plain FIFO path {plain_done} of {n}; offered cell-updates per second "
-                     f"of arrivals are at least {served_cells / duration:.3g} (what the server delivered, divided by "
-                     f"the {duration} s arrival window), i.e. at least {span / duration:.0f}x the served rate"),
+            "load": (f"offered cell-updates per second of arrivals: {'' if unfinished == 0 else 'at least '}"
+                     f"{served_cells / duration:.3g} (the cell-updates the requests{'' if unfinished == 0 else ' finished or in progress'}"
+                     f" needed, over the {duration} s arrival window), {span / duration:.0f}x the served rate; in "
+                     f"{cap_s:.0f} s the served path finished {len(infos)} and the plain FIFO path {plain_done} of {n}"),
             "served": {"finished": len(infos), "unfinished": unfinished,
                        "cell_updates_per_s": served_rate, "cell_updates": served_cells,
                        "p50_s_finished": pct(lat, 50), "p99_s_finished": pct(lat, 99),
