@@ -56,6 +56,7 @@ def test_config_defaults_mirror_reference():
 
 @pytest.mark.parametrize("field,value,code", [
     ("nx", 0, C.SFV_ERR_DOMAIN),
+    ("nx", (1 << 27) - 16, C.SFV_ERR_DOMAIN),   # rows are addressed by 31-bit byte offsets
     ("gc", 0, C.SFV_ERR_DOMAIN),
     ("gc", 1, C.SFV_ERR_ASSERTION),          # LinearReconstruction needs gc > 1 (linearreconstruction.jl:59)
     ("bc", 7, C.SFV_ERR_ARGUMENT),

@@ -224,6 +224,8 @@ int validate_config(const sfv_config* c, std::string& why) {
     if (!c) { why = "config is NULL"; return SFV_ERR_ARGUMENT; }
     if (c->nx < 1 || c->ny < 1) { why = "nx and ny must be >= 1"; return SFV_ERR_DOMAIN; }
     if (c->gc < 1 || c->gc > MAX_GC) { why = "gc must be in [1, 8]"; return SFV_ERR_DOMAIN; }
+    // the kernels address within a row by 31-bit byte offsets
+    if (c->nx > (1 << 27) - 2 * MAX_GC - 1) { why = "nx must be at most 2^27 - 17"; return SFV_ERR_DOMAIN; }
     if (c->reconstruction == SFV_LINEAR_RECONSTRUCTION && !(c->gc > 1)) {
         why = "LinearReconstruction needs grid.ghostcells > 1 (linearreconstruction.jl:59)";
         return SFV_ERR_ASSERTION;

@@ -171,6 +171,41 @@ __device__ __forceinline__ double Bload(const KParams& P, const double* __restri
     return (i <= P.nxT && j <= P.nyT) ? B[(size_t)i + (size_t)(P.nxT + 1) * j] : 0.0;
 }
 
+// Row access through a raw buffer resource: the descriptor (SGPRs) spans one
+// row, base at its first element and `len` elements long, and a lane's
+// access is its 32-bit byte offset (a loop-invariant VGPR).  That keeps the
+// per-lane 64-bit address arithmetic of a global access out of the row loop,
+// and the hardware range check reads zero for (and drops a store to) an
+// offset past the row.  validate_config caps a row at 2^27 elements, so the
+// byte offsets and lengths fit in 31 bits.  The row pointer must be the same for the whole wave.
+constexpr int RSRC_WORD3 = 0x00020000;   // gfx9 raw buffer: 32-bit data format
+template <class T>
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t row_rsrc(const T* row, int len) {
+    return __builtin_amdgcn_make_buffer_rsrc((void*)row, (short)0, len * (int)sizeof(T), RSRC_WORD3);
+}
+__device__ __forceinline__ double rsrc_ld(__amdgpu_buffer_rsrc_t r, unsigned off, double) {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+}
+__device__ __forceinline__ float rsrc_ld(__amdgpu_buffer_rsrc_t r, unsigned off, float) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+}
+template <class T>
+__device__ __forceinline__ T row_ld(const T* row, int len, int col) {
+    return rsrc_ld(row_rsrc(row, len), (unsigned)col * (unsigned)sizeof(T), T());
+}
+// Bload for a row j that is the same for the whole wave (the descriptor's
+// length does Bload's range check: zero past column nxT or row nyT)
+__device__ __forceinline__ double Bload_row(const KParams& P, const double* __restrict__ B, int i, int j) {
+    return row_ld(B + (size_t)(P.nxT + 1) * j, j <= P.nyT ? P.nxT + 1 : 0, i);
+}
+__device__ __forceinline__ void row_st(double* row, int len, int col, double v) {
+    using V = decltype(__builtin_amdgcn_raw_buffer_load_b64(__amdgpu_buffer_rsrc_t(), 0u, 0, 0));
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(V, v), row_rsrc(row, len), (unsigned)col * 8u, 0, 0);
+}
+__device__ __forceinline__ void row_st(float* row, int len, int col, float v) {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), row_rsrc(row, len), (unsigned)col * 4u, 0, 0);
+}
+
 // Stage-1 result of one cell (rungekutta.jl:38-41 with the Float64 dt,
 // friction.jl:80-84 from U0 with the stored dt, timestepper.jl:53-63).
 // FAST: the friction law is none or bh2021 (march_fast).
@@ -284,24 +319,26 @@ __device__ __forceinline__ void march_tile(const KParams& P, StepCtl* ctl, int s
             if (jr < gc) { sj = P.src_lo_y[jr]; o.fy = P.flip_lo_y[jr]; }
             else if (jr >= nyT - gc) { sj = P.src_hi_y[jr - (nyT - gc)]; o.fy = P.flip_hi_y[jr - (nyT - gc)]; }
         }
-        const size_t k = (size_t)sci + (size_t)nxT * sj;
-        o.x[0] = U0[k]; o.x[1] = U0[n + k]; o.x[2] = U0[2 * n + k];
+        const size_t ro = (size_t)nxT * sj;       // row sj (uniform), column sci (per lane)
+        o.x[0] = row_ld(U0 + ro, nxT, sci); o.x[1] = row_ld(U0 + n + ro, nxT, sci);
+        o.x[2] = row_ld(U0 + 2 * n + ro, nxT, sci);
         if (STAGE == 1) {
             if (xedge && fxc) o.x[1] = (S)(-o.x[1]);
             if (o.fy) o.x[2] = (S)(-o.x[2]);
         }
         if (STAGE == 2) {
-            o.r[0] = RHS1[k]; o.r[1] = RHS1[n + k]; o.r[2] = RHS1[2 * n + k];
+            o.r[0] = row_ld(RHS1 + ro, nxT, sci); o.r[1] = row_ld(RHS1 + n + ro, nxT, sci);
+            o.r[2] = row_ld(RHS1 + 2 * n + ro, nxT, sci);
             o.remap = (xedge && sci != i) || sj != jr;
             if (o.remap)   // the source cell's bottom
-                o.bsrc = bc4(Bload(P, B, sci, sj), Bload(P, B, sci + 1, sj), Bload(P, B, sci, sj + 1),
-                                Bload(P, B, sci + 1, sj + 1));
+                o.bsrc = bc4(Bload_row(P, B, sci, sj), Bload_row(P, B, sci + 1, sj), Bload_row(P, B, sci, sj + 1),
+                             Bload_row(P, B, sci + 1, sj + 1));
         }
     };
     // bottom corner row: own column, and (lane 63 only) the next column
     auto fetch_b = [&](int row, double& own, double& extra) {
-        own = in_dom ? Bload(P, B, i, row) : 0.0;
-        extra = (lane == MW - 1 && in_dom) ? Bload(P, B, i + 1, row) : 0.0;
+        own = in_dom ? Bload_row(P, B, i, row) : 0.0;
+        extra = (lane == MW - 1 && in_dom) ? Bload_row(P, B, i + 1, row) : 0.0;
     };
     auto next_col = [&](double own, double extra) {
         const double v = shfl_dn1(own);
@@ -485,7 +522,8 @@ __device__ __forceinline__ void march_tile(const KParams& P, StepCtl* ctl, int s
         // 4. finish row p = jr-2: y part on top of its x part, then sources
         if ((FULL || r >= 4) && out_col) {
             const int p = jr - 2;
-            const size_t k = (size_t)i + (size_t)nxT * p;
+            const size_t k = (size_t)i + (size_t)nxT * p;     // for the sources' per-cell tables
+            S* orow = OUT + (size_t)nxT * p;                   // row p (uniform), column i (per lane)
             S acc[3] = {accx[0], accx[1], accx[2]};
 #pragma unroll
             for (int v = 0; v < 3; ++v) acc[v] = (S)(acc[v] - P.inv_dy * (Fy[v] - Fyp[v]));
@@ -495,9 +533,9 @@ __device__ __forceinline__ void march_tile(const KParams& P, StepCtl* ctl, int s
             }
             acc[0] = sources(acc[0], k);
             if (STAGE == 1) {
-                OUT[k] = acc[0];
-                OUT[n + k] = acc[1];
-                OUT[2 * n + k] = acc[2];
+                row_st(orow, nxT, i, acc[0]);
+                row_st(orow + n, nxT, i, acc[1]);
+                row_st(orow + 2 * n, nxT, i, acc[2]);
             } else {
                 // cA is U1 of row p
                 S o[3];
@@ -522,9 +560,9 @@ __device__ __forceinline__ void march_tile(const KParams& P, StepCtl* ctl, int s
                 o[0] = dry ? wd : o[0];
                 o[1] = dry ? (S)0.0 : o[1];
                 o[2] = dry ? (S)0.0 : o[2];
-                OUT[k] = o[0];
-                OUT[n + k] = o[1];
-                OUT[2 * n + k] = o[2];
+                row_st(orow, nxT, i, o[0]);
+                row_st(orow + n, nxT, i, o[1]);
+                row_st(orow + 2 * n, nxT, i, o[2]);
             }
         }
         // shift the windows
@@ -649,7 +687,8 @@ k_march(KParams P, StepCtl* ctl, int slot, int my, int strips, int seggroups, co
 #else
     const int strip = t / seggroups, sg = t % seggroups;
 #endif
-    const int seg = sg * MWAVES + threadIdx.x / MW;
+    // the same for every lane of the wave; readfirstlane tells the compiler so
+    const int seg = __builtin_amdgcn_readfirstlane(sg * MWAVES + threadIdx.x / MW);
     if (STAGE == 1 && remap && march_edge(P, my, strip, seg))
         march_tile<S, 1, FAST, false, true>(P, ctl, slot, my, strip, seg, U0, RHS1, OUT, B);
     else
@@ -678,7 +717,8 @@ k_march_batch(const BatchMember* __restrict__ mem, const BatchHeader* __restrict
         const int local = t - M.tile0;
         const int strip = local % M.strips, sg = local / M.strips;
         const int cur = (M.cur0 ^ lp) & 1, slot = (M.slot0 ^ lp) & 1;
-        const int seg = sg * MWAVES + threadIdx.x / MW;
+        // the same for every lane of the wave; readfirstlane tells the compiler so
+    const int seg = __builtin_amdgcn_readfirstlane(sg * MWAVES + threadIdx.x / MW);
         const S* U0 = (const S*)M.U[cur];
         if (STAGE == 1)
             if (march_edge(M.P, M.my, strip, seg))
