@@ -906,17 +906,23 @@ int march_step(sfv_handle* h, int slot, const double* U0, double* U2) {
     return SFV_OK;
 }
 
-// One forward step U0 -> U2 with the per-pass kernels (bitwise the fused
-// path), keeping the stage-1 state and both right-hand sides.
+// One forward step U0 -> U2 (bitwise the fused path), keeping the stage-1
+// state, both right-hand sides at every cell and, in ctl_out, the step's
+// control block as stage 1 leaves it (its dt and wave-speed maxima).  The
+// right-hand sides come from the march stage-1 kernel, which reads the
+// ghost cells the boundary condition filled and writes interior cells; the
+// finishing kernel stores the ghost cells' (sources only) beside them.  The
+// second march launch raises the slot's maxima again, after the copy.
 int forward_step(sfv_handle* h, hipStream_t st, double t, const double* U0, double* U1, double* R1, double* R2,
-                 double* U2) {
+                 double* U2, StepCtl* ctl_out) {
     const KParams& P = h->P;
     ATRY(launch_prepare(st, P, h->ctl, 0, t, NAN, 0, 0, 0.0, 1, 0, SFV_F64));
-    ATRY(launch_flux<double>(st, P, h->ctl, 0, U0, R1, h->B, 1));
-    ATRY(launch_finish<double>(st, P, h->ctl, 0, 1, U0, U0, R1, U1, h->B));
+    ATRY(launch_march<double>(st, P, h->ctl, 0, 1, U0, nullptr, R1, h->B));
+    ATRY(launch_finish<double>(st, P, h->ctl, 0, 1, U0, U0, R1, U1, h->B, 1, 0, R1));
+    ATRY(hipMemcpyAsync(ctl_out, h->ctl, sizeof(StepCtl), hipMemcpyDeviceToDevice, st));
     ATRY(launch_bc<double>(st, P, h->ctl, 0, U1, BC_IN_STEP));
-    ATRY(launch_flux<double>(st, P, h->ctl, 0, U1, R2, h->B, 0));
-    ATRY(launch_finish<double>(st, P, h->ctl, 0, 2, U0, U1, R2, U2, h->B));
+    ATRY(launch_march<double>(st, P, h->ctl, 0, 1, U1, nullptr, R2, h->B));
+    ATRY(launch_finish<double>(st, P, h->ctl, 0, 2, U0, U1, R2, U2, h->B, 1, 0, R2));
     ATRY(launch_bc<double>(st, P, h->ctl, 0, U2, BC_LAST_STAGE));
     return SFV_OK;
 }
@@ -971,6 +977,8 @@ extern "C" int sfv_adjoint(sfv_handle* h, int64_t nsteps, int32_t checkpoint_eve
     unsigned long long* best = (unsigned long long*)buf.alloc(1, why, rc);
     double* d_times = buf.alloc((size_t)nsteps + 1, why, rc);
     int* flag = (int*)buf.alloc(1, why, rc);
+    // the control block of the step being recomputed, without kept blocks
+    StepCtl* step_ctl = (StepCtl*)buf.alloc(sizeof(StepCtl) / sizeof(double) + 1, why, rc);
     // The stage-1 states, right-hand sides and step controls of a segment are
     // kept from its recompute when they fit in half the free device memory;
     // otherwise each reverse step recomputes its own.
@@ -1058,10 +1066,9 @@ extern "C" int sfv_adjoint(sfv_handle* h, int64_t nsteps, int32_t checkpoint_eve
             int r;
             if (keep) {
                 if ((r = forward_step(h, rs, times[(size_t)s], a, kU1s[set] + o, kR1s[set] + o, kR2s[set] + o,
-                                      a + n3)))
+                                      a + n3, kctls[set] + (s - s0))))
                     return r;
-                ATRY(hipMemcpyAsync(kctls[set] + (s - s0), h->ctl, sizeof(StepCtl), hipMemcpyDeviceToDevice, rs));
-            } else if ((r = forward_step(h, rs, times[(size_t)s], a, U1, R1, R2, a + n3))) {
+            } else if ((r = forward_step(h, rs, times[(size_t)s], a, U1, R1, R2, a + n3, step_ctl))) {
                 return r;
             }
         }
@@ -1074,14 +1081,14 @@ extern "C" int sfv_adjoint(sfv_handle* h, int64_t nsteps, int32_t checkpoint_eve
             const size_t o = (size_t)(s - s0) * n3;
             const double* U0 = segs[set] + o;
             const double *u1 = U1, *r1 = R1, *r2 = R2;
-            const StepCtl* cs = h->ctl;                 // the step's control block (its dt and maxima)
+            const StepCtl* cs = step_ctl;               // the step's control block (its dt and maxima)
             if (keep) {
                 u1 = kU1s[set] + o;
                 r1 = kR1s[set] + o;
                 r2 = kR2s[set] + o;
                 cs = kctls[set] + (s - s0);
             } else {
-                const int r = forward_step(h, st, times[(size_t)s], U0, U1, R1, R2, Ut);
+                const int r = forward_step(h, st, times[(size_t)s], U0, U1, R1, R2, Ut, step_ctl);
                 if (r) return r;
             }
             // stage 2: U2 = clamp(friction(0.5 (U0 + U1 + dt R2))); the tail

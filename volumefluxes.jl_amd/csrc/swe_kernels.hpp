@@ -10,7 +10,7 @@ template <class S>
 __global__ void k_flux(KParams P, StepCtl* ctl, int slot, const S* X, S* RHS, const double* B, int record);
 template <class S>
 __global__ void k_finish(KParams P, StepCtl* ctl, int slot, int stage, const S* U0, const S* X, const S* RHS, S* Out,
-                         const double* B);
+                         const double* B, int rhs_interior, int close, S* ghost_rhs);
 
 // ---- boundary condition and step control (swe_kernels_ctl.hip)
 template <class S>
@@ -39,7 +39,7 @@ hipError_t launch_flux(hipStream_t st, const KParams& P, StepCtl* ctl, int slot,
 template <class S>
 hipError_t launch_finish(hipStream_t st, const KParams& P, StepCtl* ctl, int slot, int stage, const S* U0, const S* X,
                          const S* RHS, S* Out, const double* B,
-                         int rhs_interior = 0, int close = 0);
+                         int rhs_interior = 0, int close = 0, S* ghost_rhs = nullptr);
 template <class S>
 hipError_t launch_bc(hipStream_t st, const KParams& P, StepCtl* ctl, int slot, S* U, int mode);
 hipError_t launch_prepare(hipStream_t st, const KParams& P, StepCtl* ctl, int slot, double t, double endtime,

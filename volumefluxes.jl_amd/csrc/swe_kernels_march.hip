@@ -42,7 +42,7 @@ namespace sfv {
 //   SFV_MARCH_XCD         block order: 1 = each XCD gets a contiguous run of tiles,
 //                         segments fastest (vertical halos stay in one L2);
 //                         2 = same, strips fastest; 0 = hardware order
-//   SFV_MARCH_UNROLL      unroll factor of the row loop
+//   SFV_MARCH_UNROLL      unroll factor of the row loop (SFV_MARCH_UNROLL1 / 2: per stage)
 //   SFV_MARCH_KEEPU0      stage 2 keeps U0 of the last two rows in registers instead of reloading it
 //   SFV_MARCH_FAST        specialised kernels for the common configuration (march_fast)
 #ifndef SFV_MARCH_PREFETCH
@@ -80,6 +80,12 @@ namespace sfv {
 #endif
 #ifndef SFV_MARCH_UNROLL
 #define SFV_MARCH_UNROLL 1
+#endif
+#ifndef SFV_MARCH_UNROLL1
+#define SFV_MARCH_UNROLL1 SFV_MARCH_UNROLL
+#endif
+#ifndef SFV_MARCH_UNROLL2
+#define SFV_MARCH_UNROLL2 SFV_MARCH_UNROLL
 #endif
 
 
@@ -592,9 +598,9 @@ __device__ __forceinline__ void march_tile(const KParams& P, StepCtl* ctl, int s
     // The row loop is unrolled by hand (its cross-lane shifts are convergent
     // operations, which the compiler will not unroll with a runtime trip count).
     int r = 4;
-    if (SFV_MARCH_UNROLL > 1)
-        for (; r + SFV_MARCH_UNROLL <= th + 3; r += SFV_MARCH_UNROLL)
-            unrolled_rows(body, r, std::make_index_sequence<SFV_MARCH_UNROLL>{});
+    constexpr int UNROLL = STAGE == 1 ? SFV_MARCH_UNROLL1 : SFV_MARCH_UNROLL2;
+    if (UNROLL > 1)
+        for (; r + UNROLL <= th + 3; r += UNROLL) unrolled_rows(body, r, std::make_index_sequence<UNROLL>{});
     for (; r < th + 3; ++r) body(r, std::true_type{}, std::true_type{});
     body(th + 3, std::true_type{}, std::false_type{});
 

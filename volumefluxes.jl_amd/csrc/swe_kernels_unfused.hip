@@ -118,12 +118,14 @@ __global__ void __launch_bounds__(256) k_flux(KParams P, StepCtl* ctl, int slot,
 // (simulator.jl:147-151); stage 2 averages with U0.  With rhs_interior set,
 // RHS holds only the interior cells (the march stage-1 kernel's output) and
 // the other cells take their right-hand side as k_flux gives it there: the
-// non-directional sources on w, zero on hu and hv.
+// non-directional sources on w, zero on hu and hv; with ghost_rhs given,
+// that right-hand side is also stored there (ghost_rhs may be RHS itself:
+// RHS is read only at interior cells, ghost_rhs written only at the others).
 template <class S>
 __global__ void __launch_bounds__(256) k_finish(KParams P, StepCtl* ctl, int slot, int stage, const S* __restrict__ U0,
-                                                const S* __restrict__ X, const S* __restrict__ RHS,
-                                                S* __restrict__ Out, const double* __restrict__ B, int rhs_interior,
-                                                int close) {
+                                                const S* __restrict__ X, const S* RHS, S* __restrict__ Out,
+                                                const double* __restrict__ B, int rhs_interior, int close,
+                                                S* ghost_rhs) {
     const StepSlot& sl = ctl->slot[slot];
     const bool closer = close && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0 && threadIdx.y == 0;
     if (!sl.active) {
@@ -159,6 +161,9 @@ __global__ void __launch_bounds__(256) k_finish(KParams P, StepCtl* ctl, int slo
         r[0] = apply_sources(P, ctl->slot[slot], S(0), k);
         r[1] = S(0);
         r[2] = S(0);
+        if (ghost_rhs)
+#pragma unroll
+            for (int v = 0; v < 3; ++v) ghost_rhs[v * n + k] = r[v];
     } else {
 #pragma unroll
         for (int v = 0; v < 3; ++v) r[v] = RHS[v * n + k];
@@ -197,11 +202,11 @@ hipError_t launch_flux(hipStream_t st, const KParams& P, StepCtl* ctl, int slot,
 }
 template <class S>
 hipError_t launch_finish(hipStream_t st, const KParams& P, StepCtl* ctl, int slot, int stage, const S* U0, const S* X,
-                         const S* RHS, S* Out, const double* B, int rhs_interior, int close) {
+                         const S* RHS, S* Out, const double* B, int rhs_interior, int close, S* ghost_rhs) {
     const dim3 block(64, 4);
     const dim3 grid((P.nxT + 63) / 64, (P.nyT + 3) / 4);
     hipLaunchKernelGGL(k_finish<S>, grid, block, 0, st, P, ctl, slot, stage, U0, X, RHS, Out, B, rhs_interior,
-                       close);
+                       close, ghost_rhs);
     return hipPeekAtLastError();
 }
 template hipError_t launch_flux<double>(hipStream_t, const KParams&, StepCtl*, int, const double*, double*,
@@ -209,15 +214,15 @@ template hipError_t launch_flux<double>(hipStream_t, const KParams&, StepCtl*, i
 template hipError_t launch_flux<float>(hipStream_t, const KParams&, StepCtl*, int, const float*, float*,
                                        const double*, int);
 template hipError_t launch_finish<double>(hipStream_t, const KParams&, StepCtl*, int, int, const double*,
-                                          const double*, const double*, double*, const double*, int, int);
+                                          const double*, const double*, double*, const double*, int, int, double*);
 template hipError_t launch_finish<float>(hipStream_t, const KParams&, StepCtl*, int, int, const float*, const float*,
-                                         const float*, float*, const double*, int, int);
+                                         const float*, float*, const double*, int, int, float*);
 
 template __global__ void k_flux<double>(KParams, StepCtl*, int, const double*, double*, const double*, int);
 template __global__ void k_flux<float>(KParams, StepCtl*, int, const float*, float*, const double*, int);
 template __global__ void k_finish<double>(KParams, StepCtl*, int, int, const double*, const double*, const double*,
-                                          double*, const double*, int, int);
+                                          double*, const double*, int, int, double*);
 template __global__ void k_finish<float>(KParams, StepCtl*, int, int, const float*, const float*, const float*,
-                                         float*, const double*, int, int);
+                                         float*, const double*, int, int, float*);
 
 }  // namespace sfv
