@@ -43,6 +43,7 @@ namespace sfv {
 //                         segments fastest (vertical halos stay in one L2);
 //                         2 = same, strips fastest; 0 = hardware order
 //   SFV_MARCH_UNROLL      unroll factor of the row loop (SFV_MARCH_UNROLL1 / 2: per stage)
+//   SFV_MARCH_STARTFETCH  rows (0..4) whose loads a wave issues together before its first row
 //   SFV_MARCH_KEEPU0      stage 2 keeps U0 of the last two rows in registers instead of reloading it
 //   SFV_MARCH_FAST        specialised kernels for the common configuration (march_fast)
 #ifndef SFV_MARCH_PREFETCH
@@ -80,6 +81,9 @@ namespace sfv {
 #endif
 #ifndef SFV_MARCH_UNROLL
 #define SFV_MARCH_UNROLL 1
+#endif
+#ifndef SFV_MARCH_STARTFETCH
+#define SFV_MARCH_STARTFETCH 3
 #endif
 #ifndef SFV_MARCH_UNROLL1
 #define SFV_MARCH_UNROLL1 SFV_MARCH_UNROLL
@@ -311,15 +315,15 @@ __device__ __forceinline__ void march_tile(const KParams& P, StepCtl* ctl, int s
         int fy;          // hv negated (stage 2, ghost row of a wall)
         bool remap;
     };
+    // Every lane loads: a lane past the grid (i >= nxT, so sci = i) reads
+    // zeros through the row descriptors' range check.  No branch surrounds
+    // the loads, so the loads of several rows can be in flight together
+    // (a load under a partial exec mask must complete before other lanes
+    // write its registers).
     auto fetch = [&](int jr, Raw& o) {
         o.remap = false;
         o.fy = 0;
         o.bsrc = 0.0;
-        if (!in_dom) {
-            o.x[0] = o.x[1] = o.x[2] = S(0);
-            o.r[0] = o.r[1] = o.r[2] = S(0);
-            return;
-        }
         int sj = jr;
         if (yedge) {
             if (jr < gc) { sj = P.src_lo_y[jr]; o.fy = P.flip_lo_y[jr]; }
@@ -329,13 +333,15 @@ __device__ __forceinline__ void march_tile(const KParams& P, StepCtl* ctl, int s
         o.x[0] = row_ld(U0 + ro, nxT, sci); o.x[1] = row_ld(U0 + n + ro, nxT, sci);
         o.x[2] = row_ld(U0 + 2 * n + ro, nxT, sci);
         if (STAGE == 1) {
+            // a lane past the grid may take -0.0 here; it feeds no output
+            // and no recorded face
             if (xedge && fxc) o.x[1] = (S)(-o.x[1]);
             if (o.fy) o.x[2] = (S)(-o.x[2]);
         }
         if (STAGE == 2) {
             o.r[0] = row_ld(RHS1 + ro, nxT, sci); o.r[1] = row_ld(RHS1 + n + ro, nxT, sci);
             o.r[2] = row_ld(RHS1 + 2 * n + ro, nxT, sci);
-            o.remap = (xedge && sci != i) || sj != jr;
+            o.remap = in_dom && ((xedge && sci != i) || sj != jr);
             if (o.remap)   // the source cell's bottom
                 o.bsrc = bc4(Bload_row(P, B, sci, sj), Bload_row(P, B, sci + 1, sj), Bload_row(P, B, sci, sj + 1),
                              Bload_row(P, B, sci + 1, sj + 1));
@@ -343,8 +349,9 @@ __device__ __forceinline__ void march_tile(const KParams& P, StepCtl* ctl, int s
     };
     // bottom corner row: own column, and (lane 63 only) the next column
     auto fetch_b = [&](int row, double& own, double& extra) {
-        own = in_dom ? Bload_row(P, B, i, row) : 0.0;
-        extra = (lane == MW - 1 && in_dom) ? Bload_row(P, B, i + 1, row) : 0.0;
+        const double a = Bload_row(P, B, i, row), b = Bload_row(P, B, i + 1, row);   // all lanes, as above
+        own = in_dom ? a : 0.0;
+        extra = (lane == MW - 1 && in_dom) ? b : 0.0;
     };
     auto next_col = [&](double own, double extra) {
         const double v = shfl_dn1(own);
@@ -359,6 +366,18 @@ __device__ __forceinline__ void march_tile(const KParams& P, StepCtl* ctl, int s
     np1 = next_col(bp1, e1_);
     Raw cur, nxt;
     if (PREFETCH) fetch(j0 - 2, cur);
+    // The first NPRE rows' inputs and the corner rows read with them are
+    // loaded together, so a wave waits for memory once before its first row
+    // rather than once per row (these loads are the ones the peeled rows
+    // r < NPRE would issue: rows j0-2+r and corner rows j0+r, r <= 3).
+    constexpr int NPRE = PREFETCH ? 0 : SFV_MARCH_STARTFETCH;
+    Raw pre[NPRE > 0 ? NPRE : 1];
+    double preb[NPRE > 0 ? NPRE : 1], pree[NPRE > 0 ? NPRE : 1];
+#pragma unroll
+    for (int k = 0; k < NPRE; ++k) {
+        fetch(j0 - 2 + k, pre[k]);
+        fetch_b(j0 + k, preb[k], pree[k]);
+    }
     // ranged face evaluation (central_upwind_rng) while the wave's inputs allow it
     bool band = P.ranged && __all(in_band(b0) && in_band(e0_) && in_band(bp1) && in_band(e1_));
 
@@ -388,9 +407,17 @@ __device__ __forceinline__ void march_tile(const KParams& P, StepCtl* ctl, int s
         const bool xrow = FULL ? decltype(x_tag)::value : r == 3;
         const bool yface = FULL || r >= 3;
         double nb, ne;
-        fetch_b(jr + 2, nb, ne);                   // next corner row (Bload bounds-checks row nyT+1)
+        const bool pre_r = !FULL && r < NPRE;      // a peeled row (r constant) loaded up front
+        if (pre_r) {
+            nb = preb[r];
+            ne = pree[r];
+        } else {
+            fetch_b(jr + 2, nb, ne);               // next corner row (Bload bounds-checks row nyT+1)
+        }
         if (PREFETCH) {
             if (r + 1 < th + 4) fetch(jr + 1, nxt);    // rows up to j0+th+1 <= nyT-1 (gc >= 2)
+        } else if (pre_r) {
+            cur = pre[r];
         } else {
             fetch(jr, cur);
         }
