@@ -168,44 +168,50 @@ def main(asm_path, csrc):
                 inner = (s_, e_, name)
         return inner[2] if inner else None
     for stage, prefix in KERNELS.items():
-        start = next(i for i, l in enumerate(text) if l.startswith(prefix) and l.rstrip().endswith(":")
-                     or (l.startswith(prefix) and ":" in l.split(";")[0]))
-        end = next(i for i in range(start, len(text)) if "s_endpgm" in text[i])
-        labels, loops = {}, []
+        start = next(i for i, l in enumerate(text) if l.startswith(prefix) and ":" in l.split(";")[0])
+        # the function runs to its end label: cold blocks follow the first s_endpgm
+        end = next(i for i in range(start, len(text)) if text[i].startswith(".Lfunc_end"))
+        # blocks of each outermost loop, from the compiler's loop annotations
+        # ("Loop Header: Depth=1", "in Loop: Header=BBx_y Depth=1"); a block of
+        # a nested loop belongs to the outermost loop around it
+        parent, members = {}, collections.defaultdict(list)
+        cur = None
+        loc_at, chain_at = {}, {}
+        loc, chain = (None, 0), []
         for i in range(start, end):
-            m = re.match(r"^(\.LBB\w+):", text[i])
+            line = text[i]
+            m = re.match(r"\s*\.loc\s+(\d+)\s+(\d+)", line)
+            if m:                          # the line-table entry in force, in layout order
+                loc, chain = (int(m.group(1)), int(m.group(2))), frames_of(line)
+            loc_at[i], chain_at[i] = loc, chain
+            m = re.match(r"^(?:\.L(BB\w+):|; %bb\.(\d+):)(.*)", line)
             if m:
-                labels[m.group(1)] = i
-            m = re.match(r"\s+s_(?:c)?branch\w*\s+(\.LBB\w+)", text[i])
-            if m and m.group(1) in labels:
-                loops.append((labels[m.group(1)], i))
-        # a loop region: the union of overlapping back-edge spans (a loop has
-        # several latches when its body ends in divergent branches)
-        chosen = []
-        for a, b in sorted(loops):
-            if chosen and a <= chosen[-1][1]:
-                chosen[-1] = (chosen[-1][0], max(chosen[-1][1], b))
-            else:
-                chosen.append((a, b))
-        for li, (a, b) in enumerate(c for c in chosen if c[1] - c[0] > 400):
-            loc, chain = (None, 0), []
+                name = m.group(1) or None
+                note = m.group(3)
+                h = re.search(r"Loop Header: Depth=(\d+)", note)
+                inl = re.search(r"in Loop: Header=(BB\w+) Depth=(\d+)", note)
+                if h and name:
+                    d = int(h.group(1))
+                    outer = cur if d > 1 else None
+                    parent[name] = parent.get(outer, outer) if outer else name
+                    cur = parent[name]
+                elif inl:
+                    cur = parent.get(inl.group(1), inl.group(1))
+                else:
+                    cur = None
+            if cur:
+                members[cur].append(i)
+        for li, (hdr, idx) in enumerate(sorted(members.items(), key=lambda kv: kv[1][0])):
+            if sum(1 for i in idx if re.match(r"\s+v_", text[i])) < 300:
+                continue
+            a, b = idx[0], idx[-1]
             by_phase = collections.Counter()
             by_helper = collections.Counter()
             cls = collections.Counter()
             ops = collections.Counter()
-            # the .loc in force at the loop's first line
-            for i in range(a, start, -1):
-                m = re.match(r"\s*\.loc\s+(\d+)\s+(\d+)", text[i])
-                if m:
-                    loc = (int(m.group(1)), int(m.group(2)))
-                    break
-            for i in range(a, b + 1):
+            for i in idx:
                 line = text[i]
-                m = re.match(r"\s*\.loc\s+(\d+)\s+(\d+)", line)
-                if m:
-                    loc = (int(m.group(1)), int(m.group(2)))
-                    chain = frames_of(line)
-                    continue
+                loc, chain = loc_at[i], chain_at[i]
                 m = re.match(r"\s+([sv]_\w+|buffer_\w+|global_\w+|scratch_\w+|ds_\w+)", line)
                 if not m:
                     continue
@@ -233,7 +239,7 @@ def main(asm_path, csrc):
                         break
                 by_phase[ph] += 1
             total = cls["VALU"]
-            print(f"{stage}, loop {li + 1} (assembly lines {a + 1}..{b + 1}): VALU {total}, SALU {cls['SALU']}, "
+            print(f"{stage}, loop at {hdr} (assembly lines {a + 1}..{b + 1}): VALU {total}, SALU {cls['SALU']}, "
                   f"VMEM {cls['VMEM']}, LDS {cls['LDS']}")
             print("  by phase (helpers counted in the phase that calls them):")
             for ph, c in by_phase.most_common():
