@@ -379,7 +379,7 @@ __device__ __forceinline__ void march_tile(const KParams& P, StepCtl* ctl, int s
         fetch_b(j0 + k, preb[k], pree[k]);
     }
     // ranged face evaluation (central_upwind_rng) while the wave's inputs allow it
-    bool band = P.ranged && __all(in_band(b0) && in_band(e0_) && in_band(bp1) && in_band(e1_));
+    bool band = P.ranged && __all(in_band5(b0, e0_, bp1, e1_, 0.0));
 
     S cA[3] = {S(0), S(0), S(0)}, cB[3] = {S(0), S(0), S(0)};
     // stage 2 with SFV_MARCH_KEEPU0: U0 of rows jr-2, jr-1 as loaded, for the
@@ -438,7 +438,7 @@ __device__ __forceinline__ void march_tile(const KParams& P, StepCtl* ctl, int s
         // corner this wave has taken in lies in band (central_upwind_rng);
         // the faces of this iteration read rows up to jr and corner rows up
         // to jr, the corner row jr+2 taken in here is read two rows later
-        band = band && __all(in_band(cC[0]) && in_band(cC[1]) && in_band(cC[2]) && in_band(nb) && in_band(ne));
+        band = band && __all(in_band5(cC[0], cC[1], cC[2], nb, ne));
 
         // 1. y reconstruction of row m = jr-1 (rows jr-2, jr-1, jr) and x
         //    reconstruction of row q = jr-1 (this lane's cell cB and its x

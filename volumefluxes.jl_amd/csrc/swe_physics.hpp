@@ -403,6 +403,21 @@ __device__ __forceinline__ bool in_band(double v) {
     const double a = __builtin_fabs(v);
     return a == 0.0 || (a >= 0x1p-100 && a <= 0x1p40);
 }
+// Every value 0 or in [2^-100, 2^40) in magnitude (a subset of in_band's
+// set), tested through exponents: the hardware's frexp exponent e (v = m 2^e
+// with |m| in [0.5, 1); 0 for 0, the true exponent for subnormals, 0 for Inf
+// and NaN) lies in [-99, 40] exactly for 0 and the magnitudes in
+// [2^-100, 2^40), and for Inf and NaN, which make the sum non-finite.  The
+// exponents combine with integer min/max, so the test costs one lane mask
+// instead of three per value.
+__device__ __forceinline__ bool in_band5(double a, double b, double c, double d, double e) {
+    const int ea = __builtin_amdgcn_frexp_exp(a), eb = __builtin_amdgcn_frexp_exp(b),
+              ec = __builtin_amdgcn_frexp_exp(c), ed = __builtin_amdgcn_frexp_exp(d),
+              ee = __builtin_amdgcn_frexp_exp(e);
+    const int lo = min(min(min(ea, eb), min(ec, ed)), ee), hi = max(max(max(ea, eb), max(ec, ed)), ee);
+    const double s = ((a + b) + (c + d)) + e;      // finite: no value is Inf or NaN (|v| < 2^40 cannot overflow it)
+    return (unsigned)(lo + 99) <= 139u && (unsigned)(hi + 99) <= 139u && __builtin_isfinite(s);
+}
 
 // swe_2D.jl:43-81 with the ranged quotients and square root (h > cutoff).
 template <class S>
