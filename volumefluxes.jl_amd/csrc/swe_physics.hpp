@@ -358,7 +358,8 @@ __device__ __forceinline__ double central_upwind_wet(const S m[3], const S q[3],
 // central_upwind_rng is central_upwind with these forms.  It is used only by
 // a wave whose stage inputs (w, hu, hv of every cell the face stencils read)
 // and bottom corners all lie in {0} U [2^-100, 2^40] in magnitude (checked as
-// the rows arrive, in_band), with θ (linear reconstruction), ρ and g in
+// the rows arrive, in_band5, whose set is the subset {0} U [2^-100, 2^40)),
+// with θ (linear reconstruction), ρ and g in
 // [2^-10, 2^10] and depth_cutoff and κ in [2^-39, 2^10] (KParams::ranged).
 // Then (exact differences of such values are multiples of 2^-152, rounded
 // products keep their exponent sums, and cancellation in u +- c leaves at
