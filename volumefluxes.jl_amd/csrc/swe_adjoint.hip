@@ -474,7 +474,8 @@ __global__ void __launch_bounds__(256) k_adj_tail(KParams P, const StepCtl* ctl,
                                                   double* __restrict__ part_dt, double* __restrict__ part_cz,
                                                   unsigned long long* __restrict__ best) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    // the launchers' blocks are 64 wide, so a wave shares threadIdx.y
+    const int j = __builtin_amdgcn_readfirstlane(blockIdx.y * blockDim.y + threadIdx.y);
     const int nblk = gridDim.x * gridDim.y, bid = blockIdx.x + gridDim.x * blockIdx.y;
     if (stage == 2 && bid == 0 && threadIdx.x == 0 && threadIdx.y == 0) *best = ~0ull;
     double gdt = 0.0, gcz = 0.0;
@@ -681,7 +682,8 @@ __global__ void __launch_bounds__(256) k_adj_recon(KParams P, const double* __re
                                                    const double* __restrict__ LfR1, double* __restrict__ LU,
                                                    double* __restrict__ LBf) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    // the launchers' blocks are 64 wide, so a wave shares threadIdx.y
+    const int j = __builtin_amdgcn_readfirstlane(blockIdx.y * blockDim.y + threadIdx.y);
     if (i >= P.nxT || j >= P.nyT) return;
     double ax[3] = {0.0, 0.0, 0.0}, ay[3] = {0.0, 0.0, 0.0};
     const bool hx = recon_dir(P, 0, i, j, U, B, LR, LfL0, LfR0, ax, LBf);
@@ -787,7 +789,8 @@ __global__ void __launch_bounds__(256) k_adj_objective(KParams P, const double* 
                                                        const double* __restrict__ target, double* __restrict__ L,
                                                        double* __restrict__ LBc, double* __restrict__ part) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    // the launchers' blocks are 64 wide, so a wave shares threadIdx.y
+    const int j = __builtin_amdgcn_readfirstlane(blockIdx.y * blockDim.y + threadIdx.y);
     double e2 = 0.0;
     if (i >= P.gc && i < P.nxT - P.gc && j >= P.gc && j < P.nyT - P.gc) {
         const size_t k = cid(P, i, j);

@@ -49,7 +49,8 @@ __global__ void __launch_bounds__(256) k_flux(KParams P, StepCtl* ctl, int slot,
                                               S* __restrict__ RHS, const double* __restrict__ B, int record) {
     if (!ctl->slot[slot].active) return;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    // the launchers' blocks are 64 wide, so a wave shares threadIdx.y
+    const int j = __builtin_amdgcn_readfirstlane(blockIdx.y * blockDim.y + threadIdx.y);
     const size_t n = (size_t)P.nxT * P.nyT;
     unsigned long long sx = 0, sy = 0;
     if (i < P.nxT && j < P.nyT) {
@@ -133,7 +134,8 @@ __global__ void __launch_bounds__(256) k_finish(KParams P, StepCtl* ctl, int slo
         return;
     }
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    // the launchers' blocks are 64 wide, so a wave shares threadIdx.y
+    const int j = __builtin_amdgcn_readfirstlane(blockIdx.y * blockDim.y + threadIdx.y);
     double dt_d;
     S dt_s;
     if (stage == 1) {
