@@ -70,7 +70,8 @@ enum sfv_layout { SFV_LAYOUT_FULL = 0, SFV_LAYOUT_INTERIOR = 1 };
  * dx, dy are grid.Δ as the reference computes them: (x1 - x0) / nx.
  */
 typedef struct sfv_config {
-    int32_t nx, ny, gc;
+    int32_t nx, ny, gc;            /* nx <= 2^27 - 17 (the kernels address a row with 31-bit
+                                      byte offsets), else SFV_ERR_DOMAIN */
     double dx, dy;
     int32_t bc;                    /* enum sfv_bc */
     double g, rho, depth_cutoff, desingularizing_kappa;
