@@ -58,7 +58,10 @@ hipError_t launch_prepare_batch(hipStream_t st, const BatchMember* mem, const Ba
 int march_batch_blocks_per_cu(int precision, int stage);
 int march_rows_per_wave(const KParams& P, int stage);
 constexpr int MARCH_COLS = 60;   // output columns per wave strip
-constexpr int MARCH_WAVES = 4;   // waves (segments) per workgroup
+#ifndef SFV_MARCH_WAVES
+#define SFV_MARCH_WAVES 4
+#endif
+constexpr int MARCH_WAVES = SFV_MARCH_WAVES;   // waves (segments) per workgroup
 
 // Plain copy of `bytes` (a multiple of 4) from src to dst.
 hipError_t launch_copy(hipStream_t st, const void* src, void* dst, size_t bytes);
