@@ -46,6 +46,8 @@ namespace sfv {
 //   SFV_MARCH_STARTFETCH  rows (0..4) whose loads a wave issues together before its first row
 //   SFV_MARCH_KEEPU0      stage 2 keeps U0 of the last two rows in registers instead of reloading it
 //   SFV_MARCH_FAST        specialised kernels for the common configuration (march_fast)
+//   SFV_MARCH_PRIO        1 = waves with more rows left issue first; 0 = hardware order
+//                         (oldest wave first)
 #ifndef SFV_MARCH_PREFETCH
 #define SFV_MARCH_PREFETCH 0
 #endif
@@ -88,10 +90,23 @@ namespace sfv {
 #ifndef SFV_MARCH_UNROLL1
 #define SFV_MARCH_UNROLL1 SFV_MARCH_UNROLL
 #endif
+#ifndef SFV_MARCH_PRIO
+#define SFV_MARCH_PRIO 1
+#endif
 #ifndef SFV_MARCH_UNROLL2
 #define SFV_MARCH_UNROLL2 SFV_MARCH_UNROLL
 #endif
 
+
+// s_setprio takes an immediate: one branch per level
+__device__ __forceinline__ void set_issue_priority(int lvl) {
+    switch (lvl) {
+        case 0: __builtin_amdgcn_s_setprio(0); break;
+        case 1: __builtin_amdgcn_s_setprio(1); break;
+        case 2: __builtin_amdgcn_s_setprio(2); break;
+        default: __builtin_amdgcn_s_setprio(3); break;
+    }
+}
 
 template <class F, size_t... I>
 __device__ __forceinline__ void unrolled_rows(F& body, int r, std::index_sequence<I...>) {
@@ -248,7 +263,7 @@ template <class S, int STAGE, bool FAST = false, bool BATCH = false, bool RMP1 =
 __device__ __forceinline__ void march_tile(const KParams& P, StepCtl* ctl, int slot, int my, int strip, int seg,
                                            const S* __restrict__ U0, const S* __restrict__ RHS1,
                                            S* __restrict__ OUT, const double* __restrict__ B, int par = 0,
-                                           int close = 0) {
+                                           int close = 0, int prio_after = 0, int prio_total = 0) {
     const StepSlot& sl = ctl->slot[slot];
     const int lane = threadIdx.x & (MW - 1);
     if (!sl.active) {                              // the same for every wave of the block
@@ -404,6 +419,16 @@ __device__ __forceinline__ void march_tile(const KParams& P, StepCtl* ctl, int s
     auto body = [&](int r, auto full_tag, auto x_tag) {
         constexpr bool FULL = decltype(full_tag)::value;   // r >= 4: every step applies
         const int jr = j0 - 2 + r;
+        // Issue priority by iterations left (this segment's and, in the
+        // persistent batch kernel, those of the wave's later tiles), in
+        // quarters of the wave's total: the waves sharing a SIMD then progress
+        // together and end together, instead of the oldest finishing first
+        // and the youngest running on alone at the end of the launch.
+        // prio_after < 0: the caller keeps its own priority.
+        if (SFV_MARCH_PRIO && prio_after >= 0) {
+            const int left = th + 3 - r + prio_after, q = max(1, (prio_total > 0 ? prio_total : th + 4) / 4);
+            set_issue_priority(left >= 3 * q ? 3 : left >= 2 * q ? 2 : left >= q ? 1 : 0);
+        }
         const bool xrow = FULL ? decltype(x_tag)::value : r == 3;
         const bool yface = FULL || r >= 3;
         double nb, ne;
@@ -744,6 +769,8 @@ k_march_batch(const BatchMember* __restrict__ mem, const BatchHeader* __restrict
     const int q = tiles / N_XCD, r = tiles % N_XCD;
     const int c0 = x * q + min(x, r), c1 = c0 + q + (x < r ? 1 : 0);
     int par = 0;
+    // this wave's tiles: c0 + k, c0 + k + per, ... below c1
+    const int ntiles = c1 > c0 + k ? (c1 - 1 - c0 - k) / per + 1 : 0;
     for (int t = c0 + k; t < c1; t += per, par ^= 1) {
         const int m = batch_find(mem, members, t);
         const BatchMember& M = mem[m];
@@ -751,16 +778,21 @@ k_march_batch(const BatchMember* __restrict__ mem, const BatchHeader* __restrict
         const int strip = local % M.strips, sg = local / M.strips;
         const int cur = (M.cur0 ^ lp) & 1, slot = (M.slot0 ^ lp) & 1;
         // the same for every lane of the wave; readfirstlane tells the compiler so
-    const int seg = __builtin_amdgcn_readfirstlane(sg * MWAVES + threadIdx.x / MW);
+        const int seg = __builtin_amdgcn_readfirstlane(sg * MWAVES + threadIdx.x / MW);
         const S* U0 = (const S*)M.U[cur];
+        // iterations of this wave's later tiles, counted at this member's height
+        const int after = hdr->prio ? -1 : ((c1 - 1 - t) / per) * (M.my + 4);
+        const int total = ntiles * (M.my + 4);
         if (STAGE == 1)
             if (march_edge(M.P, M.my, strip, seg))
-                march_tile<S, 1, FAST, true, true>(M.P, M.ctl, slot, M.my, strip, seg, U0, nullptr, (S*)M.RHS, M.B, par);
+                march_tile<S, 1, FAST, true, true>(M.P, M.ctl, slot, M.my, strip, seg, U0, nullptr, (S*)M.RHS, M.B, par,
+                                                   0, after, total);
             else
-                march_tile<S, 1, FAST, true>(M.P, M.ctl, slot, M.my, strip, seg, U0, nullptr, (S*)M.RHS, M.B, par);
+                march_tile<S, 1, FAST, true>(M.P, M.ctl, slot, M.my, strip, seg, U0, nullptr, (S*)M.RHS, M.B, par, 0,
+                                             after, total);
         else
             march_tile<S, 2, FAST, true>(M.P, M.ctl, slot, M.my, strip, seg, U0, (const S*)M.RHS, (S*)M.U[cur ^ 1], M.B, 0,
-                                   1);
+                                         1, after, total);
     }
 }
 
