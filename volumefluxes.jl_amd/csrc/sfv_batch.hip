@@ -154,7 +154,7 @@ int BatchEngine::add(sfv_handle* h, const RunParams& rp, int* slot_out, std::str
     M.B = h->B;
     M.cur0 = (int)(h->cur & 1);
     M.slot0 = (int)(h->step_parity & 1);
-    M.my = h->P.rows > 0 ? h->P.rows : march_rows_per_wave(h->P, 1);
+    M.my = h->P.rows > 0 ? h->P.rows : march_rows_per_wave(h->P, 1, h->cfg.precision == SFV_F64 ? 0 : 1);
     const int segs = (h->P.ny + M.my - 1) / M.my;
     M.strips = (h->P.nx + MARCH_COLS - 1) / MARCH_COLS;
     M.seggroups = (segs + MARCH_WAVES - 1) / MARCH_WAVES;

@@ -274,7 +274,7 @@ int tune_march_rows(sfv_handle* h, std::string& why) {
     if ((h->fused != KV_MARCH && h->fused != KV_HYBRID) || h->march_rows > 0) return SFV_OK;
     // a march handle whose variant the caller left open also times the tiled kernels
     const bool try_tiled = h->fused == KV_MARCH && !h->variant_forced;
-    const int fallback = march_rows_per_wave(h->P, 1);
+    const int fallback = march_rows_per_wave(h->P, 1, h->cfg.precision == SFV_F64 ? 0 : 1);
     if (!autotune_enabled()) {
         h->march_rows = fallback;
         h->P.rows = fallback;
@@ -287,6 +287,10 @@ int tune_march_rows(sfv_handle* h, std::string& why) {
     for (int c : {1, 2, 3, 4, 6, 8, 12, 16, 24})
         if (c <= h->cfg.ny) cand.push_back(c);
     if (cand.empty() || cand.back() < std::min(h->cfg.ny, 24)) cand.push_back(std::min(h->cfg.ny, 24));
+    if (std::find(cand.begin(), cand.end(), fallback) == cand.end()) {   // the model's height is timed too
+        cand.push_back(fallback);
+        std::sort(cand.begin(), cand.end());
+    }
     const int slot = (int)(h->step_parity & 1);
     const bool f64 = h->cfg.precision == SFV_F64;
     hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -335,17 +339,18 @@ int tune_march_rows(sfv_handle* h, std::string& why) {
     if (e != hipSuccess) return bail(e);
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    // the fastest, unless the build default is close to it: the timing starts
+    // the fastest, unless the model's height is close to it: the timing starts
     // from the current state (often a state at rest), where heights close in
-    // time can order differently than in the developed flow.  The default is
-    // the measured best for grids whose waves fill the device's resident wave
-    // slots (3 per SIMD), so there it must lose by 10% to be replaced; for
-    // smaller grids, where the serial path of a wave decides, by 3%.
+    // time can order differently than in the developed flow.  The model's
+    // height is the measured best for grids whose waves fill at least three
+    // quarters of the device's resident wave slots (3 per SIMD), so there it
+    // must lose by 10% to be replaced; for smaller grids, where the serial
+    // path of a wave decides, by 3%.
     int cus = 256;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, h->device) != hipSuccess || cus <= 0)
         cus = 256;
     const long waves = (long)((h->cfg.nx + MARCH_COLS - 1) / MARCH_COLS) * ((h->cfg.ny + fallback - 1) / fallback);
-    const double margin = waves >= 12L * cus ? 1.10 : 1.03;
+    const double margin = waves >= 9L * cus ? 1.10 : 1.03;
     size_t fast = 0;
     for (size_t c = 1; c < cand.size(); ++c)
         if (best[c] < best[fast]) fast = c;

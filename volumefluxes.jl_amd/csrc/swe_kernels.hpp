@@ -56,7 +56,7 @@ bool march_fast(const KParams& P);
 hipError_t launch_prepare_batch(hipStream_t st, const BatchMember* mem, const BatchHeader* hdr, int members);
 // Resident workgroups per CU of the batched stage kernel (occupancy).
 int march_batch_blocks_per_cu(int precision, int stage);
-int march_rows_per_wave(const KParams& P, int stage);
+int march_rows_per_wave(const KParams& P, int stage, int precision);   // precision: 0 f64, 1 f32
 constexpr int MARCH_COLS = 60;   // output columns per wave strip
 #ifndef SFV_MARCH_WAVES
 #define SFV_MARCH_WAVES 4

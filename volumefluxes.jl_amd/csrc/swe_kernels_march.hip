@@ -25,6 +25,7 @@
 // (ghost cells through the boundary-condition source maps), so the stage-1
 // state never touches HBM.
 #include <cmath>
+#include <mutex>
 #include <type_traits>
 #include <utility>
 
@@ -64,7 +65,7 @@ namespace sfv {
 #define SFV_MARCH_WETFAST 0
 #endif
 #ifndef SFV_MARCH_ROWS
-#define SFV_MARCH_ROWS 12
+#define SFV_MARCH_ROWS 0
 #endif
 #ifndef SFV_MARCH_MIN_WAVES
 #define SFV_MARCH_MIN_WAVES 3
@@ -796,40 +797,6 @@ k_march_batch(const BatchMember* __restrict__ mem, const BatchHeader* __restrict
     }
 }
 
-// Rows per wave segment for this grid: the makespan is about the number of
-// rounds of resident waves times the rows each wave marches (segment + 4
-// halo rows), so pick the height in [8, 64] that minimises it.
-int march_rows_per_wave(const KParams& P, int stage) {
-    if (SFV_MARCH_ROWS > 0) return SFV_MARCH_ROWS;
-    static int cus = 0, wpc[2] = {0, 0};
-    if (!cus) {
-        int dev = 0;
-        hipDeviceProp_t prop;
-        if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess)
-            cus = prop.multiProcessorCount;
-        if (cus <= 0) cus = 256;
-        for (int s = 0; s < 2; ++s) {
-            int blocks = 0;
-            const hipError_t e = s == 0 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                                              &blocks, k_march<double, 1, false>, MW * MWAVES, 0)
-                                        : hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                                              &blocks, k_march<double, 2, false>, MW * MWAVES, 0);
-            wpc[s] = (e == hipSuccess && blocks > 0) ? blocks * MWAVES : 2 * MWAVES;
-        }
-    }
-    const long slots = (long)cus * wpc[stage == 1 ? 0 : 1];
-    const long strips = (P.nx + MX - 1) / MX;
-    int best = 16;
-    long best_cost = -1;
-    for (int my = 8; my <= 64; ++my) {
-        const long waves = strips * ((P.ny + my - 1) / my);
-        const long rounds = (waves + slots - 1) / slots;
-        const long cost = rounds * (my + 4);
-        if (best_cost < 0 || cost < best_cost) { best_cost = cost; best = my; }
-    }
-    return best;
-}
-
 // The configurations the FAST kernels cover.
 bool march_fast(const KParams& P) {
     return SFV_MARCH_FAST && P.rec == 1 && std::isfinite(P.theta) && P.theta != 0.0 && P.bottom && !P.bcell_1d &&
@@ -837,10 +804,55 @@ bool march_fast(const KParams& P) {
            (P.rain_kind == 0 || P.rain_kind == 1 || P.rain_kind == 2);
 }
 
+// Rows per wave segment for this grid.  A launch runs in rounds of resident
+// workgroups (CUs x workgroups per CU), and each round lasts about one
+// segment's march (its rows + 4 halo rows), so the makespan is about
+//   rounds x (my + 4),  rounds = ceil(strips * ceil(segments / MWAVES) / resident);
+// the height in [8, 64] that minimises it wins (the smallest on a tie).  At
+// 2048^2 that is 25 rows: 735 workgroups, one round (24 rows would give 770
+// and a second round for 2 of them).
+template <class S, int STAGE, bool FAST>
+static int march_resident_blocks() {
+    int blocks = 0;
+    const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_march<S, STAGE, FAST>, MW * MWAVES, 0);
+    return (e == hipSuccess && blocks > 0) ? blocks : 2;
+}
+int march_rows_per_wave(const KParams& P, int stage, int precision) {
+    if (SFV_MARCH_ROWS > 0) return SFV_MARCH_ROWS;
+    static int cus = 0, bpc[2][2][2];       // [precision][stage - 1][fast]
+    static std::once_flag once;
+    std::call_once(once, [] {
+        int dev = 0;
+        hipDeviceProp_t prop;
+        if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess)
+            cus = prop.multiProcessorCount;
+        if (cus <= 0) cus = 256;
+        bpc[0][0][0] = march_resident_blocks<double, 1, false>();
+        bpc[0][0][1] = march_resident_blocks<double, 1, true>();
+        bpc[0][1][0] = march_resident_blocks<double, 2, false>();
+        bpc[0][1][1] = march_resident_blocks<double, 2, true>();
+        bpc[1][0][0] = march_resident_blocks<float, 1, false>();
+        bpc[1][0][1] = march_resident_blocks<float, 1, true>();
+        bpc[1][1][0] = march_resident_blocks<float, 2, false>();
+        bpc[1][1][1] = march_resident_blocks<float, 2, true>();
+    });
+    const long resident = (long)cus * bpc[precision ? 1 : 0][stage == 1 ? 0 : 1][march_fast(P) ? 1 : 0];
+    const long strips = (P.nx + MX - 1) / MX;
+    int best = 8;
+    long best_cost = -1;
+    for (int my = 8; my <= 64; ++my) {
+        const long blocks = strips * (((P.ny + my - 1) / my + MWAVES - 1) / MWAVES);
+        const long rounds = (blocks + resident - 1) / resident;
+        const long cost = rounds * (my + 4);
+        if (best_cost < 0 || cost < best_cost) { best_cost = cost; best = my; }
+    }
+    return best;
+}
+
 template <class S>
 hipError_t launch_march(hipStream_t st, const KParams& P, StepCtl* ctl, int slot, int stage, const S* U0,
                         const S* RHS1, S* OUT, const double* B, int close, int remap_halo) {
-    const int my = P.rows > 0 ? P.rows : march_rows_per_wave(P, stage);
+    const int my = P.rows > 0 ? P.rows : march_rows_per_wave(P, stage, std::is_same<S, float>::value ? 1 : 0);
     const int segs = (P.ny + my - 1) / my;
     const int strips = (P.nx + MX - 1) / MX, seggroups = (segs + MWAVES - 1) / MWAVES;
     const dim3 grid(strips * seggroups);
