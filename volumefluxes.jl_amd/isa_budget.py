@@ -42,6 +42,8 @@ PHASE = {
     "apply_friction": "friction (scalar and implicit update)",
     "stage1_cell": "U1 rebuild: RK stage 1 and dry clamp",
     "in_band": "band checks and votes",
+    "in_band5": "band checks and votes",
+    "set_issue_priority": "issue priority (rows left)",
     "dpp_shift": "lane shifts (DPP)",
     "shfl_dn1": "lane shifts (DPP)",
     "shfl_up1": "lane shifts (DPP)",
