@@ -49,6 +49,8 @@ namespace sfv {
 //   SFV_MARCH_FAST        specialised kernels for the common configuration (march_fast)
 //   SFV_MARCH_PRIO        1 = waves with more rows left issue first; 0 = hardware order
 //                         (oldest wave first)
+//   SFV_MARCH_PRIO_DIV    the priority steps down every 1/DIV of a wave's rows, over
+//                         its last 3/DIV
 #ifndef SFV_MARCH_PREFETCH
 #define SFV_MARCH_PREFETCH 0
 #endif
@@ -93,6 +95,9 @@ namespace sfv {
 #endif
 #ifndef SFV_MARCH_PRIO
 #define SFV_MARCH_PRIO 1
+#endif
+#ifndef SFV_MARCH_PRIO_DIV
+#define SFV_MARCH_PRIO_DIV 8
 #endif
 #ifndef SFV_MARCH_UNROLL2
 #define SFV_MARCH_UNROLL2 SFV_MARCH_UNROLL
@@ -421,13 +426,15 @@ __device__ __forceinline__ void march_tile(const KParams& P, StepCtl* ctl, int s
         constexpr bool FULL = decltype(full_tag)::value;   // r >= 4: every step applies
         const int jr = j0 - 2 + r;
         // Issue priority by iterations left (this segment's and, in the
-        // persistent batch kernel, those of the wave's later tiles), in
-        // quarters of the wave's total: the waves sharing a SIMD then progress
-        // together and end together, instead of the oldest finishing first
-        // and the youngest running on alone at the end of the launch.
-        // prio_after < 0: the caller keeps its own priority.
+        // persistent batch kernel, those of the wave's later tiles): 3 until
+        // the last 3/DIV of the wave's total, then one step lower every
+        // 1/DIV.  The waves sharing a SIMD then progress together and end
+        // together, instead of the oldest finishing first and the youngest
+        // running on alone at the end of the launch.  prio_after < 0: the
+        // caller keeps its own priority.
         if (SFV_MARCH_PRIO && prio_after >= 0) {
-            const int left = th + 3 - r + prio_after, q = max(1, (prio_total > 0 ? prio_total : th + 4) / 4);
+            const int left = th + 3 - r + prio_after,
+                      q = max(1, (prio_total > 0 ? prio_total : th + 4) / SFV_MARCH_PRIO_DIV);
             set_issue_priority(left >= 3 * q ? 3 : left >= 2 * q ? 2 : left >= q ? 1 : 0);
         }
         const bool xrow = FULL ? decltype(x_tag)::value : r == 3;
