@@ -51,6 +51,8 @@ namespace sfv {
 //                         (oldest wave first)
 //   SFV_MARCH_PRIO_DIV    the priority steps down every 1/DIV of a wave's rows, over
 //                         its last 3/DIV
+//   SFV_MARCH_LINEAR      1 = waves numbered across the launch (no per-strip padding of
+//                         segments to whole workgroups); 0 = a workgroup per 4 segments of a strip
 #ifndef SFV_MARCH_PREFETCH
 #define SFV_MARCH_PREFETCH 0
 #endif
@@ -95,6 +97,9 @@ namespace sfv {
 #endif
 #ifndef SFV_MARCH_PRIO
 #define SFV_MARCH_PRIO 1
+#endif
+#ifndef SFV_MARCH_LINEAR
+#define SFV_MARCH_LINEAR 1
 #endif
 #ifndef SFV_MARCH_PRIO_DIV
 #define SFV_MARCH_PRIO_DIV 8
@@ -740,21 +745,39 @@ __device__ __forceinline__ bool march_edge(const KParams& P, int my, int strip, 
     return strip == 0 || gc + strip * MX + MW - 3 >= P.nxT - gc || j0 - 2 < gc || j0 + th + 1 >= P.nyT - gc;
 }
 
-// 1D grid of strips x seggroups blocks; a block is 4 waves on 4 consecutive
-// segments of one strip.
+// Grid of march_blocks(strips, segs) blocks of MWAVES waves.  With
+// SFV_MARCH_LINEAR the waves are numbered across the launch (the block's
+// logical tile times MWAVES plus the wave) and a workgroup's waves are
+// neighbouring strips of one segment row, so only the last workgroup has idle
+// waves; otherwise a block is MWAVES consecutive segments of one strip, and a
+// strip's segments are padded to a multiple of MWAVES.
+__host__ __device__ __forceinline__ int march_blocks(int strips, int segs) {
+    return SFV_MARCH_LINEAR ? (strips * segs + MWAVES - 1) / MWAVES : strips * ((segs + MWAVES - 1) / MWAVES);
+}
 template <class S, int STAGE, bool FAST>
 __global__ void __launch_bounds__(MW* MWAVES)
 __attribute__((amdgpu_waves_per_eu(STAGE == 1 ? SFV_MARCH_MIN_WAVES1 : SFV_MARCH_MIN_WAVES)))
-k_march(KParams P, StepCtl* ctl, int slot, int my, int strips, int seggroups, const S* __restrict__ U0,
+k_march(KParams P, StepCtl* ctl, int slot, int my, int strips, int segs, const S* __restrict__ U0,
         const S* __restrict__ RHS1, S* __restrict__ OUT, const double* __restrict__ B, int close, int remap) {
-    const int t = xcd_tile(blockIdx.x, strips * seggroups);
+    const int t = xcd_tile(blockIdx.x, march_blocks(strips, segs));
+    // the same for every lane of the wave; readfirstlane tells the compiler so
+#if SFV_MARCH_LINEAR
+    const int w = __builtin_amdgcn_readfirstlane(t * MWAVES + threadIdx.x / MW);
+    const bool idle = w >= strips * segs;          // past the last tile: seg = segs is past the grid
+#if SFV_MARCH_XCD == 2
+    const int strip = idle ? 0 : w % strips, seg = idle ? segs : w / strips;
+#else
+    const int strip = idle ? 0 : w / segs, seg = idle ? segs : w % segs;
+#endif
+#else
+    const int seggroups = (segs + MWAVES - 1) / MWAVES;
 #if SFV_MARCH_XCD == 2
     const int strip = t % strips, sg = t / strips;
 #else
     const int strip = t / seggroups, sg = t % seggroups;
 #endif
-    // the same for every lane of the wave; readfirstlane tells the compiler so
     const int seg = __builtin_amdgcn_readfirstlane(sg * MWAVES + threadIdx.x / MW);
+#endif
     if (STAGE == 1 && remap && march_edge(P, my, strip, seg))
         march_tile<S, 1, FAST, false, true>(P, ctl, slot, my, strip, seg, U0, RHS1, OUT, B);
     else
@@ -814,10 +837,10 @@ bool march_fast(const KParams& P) {
 // Rows per wave segment for this grid.  A launch runs in rounds of resident
 // workgroups (CUs x workgroups per CU), and each round lasts about one
 // segment's march (its rows + 4 halo rows), so the makespan is about
-//   rounds x (my + 4),  rounds = ceil(strips * ceil(segments / MWAVES) / resident);
+//   rounds x (my + 4),  rounds = ceil(march_blocks(strips, segments) / resident);
 // the height in [8, 64] that minimises it wins (the smallest on a tie).  At
-// 2048^2 that is 25 rows: 735 workgroups, one round (24 rows would give 770
-// and a second round for 2 of them).
+// 2048^2 that is 24 rows: 35 x 86 waves in 753 workgroups, one round of the
+// 768 resident ones (23 rows would need 788).
 template <class S, int STAGE, bool FAST>
 static int march_resident_blocks() {
     int blocks = 0;
@@ -848,7 +871,7 @@ int march_rows_per_wave(const KParams& P, int stage, int precision) {
     int best = 8;
     long best_cost = -1;
     for (int my = 8; my <= 64; ++my) {
-        const long blocks = strips * (((P.ny + my - 1) / my + MWAVES - 1) / MWAVES);
+        const long blocks = march_blocks((int)strips, (P.ny + my - 1) / my);
         const long rounds = (blocks + resident - 1) / resident;
         const long cost = rounds * (my + 4);
         if (best_cost < 0 || cost < best_cost) { best_cost = cost; best = my; }
@@ -861,22 +884,22 @@ hipError_t launch_march(hipStream_t st, const KParams& P, StepCtl* ctl, int slot
                         const S* RHS1, S* OUT, const double* B, int close, int remap_halo) {
     const int my = P.rows > 0 ? P.rows : march_rows_per_wave(P, stage, std::is_same<S, float>::value ? 1 : 0);
     const int segs = (P.ny + my - 1) / my;
-    const int strips = (P.nx + MX - 1) / MX, seggroups = (segs + MWAVES - 1) / MWAVES;
-    const dim3 grid(strips * seggroups);
+    const int strips = (P.nx + MX - 1) / MX;
+    const dim3 grid(march_blocks(strips, segs));
     const dim3 block(MW * MWAVES);
     if (march_fast(P)) {
         if (stage == 1)
-            hipLaunchKernelGGL((k_march<S, 1, true>), grid, block, 0, st, P, ctl, slot, my, strips, seggroups, U0, RHS1,
+            hipLaunchKernelGGL((k_march<S, 1, true>), grid, block, 0, st, P, ctl, slot, my, strips, segs, U0, RHS1,
                                OUT, B, close, remap_halo);
         else
-            hipLaunchKernelGGL((k_march<S, 2, true>), grid, block, 0, st, P, ctl, slot, my, strips, seggroups, U0, RHS1,
+            hipLaunchKernelGGL((k_march<S, 2, true>), grid, block, 0, st, P, ctl, slot, my, strips, segs, U0, RHS1,
                                OUT, B, close, remap_halo);
     } else {
         if (stage == 1)
-            hipLaunchKernelGGL((k_march<S, 1, false>), grid, block, 0, st, P, ctl, slot, my, strips, seggroups, U0,
+            hipLaunchKernelGGL((k_march<S, 1, false>), grid, block, 0, st, P, ctl, slot, my, strips, segs, U0,
                                RHS1, OUT, B, close, remap_halo);
         else
-            hipLaunchKernelGGL((k_march<S, 2, false>), grid, block, 0, st, P, ctl, slot, my, strips, seggroups, U0,
+            hipLaunchKernelGGL((k_march<S, 2, false>), grid, block, 0, st, P, ctl, slot, my, strips, segs, U0,
                                RHS1, OUT, B, close, remap_halo);
     }
     return hipPeekAtLastError();
