@@ -230,7 +230,8 @@ int32_t sfv_get_fused(const sfv_handle* h);
  * default) lets the library choose on the first step: candidate heights are
  * timed on the handle's own state and the fastest is kept for grids of that
  * size class (quarter octaves of nx and ny) on that device; the environment
- * variable SFV_AUTOTUNE=0 turns the measurement off (a build default is used).
+ * variable SFV_AUTOTUNE=0 turns the measurement off (the height that makes
+ * the launch's workgroups the fewest rounds of resident workgroups is used).
  * rows > 0 forces a height.  Results are bitwise the same for every height.
  * No counterpart in the reference. */
 int sfv_set_march_rows(sfv_handle* h, int32_t rows);
