@@ -212,7 +212,8 @@ bool BatchEngine::finished(int slot) const {
 }
 
 // Segment heights of the members for the current membership.  When the
-// batch fills the device's resident wave slots at the build default height,
+// batch fills the device's resident wave slots at the batch default height
+// (SFV_BATCH_DEFAULT_ROWS),
 // every member takes it (fewest halo instructions).  Otherwise the device is
 // underfilled and a step lasts about one wave's serial path, which shorter
 // segments shorten as long as no SIMD has to run two of the batch's waves:

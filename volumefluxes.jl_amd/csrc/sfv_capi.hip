@@ -141,7 +141,8 @@ int ensure_ready(sfv_handle* h) {
 // and switches to them when they win.  Handles on the hybrid path (ForwardEuler,
 // NeumannBC) run only the stage-1 kernel, so they time it alone and keep
 // their own cache entries.  Every height gives bitwise the same results.
-// SFV_AUTOTUNE=0 in the environment keeps the build default.
+// SFV_AUTOTUNE=0 in the environment keeps the height model's choice
+// (march_rows_per_wave: the fewest rounds of resident workgroups).
 struct TuneKey {
     int dev, prec, bx, by, kind;   // kind: 0 full march steps, 1 stage 1 only
     bool operator<(const TuneKey& o) const {

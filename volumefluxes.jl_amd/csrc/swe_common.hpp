@@ -25,7 +25,7 @@ struct KParams {
     int infil_kind; // 0 none, 1 constant, 2 horton
     int rain_first;
     int stepper;    // 1 FE, 2 RK2
-    int rows;       // rows per wave segment of the march kernels (0: the build default)
+    int rows;       // rows per wave segment of the march kernels (0: march_rows_per_wave's choice)
     int rain_n;
     const double* rain_rates;  // device (reference unit)
     const double* rain_times;  // device
