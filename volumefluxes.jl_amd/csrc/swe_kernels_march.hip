@@ -838,9 +838,9 @@ bool march_fast(const KParams& P) {
 // workgroups (CUs x workgroups per CU), and each round lasts about one
 // segment's march (its rows + 4 halo rows), so the makespan is about
 //   rounds x (my + 4),  rounds = ceil(march_blocks(strips, segments) / resident);
-// the height in [8, 64] that minimises it wins (the smallest on a tie).  At
+// the height in [1, 64] that minimises it wins (the smallest on a tie).  At
 // 2048^2 that is 24 rows: 35 x 86 waves in 753 workgroups, one round of the
-// 768 resident ones (23 rows would need 788).
+// 768 resident ones (23 rows would need 788); at 512^2, 2 rows.
 template <class S, int STAGE, bool FAST>
 static int march_resident_blocks() {
     int blocks = 0;
@@ -868,9 +868,9 @@ int march_rows_per_wave(const KParams& P, int stage, int precision) {
     });
     const long resident = (long)cus * bpc[precision ? 1 : 0][stage == 1 ? 0 : 1][march_fast(P) ? 1 : 0];
     const long strips = (P.nx + MX - 1) / MX;
-    int best = 8;
+    int best = 1;
     long best_cost = -1;
-    for (int my = 8; my <= 64; ++my) {
+    for (int my = 1; my <= 64; ++my) {
         const long blocks = march_blocks((int)strips, (P.ny + my - 1) / my);
         const long rounds = (blocks + resident - 1) / resident;
         const long cost = rounds * (my + 4);
