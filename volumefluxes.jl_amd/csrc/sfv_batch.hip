@@ -157,7 +157,8 @@ int BatchEngine::add(sfv_handle* h, const RunParams& rp, int* slot_out, std::str
     M.my = h->P.rows > 0 ? h->P.rows : march_rows_per_wave(h->P, 1, h->cfg.precision == SFV_F64 ? 0 : 1);
     const int segs = (h->P.ny + M.my - 1) / M.my;
     M.strips = (h->P.nx + MARCH_COLS - 1) / MARCH_COLS;
-    M.seggroups = (segs + MARCH_WAVES - 1) / MARCH_WAVES;
+    M.segs = segs;
+    M.ntiles = march_blocks(M.strips, segs);
     M.t0 = rp.t0;
     M.until_endtime = rp.until_endtime;
     M.endtime = rp.until_endtime ? rp.endtime : NAN;
@@ -250,7 +251,8 @@ void BatchEngine::choose_rows() {
         M.my = my;
         M.P.rows = my;
         const int segs = (h->P.ny + my - 1) / my;
-        M.seggroups = (segs + MARCH_WAVES - 1) / MARCH_WAVES;
+        M.segs = segs;
+        M.ntiles = march_blocks(M.strips, segs);
     }
 }
 
@@ -282,7 +284,7 @@ void BatchEngine::plan_tuning() {
     if (!any_free) return;
     const std::vector<long long> sig = signature();
     if (sig == tuned_sig_) { rows_override_ = tuned_rows_; return; }
-    static const int C[] = {1, 2, 3, 4, 6, 8, 12, 16, 24};
+    static const int C[] = {1, 2, 3, 4, 6, 8, 12, 16, 24, 32, 48};
     const int n = (int)(sizeof(C) / sizeof(C[0]));
     const int r0 = rule_rows();
     int i0 = 0;
@@ -302,7 +304,7 @@ int BatchEngine::upload_table(std::string& why) {
     for (size_t r = 0; r < rows_.size(); ++r) {
         BatchMember& M = slot_row_[rows_[r]];
         M.tile0 = tiles;
-        tiles += M.strips * M.seggroups;
+        tiles += M.ntiles;
         h_mem_[r] = M;
     }
     h_hdr_->members = (int)rows_.size();

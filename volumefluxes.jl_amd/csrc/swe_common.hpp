@@ -88,9 +88,10 @@ struct BatchMember {
     const double* B;
     int cur0, slot0;
     int my;               // rows per wave segment
-    int strips, seggroups;
+    int strips, segs;     // wave strips and segments per strip
     int tile0;            // first block tile of this member in the batch order
-    int pad[2];
+    int ntiles;           // its block tiles: march_blocks(strips, segs)
+    int pad;
     // run parameters, read by the batched prepare
     double t0, endtime, max_timestep;
     long long max_steps;
@@ -99,7 +100,7 @@ struct BatchMember {
 // Batch-wide counts, in device memory beside the member table.
 struct BatchHeader {
     int members;
-    int tiles;            // sum of strips * seggroups
+    int tiles;            // sum of the members' ntiles
     int prio;             // nonzero: the batch kernels raise their waves' issue priority
     int pad;
 };

@@ -62,6 +62,18 @@ constexpr int MARCH_COLS = 60;   // output columns per wave strip
 #define SFV_MARCH_WAVES 4
 #endif
 constexpr int MARCH_WAVES = SFV_MARCH_WAVES;   // waves (segments) per workgroup
+#ifndef SFV_MARCH_LINEAR
+#define SFV_MARCH_LINEAR 1
+#endif
+// Workgroups of a march launch (or of one batch member) over strips x segs
+// wave segments.  With SFV_MARCH_LINEAR the waves are numbered across the
+// launch (strips fastest), so only the last workgroup has idle waves;
+// otherwise a workgroup is MARCH_WAVES consecutive segments of one strip,
+// and each strip's segments are padded to a multiple of MARCH_WAVES.
+__host__ __device__ inline int march_blocks(int strips, int segs) {
+    return SFV_MARCH_LINEAR ? (strips * segs + MARCH_WAVES - 1) / MARCH_WAVES
+                            : strips * ((segs + MARCH_WAVES - 1) / MARCH_WAVES);
+}
 
 // Plain copy of `bytes` (a multiple of 4) from src to dst.
 hipError_t launch_copy(hipStream_t st, const void* src, void* dst, size_t bytes);

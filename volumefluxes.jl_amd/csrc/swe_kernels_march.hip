@@ -98,9 +98,6 @@ namespace sfv {
 #ifndef SFV_MARCH_PRIO
 #define SFV_MARCH_PRIO 1
 #endif
-#ifndef SFV_MARCH_LINEAR
-#define SFV_MARCH_LINEAR 1
-#endif
 #ifndef SFV_MARCH_PRIO_DIV
 #define SFV_MARCH_PRIO_DIV 8
 #endif
@@ -745,15 +742,10 @@ __device__ __forceinline__ bool march_edge(const KParams& P, int my, int strip, 
     return strip == 0 || gc + strip * MX + MW - 3 >= P.nxT - gc || j0 - 2 < gc || j0 + th + 1 >= P.nyT - gc;
 }
 
-// Grid of march_blocks(strips, segs) blocks of MWAVES waves.  With
-// SFV_MARCH_LINEAR the waves are numbered across the launch (the block's
-// logical tile times MWAVES plus the wave) and a workgroup's waves are
-// neighbouring strips of one segment row, so only the last workgroup has idle
-// waves; otherwise a block is MWAVES consecutive segments of one strip, and a
-// strip's segments are padded to a multiple of MWAVES.
-__host__ __device__ __forceinline__ int march_blocks(int strips, int segs) {
-    return SFV_MARCH_LINEAR ? (strips * segs + MWAVES - 1) / MWAVES : strips * ((segs + MWAVES - 1) / MWAVES);
-}
+// Grid of march_blocks(strips, segs) blocks of MWAVES waves (swe_kernels.hpp).
+// With SFV_MARCH_LINEAR a wave's number is the block's logical tile times
+// MWAVES plus the wave, and a workgroup's waves are neighbouring strips of
+// one segment row.
 template <class S, int STAGE, bool FAST>
 __global__ void __launch_bounds__(MW* MWAVES)
 __attribute__((amdgpu_waves_per_eu(STAGE == 1 ? SFV_MARCH_MIN_WAVES1 : SFV_MARCH_MIN_WAVES)))
@@ -806,10 +798,16 @@ k_march_batch(const BatchMember* __restrict__ mem, const BatchHeader* __restrict
         const int m = batch_find(mem, members, t);
         const BatchMember& M = mem[m];
         const int local = t - M.tile0;
-        const int strip = local % M.strips, sg = local / M.strips;
         const int cur = (M.cur0 ^ lp) & 1, slot = (M.slot0 ^ lp) & 1;
         // the same for every lane of the wave; readfirstlane tells the compiler so
+#if SFV_MARCH_LINEAR
+        const int w = __builtin_amdgcn_readfirstlane(local * MWAVES + threadIdx.x / MW);
+        const bool idle = w >= M.strips * M.segs;  // past the member's last tile: seg = segs is past its grid
+        const int strip = idle ? 0 : w % M.strips, seg = idle ? M.segs : w / M.strips;
+#else
+        const int strip = local % M.strips, sg = local / M.strips;
         const int seg = __builtin_amdgcn_readfirstlane(sg * MWAVES + threadIdx.x / MW);
+#endif
         const S* U0 = (const S*)M.U[cur];
         // iterations of this wave's later tiles, counted at this member's height
         const int after = hdr->prio ? -1 : ((c1 - 1 - t) / per) * (M.my + 4);
