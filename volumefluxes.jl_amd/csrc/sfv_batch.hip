@@ -272,7 +272,7 @@ std::vector<long long> BatchEngine::signature() const {
 
 // After a membership change: reuse the kept choice for the same member sizes,
 // else line up the timed trials: the rule's height, the next lower and the
-// two next higher candidates, the rule's height again.
+// three next higher candidates, the rule's height again.
 void BatchEngine::plan_tuning() {
     trials_.clear();
     trial_ms_.clear();
@@ -293,6 +293,7 @@ void BatchEngine::plan_tuning() {
     if (i0 > 0) trials_.push_back(C[i0 - 1]);
     if (i0 < n - 1) trials_.push_back(C[i0 + 1]);
     if (i0 < n - 2) trials_.push_back(C[i0 + 2]);
+    if (i0 < n - 3) trials_.push_back(C[i0 + 3]);
     trials_.push_back(r0);
     trial_ms_.assign(trials_.size(), 0.f);
     rows_override_ = trials_[0];

@@ -18,7 +18,7 @@
 namespace sfv {
 
 // Segment height of batch members when the batch fills the device.
-constexpr int SFV_BATCH_DEFAULT_ROWS = 24;
+constexpr int SFV_BATCH_DEFAULT_ROWS = 12;
 
 // How an engine's persistent grid shares the device with another engine's.
 //   ALONE: the grid fills every resident block slot.
