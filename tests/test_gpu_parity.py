@@ -138,6 +138,24 @@ def test_segment_height_keeps_results_bitwise(rows):
     assert sim.march_rows == rows
 
 
+def test_launch_of_several_rounds_bitwise():
+    """600^2 at one-row segments: 6,000 waves, more workgroups than the device
+    holds at once, numbered across the launch with idle waves in the last."""
+    sc = terrain(600, 600, seed=24)
+    print("[parity] 600x600 rows=1", flush=True)
+    o = make_oracle(sc)
+    sim = make_hip(sc, "march")
+    sim.set_march_rows(1)
+    t = 0.0
+    for k in range(3):
+        dr = o.step(float("inf"), t)
+        dh = VF.perform_step(sim, float("inf"))
+        assert dh == dr, (k, dh, dr)
+        t += dr
+        sim.t = t
+    assert np.array_equal(sim.get_full_state(), o.state())
+
+
 def test_autotuned_height_is_chosen_once_per_size_class():
     a = make_hip(terrain(90, 60, seed=29), "march")
     assert a.march_rows == 0
