@@ -138,6 +138,25 @@ def test_segment_height_keeps_results_bitwise(rows):
     assert sim.march_rows == rows
 
 
+def test_model_height_at_1024_bitwise(monkeypatch):
+    """1024^2 with the tuner off: the height model's choice (7 rows, 18 x 147
+    waves, the last workgroup partly idle) against the oracle."""
+    monkeypatch.setenv("SFV_AUTOTUNE", "0")
+    sc = terrain(1024, 1024, seed=25)
+    print("[parity] 1024x1024 model height", flush=True)
+    o = make_oracle(sc)
+    sim = make_hip(sc, "march")
+    t = 0.0
+    for k in range(3):
+        dr = o.step(float("inf"), t)
+        dh = VF.perform_step(sim, float("inf"))
+        assert dh == dr, (k, dh, dr)
+        t += dr
+        sim.t = t
+    assert sim.march_rows == 7
+    assert np.array_equal(sim.get_full_state(), o.state())
+
+
 def test_launch_of_several_rounds_bitwise():
     """600^2 at one-row segments: 6,000 waves, more workgroups than the device
     holds at once, numbered across the launch with idle waves in the last."""
