@@ -53,6 +53,8 @@ namespace sfv {
 //                         its last 3/DIV
 //   SFV_MARCH_LINEAR      1 = waves numbered across the launch (no per-strip padding of
 //                         segments to whole workgroups); 0 = a workgroup per 4 segments of a strip
+//   SFV_MARCH_TIMELINE    measurement build (`make timeline`): the single-grid kernels record
+//                         each wave's start, end, hardware slot and row start times
 #ifndef SFV_MARCH_PREFETCH
 #define SFV_MARCH_PREFETCH 0
 #endif
@@ -105,6 +107,19 @@ namespace sfv {
 #define SFV_MARCH_UNROLL2 SFV_MARCH_UNROLL
 #endif
 
+
+#ifndef SFV_MARCH_TIMELINE
+#define SFV_MARCH_TIMELINE 0
+#endif
+#if SFV_MARCH_TIMELINE
+// Wave timeline of the last single-grid launch of each stage (the 100 MHz
+// real-time counter): [stage][wave] = {start, end, HW_ID, XCC_ID << 32 |
+// strip << 16 | segment}, and the start of each of the first 32 row
+// iterations.  Read with sfv_probe_timeline (this build only).
+constexpr int TL_WAVES = 16384, TL_ROWS = 32;
+__device__ unsigned long long tl_wave[2][TL_WAVES][4];
+__device__ unsigned long long tl_rows[2][TL_WAVES][TL_ROWS];
+#endif
 
 // s_setprio takes an immediate: one branch per level
 __device__ __forceinline__ void set_issue_priority(int lvl) {
@@ -427,6 +442,12 @@ __device__ __forceinline__ void march_tile(const KParams& P, StepCtl* ctl, int s
     auto body = [&](int r, auto full_tag, auto x_tag) {
         constexpr bool FULL = decltype(full_tag)::value;   // r >= 4: every step applies
         const int jr = j0 - 2 + r;
+#if SFV_MARCH_TIMELINE
+        if (!BATCH && r < TL_ROWS && lane == 0) {
+            const int tw = blockIdx.x * MWAVES + threadIdx.x / MW;
+            if (tw < TL_WAVES) tl_rows[STAGE - 1][tw][r] = __builtin_amdgcn_s_memrealtime();
+        }
+#endif
         // Issue priority by iterations left (this segment's and, in the
         // persistent batch kernel, those of the wave's later tiles): 3 until
         // the last 3/DIV of the wave's total, then one step lower every
@@ -770,10 +791,25 @@ k_march(KParams P, StepCtl* ctl, int slot, int my, int strips, int segs, const S
 #endif
     const int seg = __builtin_amdgcn_readfirstlane(sg * MWAVES + threadIdx.x / MW);
 #endif
+#if SFV_MARCH_TIMELINE
+    const unsigned long long tl0 = __builtin_amdgcn_s_memrealtime();
+#endif
     if (STAGE == 1 && remap && march_edge(P, my, strip, seg))
         march_tile<S, 1, FAST, false, true>(P, ctl, slot, my, strip, seg, U0, RHS1, OUT, B);
     else
         march_tile<S, STAGE, FAST>(P, ctl, slot, my, strip, seg, U0, RHS1, OUT, B, 0, close);
+#if SFV_MARCH_TIMELINE
+    const unsigned long long tl1 = __builtin_amdgcn_s_memrealtime();
+    const int tw = blockIdx.x * MWAVES + threadIdx.x / MW;
+    if ((threadIdx.x & (MW - 1)) == 0 && tw < TL_WAVES) {
+        unsigned long long* o = tl_wave[STAGE - 1][tw];
+        o[0] = tl0;
+        o[1] = tl1;
+        o[2] = __builtin_amdgcn_s_getreg((31 << 11) | 4);                       // HW_ID
+        o[3] = ((unsigned long long)__builtin_amdgcn_s_getreg((15 << 11) | 20) << 32) |   // XCC_ID
+               ((unsigned long long)(strip & 0xFFFF) << 16) | (unsigned)(seg & 0xFFFF);
+    }
+#endif
 }
 
 // Batched stage kernel: a persistent grid (a multiple of 8 workgroups) walks
@@ -941,3 +977,22 @@ template hipError_t launch_march<float>(hipStream_t, const KParams&, StepCtl*, i
                                         float*, const double*, int, int);
 
 }  // namespace sfv
+
+#if SFV_MARCH_TIMELINE
+// Measurement build only: copy the timeline buffers out (waves: 2 x 16384 x 4,
+// rows: 2 x 16384 x 32 unsigned 64-bit words) or clear them.
+extern "C" int sfv_probe_timeline(unsigned long long* waves, unsigned long long* rows) {
+    hipError_t e = hipMemcpyFromSymbol(waves, HIP_SYMBOL(sfv::tl_wave), sizeof(sfv::tl_wave), 0, hipMemcpyDeviceToHost);
+    if (e == hipSuccess)
+        e = hipMemcpyFromSymbol(rows, HIP_SYMBOL(sfv::tl_rows), sizeof(sfv::tl_rows), 0, hipMemcpyDeviceToHost);
+    return (int)e;
+}
+extern "C" int sfv_probe_timeline_clear() {
+    void* p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(sfv::tl_wave));
+    if (e == hipSuccess) e = hipMemset(p, 0, sizeof(sfv::tl_wave));
+    if (e == hipSuccess) e = hipGetSymbolAddress(&p, HIP_SYMBOL(sfv::tl_rows));
+    if (e == hipSuccess) e = hipMemset(p, 0, sizeof(sfv::tl_rows));
+    return (int)e;
+}
+#endif
