@@ -310,7 +310,7 @@ int BatchEngine::upload_table(std::string& why) {
     }
     h_hdr_->members = (int)rows_.size();
     h_hdr_->tiles = tiles;
-    h_hdr_->prio = share_ == Share::GUEST ? 1 : 0;
+    h_hdr_->prio = share_ == Share::GUEST ? 1 : share_ == Share::HOST ? 2 : 0;
     h_hdr_->pad = 0;
     if (!rows_.empty())
         ETRY(hipMemcpyAsync(d_mem_, h_mem_, sizeof(BatchMember) * rows_.size(), hipMemcpyHostToDevice, stream_));

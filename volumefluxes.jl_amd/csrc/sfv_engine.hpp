@@ -22,7 +22,8 @@ constexpr int SFV_BATCH_DEFAULT_ROWS = 12;
 
 // How an engine's persistent grid shares the device with another engine's.
 //   ALONE: the grid fills every resident block slot.
-//   HOST:  one block slot per CU is left free for a GUEST engine.
+//   HOST:  one block slot per CU is left free for a GUEST engine, and the
+//          waves' rows-left issue priority stays below the guest's.
 //   GUEST: one block per CU, and the waves raise their issue priority; for
 //          small, latency-bound grids stepping beside a HOST engine.
 enum class Share { ALONE, HOST, GUEST };

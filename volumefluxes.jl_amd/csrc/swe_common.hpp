@@ -101,7 +101,8 @@ struct BatchMember {
 struct BatchHeader {
     int members;
     int tiles;            // sum of the members' ntiles
-    int prio;             // nonzero: the batch kernels raise their waves' issue priority
+    int prio;             // 1: the waves take the highest issue priority (a GUEST engine); 2: their
+                          // rows-left priority stays below it (a HOST engine beside a guest); 0: alone
     int pad;
 };
 

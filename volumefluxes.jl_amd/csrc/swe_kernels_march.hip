@@ -286,7 +286,8 @@ template <class S, int STAGE, bool FAST = false, bool BATCH = false, bool RMP1 =
 __device__ __forceinline__ void march_tile(const KParams& P, StepCtl* ctl, int slot, int my, int strip, int seg,
                                            const S* __restrict__ U0, const S* __restrict__ RHS1,
                                            S* __restrict__ OUT, const double* __restrict__ B, int par = 0,
-                                           int close = 0, int prio_after = 0, int prio_total = 0) {
+                                           int close = 0, int prio_after = 0, int prio_total = 0,
+                                           int prio_cap = 3) {
     const StepSlot& sl = ctl->slot[slot];
     const int lane = threadIdx.x & (MW - 1);
     if (!sl.active) {                              // the same for every wave of the block
@@ -454,11 +455,11 @@ __device__ __forceinline__ void march_tile(const KParams& P, StepCtl* ctl, int s
         // 1/DIV.  The waves sharing a SIMD then progress together and end
         // together, instead of the oldest finishing first and the youngest
         // running on alone at the end of the launch.  prio_after < 0: the
-        // caller keeps its own priority.
+        // caller keeps its own priority; prio_cap: the highest level used.
         if (SFV_MARCH_PRIO && prio_after >= 0) {
             const int left = th + 3 - r + prio_after,
                       q = max(1, (prio_total > 0 ? prio_total : th + 4) / SFV_MARCH_PRIO_DIV);
-            set_issue_priority(left >= 3 * q ? 3 : left >= 2 * q ? 2 : left >= q ? 1 : 0);
+            set_issue_priority(min(prio_cap, left >= 3 * q ? 3 : left >= 2 * q ? 2 : left >= q ? 1 : 0));
         }
         const bool xrow = FULL ? decltype(x_tag)::value : r == 3;
         const bool yface = FULL || r >= 3;
@@ -816,14 +817,16 @@ k_march(KParams P, StepCtl* ctl, int slot, int my, int strips, int segs, const S
 // the tiles of every member.  The tile order is split into 8 contiguous
 // chunks, one per XCD, and the workgroups of XCD x (blockIdx mod 8 == x) walk
 // chunk x, so a member's neighbouring tiles share that XCD's L2.  With
-// hdr->prio set (a batch of small grids stepping beside another batch) the
+// hdr->prio == 1 (a batch of small grids stepping beside another batch) the
 // waves take the highest issue priority: their short serial chains issue
-// first and the other batch's waves fill the gaps.
+// first and the other batch's waves fill the gaps; that other batch runs with
+// hdr->prio == 2, its rows-left priority capped one level below.
 template <class S, int STAGE, bool FAST>
 __global__ void __launch_bounds__(MW* MWAVES) __attribute__((amdgpu_waves_per_eu(SFV_MARCH_MIN_WAVES)))
 k_march_batch(const BatchMember* __restrict__ mem, const BatchHeader* __restrict__ hdr, int lp) {
     const int members = hdr->members, tiles = hdr->tiles;
-    if (hdr->prio) __builtin_amdgcn_s_setprio(3);
+    if (hdr->prio == 1) __builtin_amdgcn_s_setprio(3);
+    const int cap = hdr->prio == 2 ? 2 : 3;          // a host engine's waves stay below its guest's
     const int x = blockIdx.x % N_XCD, k = blockIdx.x / N_XCD, per = gridDim.x / N_XCD;
     const int q = tiles / N_XCD, r = tiles % N_XCD;
     const int c0 = x * q + min(x, r), c1 = c0 + q + (x < r ? 1 : 0);
@@ -846,18 +849,18 @@ k_march_batch(const BatchMember* __restrict__ mem, const BatchHeader* __restrict
 #endif
         const S* U0 = (const S*)M.U[cur];
         // iterations of this wave's later tiles, counted at this member's height
-        const int after = hdr->prio ? -1 : ((c1 - 1 - t) / per) * (M.my + 4);
+        const int after = hdr->prio == 1 ? -1 : ((c1 - 1 - t) / per) * (M.my + 4);
         const int total = ntiles * (M.my + 4);
         if (STAGE == 1)
             if (march_edge(M.P, M.my, strip, seg))
                 march_tile<S, 1, FAST, true, true>(M.P, M.ctl, slot, M.my, strip, seg, U0, nullptr, (S*)M.RHS, M.B, par,
-                                                   0, after, total);
+                                                   0, after, total, cap);
             else
                 march_tile<S, 1, FAST, true>(M.P, M.ctl, slot, M.my, strip, seg, U0, nullptr, (S*)M.RHS, M.B, par, 0,
-                                             after, total);
+                                             after, total, cap);
         else
             march_tile<S, 2, FAST, true>(M.P, M.ctl, slot, M.my, strip, seg, U0, (const S*)M.RHS, (S*)M.U[cur ^ 1], M.B, 0,
-                                         1, after, total);
+                                         1, after, total, cap);
     }
 }
 
